@@ -1,0 +1,305 @@
+"""Oracle: CPU restatement of FlowPM's particle-mesh step (torch-CPU).
+
+TEST INFRASTRUCTURE ONLY.  Nothing in ``flowpm_amd`` may import this module;
+only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` use it, as the checker / CPU baseline.  It restates the
+reference's *operation graph* (not TensorFlow's rounding):
+
+* full-complex 3-D FFT with forward normalisation 1/norm and the real part
+  taken after the inverse (flowpm/utils.py:213-272);
+* an 8-corner CIC whose index tensors are materialised, a scatter-add paint
+  and a gather readout (flowpm/utils.py:32-91, 163-210);
+* Fourier kernels rebuilt in NumPy float64 on every call and cast to the
+  complex type (flowpm/kernels.py:10-102, flowpm/tfpm.py:86-91, 205-212).
+
+It runs in float64/complex128 (the error oracle) or float32/complex64 (the
+reference's precision).  Every op is differentiable in torch autograd with
+TF's conventions (d|x|/dx = sign(x), sign(0) = 0; floor has no gradient), so
+the float64 oracle is also the gradient oracle.
+
+Function map (reference file:line):
+  fftk / laplace_kernel / gradient_kernel / longrange_kernel  kernels.py:10-102
+  r2c3d / c2r3d                                 utils.py:213-241 / 244-272
+  cic_paint / cic_readout                       utils.py:32-91 / 163-210
+  whitenoise_to_linear                          scripts/check_grads.py:39-65
+                                                (= tfpm.linear_field :15-63
+                                                 with the noise injected)
+  lpt1 / lpt2_source / lpt_init                 tfpm.py:66-95 / 98-145 / 148-180
+  apply_longrange / kick / drift / force / nbody
+                                                tfpm.py:183-219 / 273-292 /
+                                                295-314 / 317-366 / 425-503
+Pins: tests/test_oracle_pm.py (numpy.fft, brute-force DFT, brute-force CIC
+loops, paint/readout adjoint identity, mass and momentum conservation,
+plane-wave closed forms for lpt1 and lpt2_source, linear growth of a single
+small mode through nbody, autograd vs central finite differences).
+"""
+import numpy as np
+import torch
+
+from .background import GrowthTable
+
+_CONNECTION = torch.tensor(
+    [[0, 0, 0], [1., 0, 0], [0., 1, 0], [0., 0, 1], [1., 1, 0], [1., 0, 1],
+     [0., 1, 1], [1., 1, 1]])  # utils.py:67-69
+
+
+def _cplx(dtype):
+  return torch.complex128 if dtype == torch.float64 else torch.complex64
+
+
+# ----------------------------------------------------------------- kernels --
+def fftk(shape, symmetric=True, dtype=np.float64):
+  """kernels.py:10-25."""
+  k = []
+  for d in range(len(shape)):
+    kd = np.fft.fftfreq(shape[d]) * 2 * np.pi
+    kdshape = np.ones(len(shape), dtype="int")
+    if symmetric and d == len(shape) - 1:
+      kd = kd[:shape[d] // 2 + 1]
+    kdshape[d] = len(kd)
+    k.append(kd.reshape(kdshape).astype(dtype))
+  return k
+
+
+def laplace_kernel(kvec):
+  """kernels.py:28-48.  The zero mode is set to kk=1 *before* the mask is
+  computed, so the returned DC value is 1 (not 0); every caller multiplies
+  by a gradient kernel that vanishes there."""
+  kk = sum(ki**2 for ki in kvec)
+  mask = (kk == 0).nonzero()
+  kk[mask] = 1
+  wts = 1. / kk
+  imask = (~(kk == 0)).astype(int)
+  return wts * imask
+
+
+def gradient_kernel(kvec, direction, order=1):
+  """kernels.py:51-78 (4th-order finite-difference gradient for order=1)."""
+  if order == 0:
+    wts = np.squeeze(1j * kvec[direction])
+    wts[len(wts) // 2] = 0
+    return wts.reshape(kvec[direction].shape)
+  w = kvec[direction]
+  return 1 / 6.0 * (8 * np.sin(w) - np.sin(2 * w)) * 1j
+
+
+def longrange_kernel(kvec, r_split):
+  """kernels.py:81-102."""
+  if r_split != 0:
+    kk = sum(ki**2 for ki in kvec)
+    return np.exp(-kk * r_split**2)
+  return 1.
+
+
+def _t(a, cdt):
+  return torch.as_tensor(np.asarray(a)).to(cdt)
+
+
+# ------------------------------------------------------------------- FFTs --
+def r2c3d(x, norm=None):
+  if norm is None:
+    norm = x.shape[-1] * x.shape[-2] * x.shape[-3]
+  c = torch.fft.fftn(x.to(_cplx(x.dtype)), dim=(-3, -2, -1))
+  return c * (1.0 / norm)
+
+
+def c2r3d(c, norm=None):
+  if norm is None:
+    norm = c.shape[-1] * c.shape[-2] * c.shape[-3]
+  rdt = torch.float64 if c.dtype == torch.complex128 else torch.float32
+  return torch.fft.ifftn(c, dim=(-3, -2, -1)).real.to(rdt) * norm
+
+
+# -------------------------------------------------------------------- CIC --
+def _cic_corners(part, nc):
+  """Returns integer neighbour coords (B,N,8,3) wrapped by floormod and the
+  8 CIC weights (B,N,8), utils.py:62-85."""
+  part = part.unsqueeze(2)
+  floor = torch.floor(part)
+  nb = floor + _CONNECTION.to(part.dtype)
+  kernel = 1. - torch.abs(part - nb)
+  kernel = kernel[..., 0] * kernel[..., 1] * kernel[..., 2]
+  nb = nb.to(torch.int64)
+  ncs = torch.tensor(nc, dtype=torch.int64)
+  nb = torch.remainder(nb, ncs)  # tf.math.mod == floormod
+  return nb, kernel
+
+
+def cic_paint(mesh, part, weight=None):
+  B, nx, ny, nz = mesh.shape
+  part = part.reshape(B, -1, 3)
+  nb, kernel = _cic_corners(part, [nx, ny, nz])
+  if weight is not None:
+    kernel = weight.reshape(B, -1, 1) * kernel
+  flat = ((torch.arange(B).view(B, 1, 1) * nx + nb[..., 0]) * ny +
+          nb[..., 1]) * nz + nb[..., 2]
+  upd = torch.zeros(B * nx * ny * nz, dtype=mesh.dtype)
+  upd = upd.index_add(0, flat.reshape(-1), kernel.reshape(-1).to(mesh.dtype))
+  return mesh + upd.view(B, nx, ny, nz)
+
+
+def cic_readout(mesh, part):
+  B, nx, ny, nz = mesh.shape
+  part = part.reshape(B, -1, 3)
+  nb, kernel = _cic_corners(part, [nx, ny, nz])
+  flat = (nb[..., 0] * ny + nb[..., 1]) * nz + nb[..., 2]
+  vals = torch.gather(mesh.reshape(B, -1), 1, flat.reshape(B, -1))
+  return (vals.view(kernel.shape) * kernel).sum(-1)
+
+
+# ------------------------------------------------------------ initial field --
+def kmesh(nc, boxsize):
+  """|k| in h/Mpc on the full FFT grid (tfpm.py:56)."""
+  kvec = fftk(nc, symmetric=False)
+  return sum((kk / boxsize[i] * nc[i])**2 for i, kk in enumerate(kvec))**0.5
+
+
+def whitenoise_to_linear(white, boxsize, pk):
+  """scripts/check_grads.py:39-65 and tfpm.py:15-63 with the white noise
+  injected: linear = c2r(r2c(white) * sqrt(pk(|k|)/V)).
+
+  ``white`` is the real-space N(0, sqrt(M)) field (utils.py:346-350);
+  ``pk`` maps a float64 numpy |k| array to P(k)."""
+  nc = list(white.shape[1:])
+  if np.isscalar(boxsize):
+    boxsize = [boxsize] * 3
+  M = nc[0] * nc[1] * nc[2]
+  cdt = _cplx(white.dtype)
+  whitec = r2c3d(white, norm=M)
+  amp = (pk(kmesh(nc, boxsize)) / (boxsize[0] * boxsize[1] * boxsize[2]))**0.5
+  return c2r3d(whitec * _t(amp, cdt), norm=M)
+
+
+# -------------------------------------------------------------------- LPT --
+def lpt1(dlin_k, pos):
+  nc = list(dlin_k.shape[1:])
+  cdt = dlin_k.dtype
+  kvec = fftk(nc, symmetric=False)
+  lap = _t(laplace_kernel(kvec), cdt)
+  out = []
+  for d in range(3):
+    kweight = _t(gradient_kernel(kvec, d), cdt) * lap
+    disp = c2r3d(dlin_k * kweight, norm=nc[0] * nc[1] * nc[2])
+    out.append(cic_readout(disp, pos))
+  return torch.stack(out, dim=2)
+
+
+def lpt2_source(dlin_k):
+  nc = list(dlin_k.shape[1:])
+  cdt = dlin_k.dtype
+  M = nc[0] * nc[1] * nc[2]
+  kvec = fftk(nc, symmetric=False)
+  D1 = [1, 2, 0]
+  D2 = [2, 0, 1]
+  lap = _t(laplace_kernel(kvec), cdt)
+  phi_ii = []
+  for d in range(3):
+    grad = _t(gradient_kernel(kvec, d), cdt)
+    phi_ii.append(c2r3d(dlin_k * (lap * grad * grad), norm=M))
+  source = 0
+  for d in range(3):
+    source = source + phi_ii[D1[d]] * phi_ii[D2[d]]
+  for d in range(3):
+    gi = _t(gradient_kernel(kvec, D1[d]), cdt)
+    gj = _t(gradient_kernel(kvec, D2[d]), cdt)
+    phi = c2r3d(dlin_k * (lap * gi * gj), norm=M)
+    source = source - phi * phi
+  return r2c3d(source * (3.0 / 7.), norm=M)
+
+
+def lagrangian_grid(nc, batch, dtype):
+  """tfpm.py:161-164: np.indices, C-order, float."""
+  q = np.indices(nc).reshape(3, -1).T.astype(np.float32)
+  q = np.repeat(q.reshape((1, -1, 3)), batch, axis=0)
+  return torch.as_tensor(q).to(dtype)
+
+
+def lpt_init(growth, linear, a, order=2):
+  """tfpm.py:148-180; ``growth`` is a GrowthTable (host scalars)."""
+  assert order in (1, 2)
+  nc = list(linear.shape[1:])
+  B = linear.shape[0]
+  dt = linear.dtype
+  M = nc[0] * nc[1] * nc[2]
+  Q = lagrangian_grid(nc, B, dt)
+  g = growth
+  lineark = r2c3d(linear, norm=M)
+  DX = float(g.D1(a)) * lpt1(lineark, Q)
+  P = float(a**2 * g.f1(a) * g.E(a)) * DX
+  F = float(a**2 * g.E(a) * g.gf(a) / g.D1(a)) * DX
+  if order == 2:
+    DX2 = float(g.D2(a)) * lpt1(lpt2_source(lineark), Q)
+    P = P + float(a**2 * g.f2(a) * g.E(a)) * DX2
+    F = F + float(a**2 * g.E(a) * g.gf2(a) / g.D2(a)) * DX2
+    DX = DX + DX2
+  return torch.stack((DX + Q, P, F), dim=0)
+
+
+# ------------------------------------------------------------------ nbody --
+def apply_longrange(x, delta_k, split=0, factor=1.):
+  nc = list(delta_k.shape[1:])
+  cdt = delta_k.dtype
+  kvec = fftk(nc, symmetric=False)
+  lap = _t(laplace_kernel(kvec), cdt)
+  kweight = lap * longrange_kernel(kvec, split)
+  pot_k = delta_k * kweight
+  f = []
+  for d in range(3):
+    forced = c2r3d(pot_k * _t(gradient_kernel(kvec, d), cdt),
+                   norm=nc[0] * nc[1] * nc[2])
+    f.append(cic_readout(forced, x))
+  return torch.stack(f, dim=2) * factor
+
+
+def kick(growth, state, ai, ac, af):
+  g = growth
+  fac = float(1 / (ac**2 * g.E(ac)) * (g.Gf(af) - g.Gf(ai)) / g.gf(ac))
+  return torch.stack((state[0], state[1] + fac * state[2], state[2]))
+
+
+def drift(growth, state, ai, ac, af):
+  g = growth
+  fac = float(1. / (ac**3 * g.E(ac)) * (g.D1(af) - g.D1(ai)) / g.D1f(ac))
+  return torch.stack((state[0] + fac * state[1], state[1], state[2]))
+
+
+def force(growth, state, nc, pm_nc_factor=1):
+  B = state.shape[1]
+  ncf = [n * pm_nc_factor for n in nc]
+  dt = state.dtype
+  rho = torch.zeros([B] + ncf, dtype=dt)
+  nbar = nc[0] * nc[1] * nc[2] / (ncf[0] * ncf[1] * ncf[2])
+  xs = state[0] * pm_nc_factor
+  rho = cic_paint(rho, xs)
+  rho = rho * (1. / nbar)
+  delta_k = r2c3d(rho, norm=ncf[0] * ncf[1] * ncf[2])
+  fac = 1.5 * growth.c.Omega_m
+  if dt == torch.float32:
+    fac = float(np.float32(fac))
+  upd = apply_longrange(xs, delta_k, split=0, factor=fac) / pm_nc_factor
+  return torch.stack((state[0], state[1], upd))
+
+
+def nbody(growth, state, stages, nc, pm_nc_factor=1,
+          return_intermediate_states=False):
+  if isinstance(nc, int):
+    nc = [nc, nc, nc]
+  if len(stages) == 0:
+    return state
+  ai = stages[0]
+  state = force(growth, state, nc, pm_nc_factor)
+  x, p, f = ai, ai, ai
+  inter = []
+  for i in range(len(stages) - 1):
+    a0, a1 = stages[i], stages[i + 1]
+    ah = (a0 * a1)**0.5
+    state = kick(growth, state, p, f, ah)
+    p = ah
+    state = drift(growth, state, x, p, a1)
+    x = a1
+    state = force(growth, state, nc, pm_nc_factor)
+    f = a1
+    state = kick(growth, state, p, f, a1)
+    p = a1
+    inter.append((a1, state))
+  return inter if return_intermediate_states else state
