@@ -1,0 +1,384 @@
+// Cloud-in-cell paint / readout and their adjoints, for gfx950.
+//
+// Semantics follow flowpm/utils.py:32-91 (cic_paint) and :163-210
+// (cic_readout): neighbour = floor(x) + {0,1}^3 in the order
+// (000,100,010,001,110,101,011,111); weight = prod_d (1 - |x_d - n_d|)
+// evaluated as (wx*wy)*wz; periodic wrap by int floormod; positions are
+// never wrapped themselves.  Non-finite positions are skipped (the reference
+// raises on CPU and silently drops out-of-range indices on GPU).
+//
+// Adjoints use TF's conventions: d|u|/du = sign(u) with sign(0) = 0, floor
+// has no gradient.  For the lower corner d/dx (1 - (x - fl)) = -1 when
+// x > fl and 0 when x == fl; for the upper corner the derivative is +1.
+#include "pmx_internal.h"
+
+namespace pmx {
+
+struct Cic1 {
+  int i0, i1;    // wrapped indices of the two neighbours
+  float w0, w1;  // weights
+  float d0, d1;  // derivatives of the weights w.r.t. x
+};
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ Cic1 cic1(float x, int n) {
+  Cic1 c;
+  const float fl = floorf(x);
+  const int fi = (int)fl;
+  c.w0 = 1.0f - fabsf(x - fl);
+  c.w1 = 1.0f - fabsf(x - (fl + 1.0f));
+  c.d0 = (x - fl > 0.0f) ? -1.0f : 0.0f;
+  c.d1 = 1.0f;
+  c.i0 = wrapi(fi, n);
+  c.i1 = wrapi(fi + 1, n);
+  return c;
+}
+
+__device__ __forceinline__ bool pos_ok(float x, float y, float z) {
+  // beyond 2^30 the int conversion of floor would overflow
+  return fabsf(x) < 1.0e9f && fabsf(y) < 1.0e9f && fabsf(z) < 1.0e9f;
+}
+
+#define PMX_CORNERS(BODY)                                                 \
+  {                                                                       \
+    constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};                       \
+    constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};                       \
+    constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};                       \
+    _Pragma("unroll") for (int cc = 0; cc < 8; ++cc) {                    \
+      const int ix = OX[cc] ? cx.i1 : cx.i0;                              \
+      const int iy = OY[cc] ? cy.i1 : cy.i0;                              \
+      const int iz = OZ[cc] ? cz.i1 : cz.i0;                              \
+      const float wx = OX[cc] ? cx.w1 : cx.w0;                            \
+      const float wy = OY[cc] ? cy.w1 : cy.w0;                            \
+      const float wz = OZ[cc] ? cz.w1 : cz.w0;                            \
+      const float dx = OX[cc] ? cx.d1 : cx.d0;                            \
+      const float dy = OY[cc] ? cy.d1 : cy.d0;                            \
+      const float dz = OZ[cc] ? cz.d1 : cz.d0;                            \
+      const long long cell =                                              \
+          (((long long)b * d.nx + ix) * d.ny + iy) * d.nz + iz;           \
+      (void)dx; (void)dy; (void)dz;                                       \
+      BODY                                                                \
+    }                                                                     \
+  }
+
+__global__ void k_fill(float* __restrict__ p, long long n, float v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Plain scatter: one thread per particle, 8 hardware float atomics.
+__global__ __launch_bounds__(256) void k_paint(
+    float* __restrict__ mesh, const float* __restrict__ pos,
+    const float* __restrict__ w, float wscale, MeshDims d, float ps) {
+  const long long tot = (long long)d.B * d.N;
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= tot) return;
+  const int b = (int)(p / d.N);
+  const float x = pos[3 * p] * ps, y = pos[3 * p + 1] * ps,
+              z = pos[3 * p + 2] * ps;
+  if (!pos_ok(x, y, z)) return;
+  const float wt = w ? w[p] * wscale : wscale;
+  const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+  PMX_CORNERS({
+    const float k = (wx * wy) * wz;
+    atomicAdd(mesh + cell, wt * k);
+  })
+}
+
+__global__ __launch_bounds__(256) void k_readout(
+    float* __restrict__ out, const float* __restrict__ mesh,
+    const float* __restrict__ pos, MeshDims d, float ps) {
+  const long long tot = (long long)d.B * d.N;
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= tot) return;
+  const int b = (int)(p / d.N);
+  const float x = pos[3 * p] * ps, y = pos[3 * p + 1] * ps,
+              z = pos[3 * p + 2] * ps;
+  if (!pos_ok(x, y, z)) {
+    out[p] = 0.0f;
+    return;
+  }
+  const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+  float acc = 0.0f;
+  PMX_CORNERS({ acc += mesh[cell] * ((wx * wy) * wz); })
+  out[p] = acc;
+}
+
+// d(pos) = gscale * w * sum_c g[c] dK_c/dx ; d(w) = sum_c g[c] K_c
+__global__ __launch_bounds__(256) void k_paint_adj(
+    float* __restrict__ dpos, float* __restrict__ dw,
+    const float* __restrict__ g, const float* __restrict__ pos,
+    const float* __restrict__ w, MeshDims d, float ps, float gscale,
+    int accumulate) {
+  const long long tot = (long long)d.B * d.N;
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= tot) return;
+  const int b = (int)(p / d.N);
+  const float x = pos[3 * p] * ps, y = pos[3 * p + 1] * ps,
+              z = pos[3 * p + 2] * ps;
+  float gx = 0.f, gy = 0.f, gz = 0.f, gw = 0.f;
+  if (pos_ok(x, y, z)) {
+    const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+    PMX_CORNERS({
+      const float v = g[cell];
+      gw += v * ((wx * wy) * wz);
+      gx += v * ((dx * wy) * wz);
+      gy += v * ((wx * dy) * wz);
+      gz += v * ((wx * wy) * dz);
+    })
+  }
+  const float wt = (w ? w[p] : 1.0f) * gscale * ps;
+  if (dpos) {
+    if (accumulate) {
+      dpos[3 * p] += wt * gx;
+      dpos[3 * p + 1] += wt * gy;
+      dpos[3 * p + 2] += wt * gz;
+    } else {
+      dpos[3 * p] = wt * gx;
+      dpos[3 * p + 1] = wt * gy;
+      dpos[3 * p + 2] = wt * gz;
+    }
+  }
+  if (dw) dw[p] = accumulate ? dw[p] + gw * gscale : gw * gscale;
+}
+
+// readout adjoint: gmesh[c] += gout_p K_c ; dpos = gout_p sum_c m[c] dK_c/dx
+__global__ __launch_bounds__(256) void k_readout_adj(
+    float* __restrict__ gmesh, float* __restrict__ dpos,
+    const float* __restrict__ gout, const float* __restrict__ mesh,
+    const float* __restrict__ pos, MeshDims d, float ps, int accumulate) {
+  const long long tot = (long long)d.B * d.N;
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= tot) return;
+  const int b = (int)(p / d.N);
+  const float x = pos[3 * p] * ps, y = pos[3 * p + 1] * ps,
+              z = pos[3 * p + 2] * ps;
+  const float go = gout[p];
+  float gx = 0.f, gy = 0.f, gz = 0.f;
+  if (pos_ok(x, y, z)) {
+    const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+    PMX_CORNERS({
+      if (gmesh) atomicAdd(gmesh + cell, go * ((wx * wy) * wz));
+      if (dpos) {
+        const float v = mesh[cell];
+        gx += v * ((dx * wy) * wz);
+        gy += v * ((wx * dy) * wz);
+        gz += v * ((wx * wy) * dz);
+      }
+    })
+  }
+  if (dpos) {
+    const float s = go * ps;
+    if (accumulate) {
+      dpos[3 * p] += s * gx;
+      dpos[3 * p + 1] += s * gy;
+      dpos[3 * p + 2] += s * gz;
+    } else {
+      dpos[3 * p] = s * gx;
+      dpos[3 * p + 1] = s * gy;
+      dpos[3 * p + 2] = s * gz;
+    }
+  }
+}
+
+// Fused force readout + kick (+ next kick + drift), one thread per particle.
+// F_d = (readout(mesh_d, f X) * fac) / f          tfpm.py:352-356
+// P  += k2 F                                      tfpm.py:284-291
+// P  += k1 F ; X += d P                           tfpm.py:306-313
+__global__ __launch_bounds__(256) void k_readout_kick_drift(
+    float* __restrict__ X, float* __restrict__ P, float* __restrict__ F,
+    const float* __restrict__ fxm, const float* __restrict__ fym,
+    const float* __restrict__ fzm, MeshDims d, KickDrift c) {
+  const long long tot = (long long)d.B * d.N;
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= tot) return;
+  const int b = (int)(p / d.N);
+  float px = X[3 * p], py = X[3 * p + 1], pz = X[3 * p + 2];
+  const float x = px * c.f, y = py * c.f, z = pz * c.f;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  if (pos_ok(x, y, z)) {
+    const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+    PMX_CORNERS({
+      const float k = (wx * wy) * wz;
+      ax += fxm[cell] * k;
+      ay += fym[cell] * k;
+      az += fzm[cell] * k;
+    })
+  }
+  const float Fx = (ax * c.fac) / c.f, Fy = (ay * c.fac) / c.f,
+              Fz = (az * c.fac) / c.f;
+  F[3 * p] = Fx;
+  F[3 * p + 1] = Fy;
+  F[3 * p + 2] = Fz;
+  if (c.do_k2 || c.do_k1d) {
+    float vx = P[3 * p], vy = P[3 * p + 1], vz = P[3 * p + 2];
+    if (c.do_k2) {
+      vx = vx + c.k2 * Fx;
+      vy = vy + c.k2 * Fy;
+      vz = vz + c.k2 * Fz;
+    }
+    if (c.do_k1d) {
+      vx = vx + c.k1 * Fx;
+      vy = vy + c.k1 * Fy;
+      vz = vz + c.k1 * Fz;
+      px = px + c.d * vx;
+      py = py + c.d * vy;
+      pz = pz + c.d * vz;
+      X[3 * p] = px;
+      X[3 * p + 1] = py;
+      X[3 * p + 2] = pz;
+    }
+    P[3 * p] = vx;
+    P[3 * p + 1] = vy;
+    P[3 * p + 2] = vz;
+  }
+}
+
+// Kick / drift from the stored force (no readout).
+__global__ __launch_bounds__(256) void k_kick_drift_only(
+    float* __restrict__ X, float* __restrict__ P, const float* __restrict__ F,
+    long long n3, KickDrift c) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  float v = P[i];
+  if (c.do_k2) v = v + c.k2 * F[i];
+  if (c.do_k1d) {
+    v = v + c.k1 * F[i];
+    X[i] = X[i] + c.d * v;
+  }
+  P[i] = v;
+}
+
+// Force VJP, particle side.  With F_d = c * readout(mesh_d, f X):
+//   gmesh_d[cell] += c gF_d K_cell           (mesh cotangents, 24 atomics)
+//   dX += c f sum_d gF_d sum_cell mesh_d[cell] dK_cell/dx
+__global__ __launch_bounds__(256) void k_force_vjp_particles(
+    float* __restrict__ gmx, float* __restrict__ gmy, float* __restrict__ gmz,
+    float* __restrict__ dX, const float* __restrict__ gF,
+    const float* __restrict__ X, const float* __restrict__ fxm,
+    const float* __restrict__ fym, const float* __restrict__ fzm, MeshDims d,
+    float f, float cf) {
+  const long long tot = (long long)d.B * d.N;
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= tot) return;
+  const int b = (int)(p / d.N);
+  const float x = X[3 * p] * f, y = X[3 * p + 1] * f, z = X[3 * p + 2] * f;
+  const float gfx = gF[3 * p] * cf, gfy = gF[3 * p + 1] * cf,
+              gfz = gF[3 * p + 2] * cf;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  if (pos_ok(x, y, z)) {
+    const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+    PMX_CORNERS({
+      const float k = (wx * wy) * wz;
+      atomicAdd(gmx + cell, gfx * k);
+      atomicAdd(gmy + cell, gfy * k);
+      atomicAdd(gmz + cell, gfz * k);
+      const float v = gfx * fxm[cell] + gfy * fym[cell] + gfz * fzm[cell];
+      sx += v * ((dx * wy) * wz);
+      sy += v * ((wx * dy) * wz);
+      sz += v * ((wx * wy) * dz);
+    })
+  }
+  dX[3 * p] += f * sx;
+  dX[3 * p + 1] += f * sy;
+  dX[3 * p + 2] += f * sz;
+}
+
+// ------------------------------------------------------------ launchers --
+static inline unsigned nblk(long long n, int t) {
+  return (unsigned)((n + t - 1) / t);
+}
+
+int launch_fill(float* p, long long n, float v, hipStream_t s) {
+  if (n <= 0) return kOk;
+  const unsigned g = (unsigned)std::min<long long>(nblk(n, 256), 8192);
+  hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, p, n, v);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_paint(float* mesh, const float* pos, const float* w, float wscale,
+                 const MeshDims& d, float ps, int mode, hipStream_t s) {
+  (void)mode;
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  hipLaunchKernelGGL(k_paint, dim3(nblk(tot, 256)), dim3(256), 0, s, mesh,
+                     pos, w, wscale, d, ps);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_readout(float* out, const float* mesh, const float* pos,
+                   const MeshDims& d, float ps, hipStream_t s) {
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  hipLaunchKernelGGL(k_readout, dim3(nblk(tot, 256)), dim3(256), 0, s, out,
+                     mesh, pos, d, ps);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_paint_adj(float* dpos, float* dw, const float* g, const float* pos,
+                     const float* w, const MeshDims& d, float ps, float gscale,
+                     int accumulate, hipStream_t s) {
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  hipLaunchKernelGGL(k_paint_adj, dim3(nblk(tot, 256)), dim3(256), 0, s, dpos,
+                     dw, g, pos, w, d, ps, gscale, accumulate);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_readout_adj(float* gmesh, float* dpos, const float* gout,
+                       const float* mesh, const float* pos, const MeshDims& d,
+                       float ps, int accumulate, hipStream_t s) {
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  hipLaunchKernelGGL(k_readout_adj, dim3(nblk(tot, 256)), dim3(256), 0, s,
+                     gmesh, dpos, gout, mesh, pos, d, ps, accumulate);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_readout_kick_drift(float* X, float* P, float* F, const float* fx,
+                              const float* fy, const float* fz,
+                              const MeshDims& d, const KickDrift& c,
+                              hipStream_t s) {
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  hipLaunchKernelGGL(k_readout_kick_drift, dim3(nblk(tot, 256)), dim3(256), 0,
+                     s, X, P, F, fx, fy, fz, d, c);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_kick_drift_only(float* X, float* P, const float* F,
+                           const MeshDims& d, const KickDrift& c,
+                           hipStream_t s) {
+  const long long n3 = (long long)d.B * d.N * 3;
+  if (n3 == 0) return kOk;
+  hipLaunchKernelGGL(k_kick_drift_only, dim3(nblk(n3, 256)), dim3(256), 0, s,
+                     X, P, F, n3, c);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz, float* dX,
+                               const float* gF, const float* X,
+                               const float* fx, const float* fy,
+                               const float* fz, const MeshDims& d, float f,
+                               float c, hipStream_t s) {
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  hipLaunchKernelGGL(k_force_vjp_particles, dim3(nblk(tot, 256)), dim3(256),
+                     0, s, gmx, gmy, gmz, dX, gF, X, fx, fy, fz, d, f, c);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace pmx

@@ -1,0 +1,724 @@
+// Batched 3-D real<->complex FFT for the PM force, written for gfx950.
+//
+// Replaces tf.signal.fft3d / ifft3d as used by flowpm/utils.py:213-272
+// (r2c3d: fft of the real field cast to complex, times 1/norm; c2r3d: real
+// part of ifft times norm, i.e. an unnormalised inverse).  Only the half
+// spectrum kz = 0..nz/2 is kept (the reference's full spectrum of a real
+// field is Hermitian; every multiplied kernel preserves that symmetry).
+//
+// Structure: three line passes, each a workgroup-per-tile kernel that stages
+// a tile in LDS, runs a Stockham autosort FFT there (radix 8/4/2/3 stages,
+// register-staged, in place), and writes the tile back.
+//   Z pass  contiguous rows; two real rows are packed into one complex line.
+//   Y pass  column tiles: all ky for 16 consecutive kz (128-byte segments).
+//   X pass  column tiles: all kx for 16 consecutive kz.  The Fourier-space
+//           Poisson x gradient multiply (kernels.py:28-102, tfpm.py:183-219)
+//           is fused here between the forward and the inverse x transform,
+//           so the force pipeline never materialises the 3-D spectrum.
+#include "pmx_internal.h"
+
+namespace pmx {
+
+// --------------------------------------------------------------- complex --
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) {
+  return make_float2(a.x * s, a.y * s);
+}
+// multiply by (S * i), S = +1 or -1
+template <int S>
+__device__ __forceinline__ float2 mul_i(float2 a) {
+  return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+// multiply by i^E
+__device__ __forceinline__ float2 rot_i(float2 v, int E) {
+  switch (E & 3) {
+    case 1: return make_float2(-v.y, v.x);
+    case 2: return make_float2(-v.x, -v.y);
+    case 3: return make_float2(v.y, -v.x);
+    default: return v;
+  }
+}
+
+// ------------------------------------------------------- radix codelets --
+// D = -1: forward exp(-2 pi i jk/R); D = +1: inverse (unnormalised).
+template <int R, int D>
+struct Dft;
+
+template <int D>
+struct Dft<2, D> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <int D>
+struct Dft<4, D> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    float2 t2 = cadd(a[1], a[3]), t3 = mul_i<D>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <int D>
+struct Dft<8, D> {
+  static __device__ __forceinline__ void run(float2* a) {
+    constexpr float c = 0.70710678118654752440f;
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<4, D>::run(e);
+    Dft<4, D>::run(o);
+    // o[m] *= W8^m, W8 = exp(D 2 pi i / 8) = c (1 + D i)
+    o[1] = make_float2(c * (o[1].x - D * o[1].y), c * (o[1].y + D * o[1].x));
+    o[2] = mul_i<D>(o[2]);
+    o[3] = make_float2(c * (-o[3].x - D * o[3].y), c * (-o[3].y + D * o[3].x));
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      a[m] = cadd(e[m], o[m]);
+      a[m + 4] = csub(e[m], o[m]);
+    }
+  }
+};
+
+template <int D>
+struct Dft<3, D> {
+  static __device__ __forceinline__ void run(float2* a) {
+    constexpr float h = 0.86602540378443864676f;  // sqrt(3)/2
+    float2 s = cadd(a[1], a[2]), d = csub(a[1], a[2]);
+    float2 m = make_float2(a[0].x - 0.5f * s.x, a[0].y - 0.5f * s.y);
+    float2 r = cscale(mul_i<D>(d), h);
+    a[0] = cadd(a[0], s);
+    a[1] = cadd(m, r);
+    a[2] = csub(m, r);
+  }
+};
+
+// radix plan: 8 first, then 4, 2, 3
+__host__ __device__ constexpr int pick_radix(int n) {
+  return (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2
+                                                : (n % 3 == 0) ? 3 : n;
+}
+
+// LDS tile: element (line index i, column c) at s[i * P + c], P = TW + 1.
+template <int TW>
+struct Pitch {
+  static constexpr int P = TW + 1;
+};
+
+// One Stockham stage (Bainville's formulation): butterfly j reads
+// in[j + r N/R], twiddles by W_{NS R}^{(j mod NS) r}, and writes
+// out[(j / NS) NS R + (j mod NS) + r NS].  In place: all reads are staged in
+// registers before the barrier, all writes follow it.
+template <int N, int R, int NS, int TW, int T, int D>
+__device__ __forceinline__ void fft_stage(float2* __restrict__ s,
+                                          const float2* __restrict__ tw) {
+  constexpr int P = Pitch<TW>::P;
+  constexpr int NB = (N / R) * TW;
+  constexpr int BPT = (NB + T - 1) / T;
+  float2 a[BPT][R];
+#pragma unroll
+  for (int u = 0; u < BPT; ++u) {
+    const int b = threadIdx.x + u * T;
+    if (NB % T == 0 || b < NB) {
+      const int j = b / TW, c = b % TW;
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[u][r] = s[(j + r * (N / R)) * P + c];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < BPT; ++u) {
+    const int b = threadIdx.x + u * T;
+    if (NB % T == 0 || b < NB) {
+      const int j = b / TW, c = b % TW;
+      const int k = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[k * r * (N / (NS * R))];
+          if (D > 0) w.y = -w.y;
+          a[u][r] = cmul(a[u][r], w);
+        }
+      }
+      Dft<R, D>::run(a[u]);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[(base + r * NS) * P + c] = a[u][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int NS, int TW, int T, int D>
+struct FftStages {
+  static __device__ __forceinline__ void run(float2* s, const float2* tw) {
+    if constexpr (NS < N) {
+      constexpr int R = pick_radix(N / NS);
+      static_assert(R == 2 || R == 3 || R == 4 || R == 8, "unsupported radix");
+      fft_stage<N, R, NS, TW, T, D>(s, tw);
+      FftStages<N, NS * R, TW, T, D>::run(s, tw);
+    }
+  }
+};
+
+template <int N, int TW, int T, int D>
+__device__ __forceinline__ void fft_tile(float2* s, const float2* tw) {
+  FftStages<N, 1, TW, T, D>::run(s, tw);
+}
+
+template <int N, int T>
+__device__ __forceinline__ void load_twiddles(float2* tw_s,
+                                              const float2* __restrict__ tw) {
+  for (int t = threadIdx.x; t < N; t += T) tw_s[t] = tw[t];
+}
+
+// Fourier multiplier for element (kx, ky, kz).
+__device__ __forceinline__ float2 apply_term(float2 v, const Term& t, int kx,
+                                             int ky, int kz,
+                                             const PlanDev& p) {
+  float c = t.scale;
+  if (t.lap) {
+    double kk = p.ax[0].k2[kx] + p.ax[1].k2[ky] + p.ax[2].k2[kz];
+    c *= (kk == 0.0) ? 1.0f : (float)(1.0 / kk);
+  }
+  const int kidx[3] = {kx, ky, kz};
+  int E = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int e = t.e[d];
+    if (e) {
+      const float gd = p.ax[d].g[kidx[d]];
+      c *= (e == 2) ? gd * gd : gd;
+      E += e;
+    }
+  }
+  return rot_i(cscale(v, c), E);
+}
+
+// ------------------------------------------------------------ Z pass -----
+// Rows of the real mesh are contiguous; a workgroup takes 2*TW consecutive
+// rows (one contiguous block), packs row pairs as re/im of TW complex lines,
+// transforms, and splits X1(k) = (Z(k) + Z*(N-k))/2, X2(k) = (Z(k) -
+// Z*(N-k))/(2i) for k = 0..N/2.
+template <int N, int TW, int T>
+__global__ __launch_bounds__(T) void k_zpass_r2c(ZJobs jobs, PlanDev p) {
+  constexpr int P = Pitch<TW>::P;
+  constexpr int H = N / 2;
+  __shared__ float2 s[N * P];
+  __shared__ float2 tw[N];
+  const ZJob J = blockIdx.y == 0 ? jobs.j[0]
+                 : (blockIdx.y == 1 ? jobs.j[1] : jobs.j[2]);
+  load_twiddles<N, T>(tw, p.ax[2].tw);
+  const long long r0 = (long long)blockIdx.x * (2 * TW);
+  const int nr = (int)min((long long)(2 * TW), p.rows - r0);
+  const float* __restrict__ src = J.rin + r0 * N;
+  float* sf = reinterpret_cast<float*>(s);
+  for (int q = threadIdx.x; q < 2 * TW * N; q += T) {
+    const int row = q / N, z = q % N;
+    const float v = row < nr ? src[q] : 0.0f;
+    sf[2 * (z * P + (row >> 1)) + (row & 1)] = v;
+  }
+  __syncthreads();
+  fft_tile<N, TW, T, -1>(s, tw);
+  float2* __restrict__ dst = J.cout + r0 * H;
+  const float sc = J.scale;
+  for (int q = threadIdx.x; q < 2 * TW * H; q += T) {
+    const int row = q / H, k = q % H;
+    if (row < nr) {
+      const float2 zk = s[k * P + (row >> 1)];
+      const float2 zn = s[((N - k) % N) * P + (row >> 1)];
+      float2 v;
+      if (row & 1)
+        v = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+      else
+        v = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+      dst[q] = cscale(v, sc);
+    }
+  }
+  for (int row = threadIdx.x; row < nr; row += T) {
+    const float2 zk = s[H * P + (row >> 1)];
+    float2 v = (row & 1) ? make_float2(zk.y, 0.0f) : make_float2(zk.x, 0.0f);
+    J.cout[p.nyq0 + r0 + row] = cscale(v, sc);
+  }
+}
+
+// Inverse: two half-spectrum rows -> Hermitian-extended Z = X1 + i X2 ->
+// inverse FFT -> row pair (re, im).  The imaginary parts of the kz=0 and
+// kz=N/2 entries are dropped (real part of the inverse, utils.py:270-271).
+template <int N, int TW, int T>
+__global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
+  constexpr int P = Pitch<TW>::P;
+  constexpr int H = N / 2;
+  constexpr int HP = H + 1;
+  __shared__ float2 s[N * P];
+  __shared__ float2 h[2 * TW * HP];
+  __shared__ float2 tw[N];
+  const ZJob J = blockIdx.y == 0 ? jobs.j[0]
+                 : (blockIdx.y == 1 ? jobs.j[1] : jobs.j[2]);
+  load_twiddles<N, T>(tw, p.ax[2].tw);
+  const long long r0 = (long long)blockIdx.x * (2 * TW);
+  const int nr = (int)min((long long)(2 * TW), p.rows - r0);
+  const float2* __restrict__ src = J.cin + r0 * H;
+  const float* __restrict__ gz = p.ax[2].g;
+  const int ez = J.ez;
+  for (int q = threadIdx.x; q < 2 * TW * H; q += T) {
+    const int row = q / H, k = q % H;
+    float2 v = row < nr ? src[q] : make_float2(0.f, 0.f);
+    if (ez) {
+      const float g = gz[k];
+      v = rot_i(cscale(v, ez == 2 ? g * g : g), ez);
+    }
+    h[row * HP + k] = v;
+  }
+  for (int row = threadIdx.x; row < 2 * TW; row += T) {
+    float2 v = row < nr ? J.cin[p.nyq0 + r0 + row] : make_float2(0.f, 0.f);
+    if (ez) {
+      const float g = gz[H];
+      v = rot_i(cscale(v, ez == 2 ? g * g : g), ez);
+    }
+    h[row * HP + H] = v;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < TW * N; q += T) {
+    const int k = q / TW, l = q % TW;
+    float2 x1, x2;
+    if (k <= H) {
+      x1 = h[(2 * l) * HP + k];
+      x2 = h[(2 * l + 1) * HP + k];
+      if (k == 0 || k == H) {
+        x1.y = 0.0f;
+        x2.y = 0.0f;
+      }
+    } else {
+      x1 = h[(2 * l) * HP + (N - k)];
+      x2 = h[(2 * l + 1) * HP + (N - k)];
+      x1.y = -x1.y;
+      x2.y = -x2.y;
+    }
+    s[k * P + l] = make_float2(x1.x - x2.y, x1.y + x2.x);
+  }
+  __syncthreads();
+  fft_tile<N, TW, T, +1>(s, tw);
+  float* __restrict__ dst = J.rout + r0 * N;
+  const float sc = J.scale;
+  for (int q = threadIdx.x; q < 2 * TW * N; q += T) {
+    const int row = q / N, z = q % N;
+    if (row < nr) {
+      const float2 v = s[z * P + (row >> 1)];
+      const float r = sc * ((row & 1) ? v.y : v.x);
+      dst[q] = J.accum ? dst[q] + r : r;
+    }
+  }
+}
+
+// ------------------------------------------------------- column tiles ----
+struct Tile {
+  long long base;
+  long long ls, cs;  // line stride, column stride (one of them is 1)
+  int ncol;
+  int nyq;           // 0: columns are kz; 1: Nyquist-plane tile
+  int fix;           // main: fixed x (Y pass) or y (X pass); nyq: first col
+};
+
+// AX = 1 (y lines) or 0 (x lines)
+template <int AX, int TW>
+__device__ __forceinline__ Tile make_tile(const PlanDev& p, int t) {
+  Tile tl;
+  const int H = p.H;
+  const int nkt = (H + TW - 1) / TW;
+  if (AX == 1) {
+    const int tiles_main = p.B * p.nx * nkt;
+    if (t < tiles_main) {
+      const int kt = t % nkt, bx = t / nkt;
+      tl.base = (long long)bx * p.ny * H + kt * TW;
+      tl.ls = H;
+      tl.cs = 1;
+      tl.ncol = min(TW, H - kt * TW);
+      tl.nyq = 0;
+      tl.fix = bx % p.nx;
+    } else {
+      const int nxt = (p.nx + TW - 1) / TW;
+      const int u = t - tiles_main;
+      const int xt = u % nxt, b = u / nxt;
+      tl.base = p.nyq0 + ((long long)b * p.nx + xt * TW) * p.ny;
+      tl.ls = 1;
+      tl.cs = p.ny;
+      tl.ncol = min(TW, p.nx - xt * TW);
+      tl.nyq = 1;
+      tl.fix = xt * TW;
+    }
+  } else {
+    const int tiles_main = p.B * p.ny * nkt;
+    if (t < tiles_main) {
+      const int kt = t % nkt, by = t / nkt;
+      const int b = by / p.ny, y = by % p.ny;
+      tl.base = ((long long)b * p.nx * p.ny + y) * H + kt * TW;
+      tl.ls = (long long)p.ny * H;
+      tl.cs = 1;
+      tl.ncol = min(TW, H - kt * TW);
+      tl.nyq = 0;
+      tl.fix = y;
+    } else {
+      const int nyt = (p.ny + TW - 1) / TW;
+      const int u = t - tiles_main;
+      const int yt = u % nyt, b = u / nyt;
+      tl.base = p.nyq0 + (long long)b * p.nx * p.ny + yt * TW;
+      tl.ls = p.ny;
+      tl.cs = 1;
+      tl.ncol = min(TW, p.ny - yt * TW);
+      tl.nyq = 1;
+      tl.fix = yt * TW;
+    }
+  }
+  return tl;
+}
+
+template <int AX, int TW>
+__device__ __forceinline__ int tile_count(const PlanDev& p) {
+  const int nkt = (p.H + TW - 1) / TW;
+  if (AX == 1) return p.B * p.nx * nkt + p.B * ((p.nx + TW - 1) / TW);
+  return p.B * p.ny * nkt + p.B * ((p.ny + TW - 1) / TW);
+}
+
+// kz of column c of a main tile: the tile's first kz is base % H.
+__device__ __forceinline__ int main_kz0(const Tile& tl, const PlanDev& p) {
+  return (int)(tl.base % p.H);
+}
+
+template <int N, int TW, int T>
+__device__ __forceinline__ void load_tile(float2* __restrict__ s,
+                                          const float2* __restrict__ g,
+                                          const Tile& tl) {
+  constexpr int P = Pitch<TW>::P;
+  if (tl.cs == 1) {
+    for (int q = threadIdx.x; q < N * TW; q += T) {
+      const int i = q / TW, c = q % TW;
+      s[i * P + c] = c < tl.ncol ? g[tl.base + i * tl.ls + c]
+                                 : make_float2(0.f, 0.f);
+    }
+  } else {
+    for (int q = threadIdx.x; q < N * TW; q += T) {
+      const int i = q % N, c = q / N;
+      s[i * P + c] = c < tl.ncol ? g[tl.base + c * tl.cs + i]
+                                 : make_float2(0.f, 0.f);
+    }
+  }
+}
+
+template <int N, int TW, int T>
+__device__ __forceinline__ void store_tile(float2* __restrict__ g,
+                                           const float2* __restrict__ s,
+                                           const Tile& tl, float scale) {
+  constexpr int P = Pitch<TW>::P;
+  if (tl.cs == 1) {
+    for (int q = threadIdx.x; q < N * TW; q += T) {
+      const int i = q / TW, c = q % TW;
+      if (c < tl.ncol) g[tl.base + i * tl.ls + c] = cscale(s[i * P + c], scale);
+    }
+  } else {
+    for (int q = threadIdx.x; q < N * TW; q += T) {
+      const int i = q % N, c = q / N;
+      if (c < tl.ncol) g[tl.base + c * tl.cs + i] = cscale(s[i * P + c], scale);
+    }
+  }
+}
+
+// Y pass.  Forward: in place (out[0] may equal in).  Inverse: each output o
+// gets IFFT_y((i g_y)^ey[o] * in) * scale[o].
+template <int N, int TW, int T, int D>
+__global__ __launch_bounds__(T) void k_ypass(YJobs jobs, PlanDev p) {
+  constexpr int P = Pitch<TW>::P;
+  __shared__ float2 a[N * P];
+  __shared__ float2 w[N * P];
+  __shared__ float2 tw[N];
+  const YJob J = blockIdx.y == 0 ? jobs.j[0]
+                 : (blockIdx.y == 1 ? jobs.j[1] : jobs.j[2]);
+  load_twiddles<N, T>(tw, p.ax[1].tw);
+  const Tile tl = make_tile<1, TW>(p, blockIdx.x);
+  load_tile<N, TW, T>(a, J.in, tl);
+  __syncthreads();
+  if (J.nout == 1 && J.ey[0] == 0) {
+    fft_tile<N, TW, T, D>(a, tw);
+    store_tile<N, TW, T>(J.out[0], a, tl, J.scale[0]);
+    return;
+  }
+  const float* __restrict__ gy = p.ax[1].g;
+#pragma unroll
+  for (int o = 0; o < kMaxJobs; ++o) {
+    if (o >= J.nout) break;
+    const int e = J.ey[o];
+    for (int q = threadIdx.x; q < N * TW; q += T) {
+      const int i = q / TW, c = q % TW;
+      float2 v = a[i * P + c];
+      if (e) {
+        const float g = gy[i];
+        v = rot_i(cscale(v, e == 2 ? g * g : g), e);
+      }
+      w[i * P + c] = v;
+    }
+    __syncthreads();
+    fft_tile<N, TW, T, D>(w, tw);
+    store_tile<N, TW, T>(J.out[o], w, tl, J.scale[o]);
+    __syncthreads();
+  }
+}
+
+// X pass: [forward FFT_x] -> multiply by Fourier terms -> [inverse FFT_x].
+template <int N, int TW, int T>
+__global__ __launch_bounds__(T) void k_xpass(XJob J, PlanDev p) {
+  constexpr int P = Pitch<TW>::P;
+  __shared__ float2 a[N * P];
+  __shared__ float2 w[N * P];
+  __shared__ float2 tw[N];
+  load_twiddles<N, T>(tw, p.ax[0].tw);
+  const Tile tl = make_tile<0, TW>(p, blockIdx.x);
+  const int kz0 = tl.nyq ? p.H : main_kz0(tl, p);
+  // per element (i = kx, c): ky, kz
+  auto kyz = [&](int c, int& ky, int& kz) {
+    if (tl.nyq) {
+      ky = min(tl.fix + c, p.ny - 1);
+      kz = p.H;
+    } else {
+      ky = tl.fix;
+      kz = min(kz0 + c, p.H);
+    }
+  };
+  if (J.nin == 1) {
+    load_tile<N, TW, T>(a, J.in[0], tl);
+    __syncthreads();
+    if (J.do_fwd) fft_tile<N, TW, T, -1>(a, tw);
+    for (int o = 0; o < J.nout; ++o) {
+      const Term tm = J.term[0][o];
+      for (int q = threadIdx.x; q < N * TW; q += T) {
+        const int i = q / TW, c = q % TW;
+        int ky, kz;
+        kyz(c, ky, kz);
+        w[i * P + c] = apply_term(a[i * P + c], tm, i, ky, kz, p);
+      }
+      __syncthreads();
+      if (J.do_inv) fft_tile<N, TW, T, +1>(w, tw);
+      store_tile<N, TW, T>(J.out[o], w, tl, 1.0f);
+      __syncthreads();
+    }
+  } else {
+    for (int q = threadIdx.x; q < N * TW; q += T) {
+      const int i = q / TW, c = q % TW;
+      w[i * P + c] = make_float2(0.f, 0.f);
+    }
+    for (int in = 0; in < J.nin; ++in) {
+      __syncthreads();
+      load_tile<N, TW, T>(a, J.in[in], tl);
+      __syncthreads();
+      if (J.do_fwd) fft_tile<N, TW, T, -1>(a, tw);
+      const Term tm = J.term[in][0];
+      for (int q = threadIdx.x; q < N * TW; q += T) {
+        const int i = q / TW, c = q % TW;
+        int ky, kz;
+        kyz(c, ky, kz);
+        w[i * P + c] = cadd(w[i * P + c],
+                            apply_term(a[i * P + c], tm, i, ky, kz, p));
+      }
+    }
+    __syncthreads();
+    if (J.do_inv) fft_tile<N, TW, T, +1>(w, tw);
+    store_tile<N, TW, T>(J.out[0], w, tl, 1.0f);
+  }
+}
+
+// -------------------------------------------- full <-> half spectrum -----
+// full[b][x][y][z] (the reference's complex64 layout) from the half spectrum.
+__global__ void k_spec_to_full(const float2* __restrict__ spec,
+                               float2* __restrict__ full, PlanDev p) {
+  const long long M = (long long)p.nx * p.ny * p.nz;
+  const long long tot = (long long)p.B * M;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(e % p.nz);
+    const long long r = e / p.nz;
+    const int y = (int)(r % p.ny);
+    const long long r2 = r / p.ny;
+    const int x = (int)(r2 % p.nx);
+    const int b = (int)(r2 / p.nx);
+    int xs = x, ys = y, zs = z;
+    bool cj = false;
+    if (z > p.H) {
+      xs = (p.nx - x) % p.nx;
+      ys = (p.ny - y) % p.ny;
+      zs = p.nz - z;
+      cj = true;
+    }
+    const long long row = ((long long)b * p.nx + xs) * p.ny + ys;
+    float2 v = (zs == p.H) ? spec[p.nyq0 + row] : spec[row * p.H + zs];
+    if (cj) v.y = -v.y;
+    full[e] = v;
+  }
+}
+
+// Hermitian projection of an arbitrary full spectrum, kept on the half grid:
+// real(ifft(c)) == C2R((c(k) + conj(c(-k)))/2).
+__global__ void k_full_to_spec(const float2* __restrict__ full,
+                               float2* __restrict__ spec, PlanDev p) {
+  const long long tot = (long long)p.rows * (p.H + 1);
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(e % (p.H + 1));
+    const long long row = e / (p.H + 1);
+    const int y = (int)(row % p.ny);
+    const long long r2 = row / p.ny;
+    const int x = (int)(r2 % p.nx);
+    const int b = (int)(r2 / p.nx);
+    const int xm = (p.nx - x) % p.nx, ym = (p.ny - y) % p.ny,
+              zm = (p.nz - kz) % p.nz;
+    const long long M = (long long)p.nx * p.ny * p.nz;
+    const float2 u = full[b * M + ((long long)x * p.ny + y) * p.nz + kz];
+    const float2 v = full[b * M + ((long long)xm * p.ny + ym) * p.nz + zm];
+    const float2 hv = make_float2(0.5f * (u.x + v.x), 0.5f * (u.y - v.y));
+    if (kz == p.H)
+      spec[p.nyq0 + row] = hv;
+    else
+      spec[row * p.H + kz] = hv;
+  }
+}
+
+// ------------------------------------------------------------ dispatch ---
+#define PMX_FFT_SIZES(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) \
+  X(384) X(512) X(768) X(1024)
+
+bool fft_size_supported(int n) {
+  switch (n) {
+#define PMX_CASE(n) case n:
+    PMX_FFT_SIZES(PMX_CASE)
+#undef PMX_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+// threads per block and tile width per line length
+template <int N>
+struct Cfg {
+  static constexpr int TW = (N >= 768) ? 8 : 16;
+  static constexpr int T = (N * TW >= 4096) ? 256 : 128;
+  static constexpr int ZTW = (N >= 1024) ? 4 : (N >= 512 ? 8 : 16);
+};
+
+template <int N>
+static int zr2c_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
+  constexpr int TW = Cfg<N>::ZTW, T = Cfg<N>::T;
+  dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)), jobs.n);
+  hipLaunchKernelGGL((k_zpass_r2c<N, TW, T>), grid, dim3(T), 0, s, jobs,
+                     pl.d);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+template <int N>
+static int zc2r_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
+  constexpr int TW = Cfg<N>::ZTW, T = Cfg<N>::T;
+  dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)), jobs.n);
+  hipLaunchKernelGGL((k_zpass_c2r<N, TW, T>), grid, dim3(T), 0, s, jobs,
+                     pl.d);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+template <int N>
+static int y_n(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
+  constexpr int TW = Cfg<N>::TW, T = Cfg<N>::T;
+  const PlanDev& p = pl.d;
+  const int nkt = (p.H + TW - 1) / TW;
+  const int tiles = p.B * p.nx * nkt + p.B * ((p.nx + TW - 1) / TW);
+  dim3 grid(tiles, jobs.n);
+  if (inv)
+    hipLaunchKernelGGL((k_ypass<N, TW, T, +1>), grid, dim3(T), 0, s, jobs, p);
+  else
+    hipLaunchKernelGGL((k_ypass<N, TW, T, -1>), grid, dim3(T), 0, s, jobs, p);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+template <int N>
+static int x_n(const Plan& pl, const XJob& job, hipStream_t s) {
+  constexpr int TW = Cfg<N>::TW, T = Cfg<N>::T;
+  const PlanDev& p = pl.d;
+  const int nkt = (p.H + TW - 1) / TW;
+  const int tiles = p.B * p.ny * nkt + p.B * ((p.ny + TW - 1) / TW);
+  hipLaunchKernelGGL((k_xpass<N, TW, T>), dim3(tiles), dim3(T), 0, s, job, p);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_zpass_r2c(const Plan& p, const ZJobs& jobs, hipStream_t s) {
+  switch (p.d.nz) {
+#define PMX_CASE(n) \
+  case n:           \
+    return zr2c_n<n>(p, jobs, s);
+    PMX_FFT_SIZES(PMX_CASE)
+#undef PMX_CASE
+  }
+  set_error("unsupported nz");
+  return kUnsupported;
+}
+int launch_zpass_c2r(const Plan& p, const ZJobs& jobs, hipStream_t s) {
+  switch (p.d.nz) {
+#define PMX_CASE(n) \
+  case n:           \
+    return zc2r_n<n>(p, jobs, s);
+    PMX_FFT_SIZES(PMX_CASE)
+#undef PMX_CASE
+  }
+  set_error("unsupported nz");
+  return kUnsupported;
+}
+int launch_ypass(const Plan& p, const YJobs& jobs, int inverse,
+                 hipStream_t s) {
+  switch (p.d.ny) {
+#define PMX_CASE(n) \
+  case n:           \
+    return y_n<n>(p, jobs, inverse, s);
+    PMX_FFT_SIZES(PMX_CASE)
+#undef PMX_CASE
+  }
+  set_error("unsupported ny");
+  return kUnsupported;
+}
+int launch_xpass(const Plan& p, const XJob& job, hipStream_t s) {
+  switch (p.d.nx) {
+#define PMX_CASE(n) \
+  case n:           \
+    return x_n<n>(p, job, s);
+    PMX_FFT_SIZES(PMX_CASE)
+#undef PMX_CASE
+  }
+  set_error("unsupported nx");
+  return kUnsupported;
+}
+
+int launch_spec_to_full(const Plan& p, const float2* spec, float2* full,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_spec_to_full, dim3(2048), dim3(256), 0, s, spec, full,
+                     p.d);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+int launch_full_to_spec(const Plan& p, const float2* full, float2* spec,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_full_to_spec, dim3(2048), dim3(256), 0, s, full, spec,
+                     p.d);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace pmx

@@ -1,0 +1,963 @@
+// Plans, PM pipelines (force, LPT, adjoints), the nbody time loop and the
+// C ABI of libpmx_hip.so.  See include/pmx.h for the contract of each entry
+// point and the reference function it replaces.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "pmx_internal.h"
+#include "../../include/pmx.h"
+
+namespace pmx {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ------------------------------------------------------------------ plan --
+static int plan_create(Plan** out, int B, int nx, int ny, int nz) {
+  PMX_REQUIRE(out != nullptr, "plan_create: null output");
+  PMX_REQUIRE(B >= 1, "plan_create: batch must be >= 1");
+  PMX_REQUIRE(fft_size_supported(nx) && fft_size_supported(ny) &&
+                  fft_size_supported(nz),
+              "plan_create: mesh sides must be in {8,12,16,24,32,48,64,96,"
+              "128,192,256,384,512,768,1024}");
+  Plan* p = new Plan();
+  PlanDev& d = p->d;
+  d.B = B;
+  d.nx = nx;
+  d.ny = ny;
+  d.nz = nz;
+  d.H = nz / 2;
+  d.rows = (long long)B * nx * ny;
+  d.nyq0 = d.rows * d.H;
+  d.S = d.rows * (d.H + 1);
+  const int n[3] = {nx, ny, nz};
+  size_t bytes = 0;
+  size_t off_tw[3], off_k2[3], off_g[3];
+  for (int a = 0; a < 3; ++a) {
+    off_k2[a] = bytes;
+    bytes += sizeof(double) * n[a];
+    off_tw[a] = bytes;
+    bytes += sizeof(float2) * n[a];
+    off_g[a] = bytes;
+    bytes += sizeof(float) * n[a];
+    bytes = (bytes + 255) & ~size_t(255);
+  }
+  std::vector<unsigned char> host(bytes, 0);
+  for (int a = 0; a < 3; ++a) {
+    double* k2 = reinterpret_cast<double*>(host.data() + off_k2[a]);
+    float2* tw = reinterpret_cast<float2*>(host.data() + off_tw[a]);
+    float* g = reinterpret_cast<float*>(host.data() + off_g[a]);
+    for (int i = 0; i < n[a]; ++i) {
+      // numpy.fft.fftfreq: i/n for i < (n+1)/2 (n even: i < n/2), else
+      // (i - n)/n; times 2 pi (kernels.py:16-17)
+      const int fi = (i < (n[a] + 1) / 2) ? i : i - n[a];
+      const double w = (double)fi / (double)n[a] * 2.0 * M_PI;
+      k2[i] = w * w;
+      g[i] = (float)((8.0 * std::sin(w) - std::sin(2.0 * w)) / 6.0);
+      const double ang = -2.0 * M_PI * (double)i / (double)n[a];
+      tw[i] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+  }
+  hipError_t e = hipMalloc(&p->dev_mem, bytes);
+  if (e != hipSuccess) {
+    delete p;
+    set_error(std::string("plan_create: hipMalloc: ") + hipGetErrorString(e));
+    return kNoMemory;
+  }
+  e = hipMemcpy(p->dev_mem, host.data(), bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(p->dev_mem);
+    delete p;
+    set_error(std::string("plan_create: hipMemcpy: ") + hipGetErrorString(e));
+    return kHipError;
+  }
+  unsigned char* base = static_cast<unsigned char*>(p->dev_mem);
+  for (int a = 0; a < 3; ++a) {
+    d.ax[a].k2 = reinterpret_cast<const double*>(base + off_k2[a]);
+    d.ax[a].tw = reinterpret_cast<const float2*>(base + off_tw[a]);
+    d.ax[a].g = reinterpret_cast<const float*>(base + off_g[a]);
+  }
+  *out = p;
+  return kOk;
+}
+
+static inline Term term(float scale, int lap, int ex, int ey, int ez) {
+  Term t;
+  t.scale = scale;
+  t.lap = lap;
+  t.e[0] = ex;
+  t.e[1] = ey;
+  t.e[2] = ez;
+  return t;
+}
+
+static inline ZJob zr2c(const float* in, float2* out, float scale) {
+  ZJob j{};
+  j.rin = in;
+  j.cout = out;
+  j.scale = scale;
+  return j;
+}
+static inline ZJob zc2r(const float2* in, float* out, float scale, int ez) {
+  ZJob j{};
+  j.cin = in;
+  j.rout = out;
+  j.scale = scale;
+  j.ez = ez;
+  return j;
+}
+static inline YJob yfwd(float2* buf) {
+  YJob j{};
+  j.in = buf;
+  j.out[0] = buf;
+  j.scale[0] = 1.0f;
+  j.nout = 1;
+  return j;
+}
+
+// ------------------------------------------------------------ pipelines --
+// Forward 3-D transform up to (not including) the x pass: real -> spec.
+static int r2c_zy(const Plan& p, const float* const* in, float2* const* spec,
+                  int n, float scale, hipStream_t s) {
+  ZJobs zj{};
+  YJobs yj{};
+  zj.n = yj.n = n;
+  for (int i = 0; i < n; ++i) {
+    zj.j[i] = zr2c(in[i], spec[i], scale);
+    yj.j[i] = yfwd(spec[i]);
+  }
+  PMX_TRY(launch_zpass_r2c(p, zj, s));
+  PMX_TRY(launch_ypass(p, yj, 0, s));
+  return kOk;
+}
+
+// From a spectrum whose x pass is pending (fwd = 1) or done (fwd = 0):
+// out_d = IFFT3(spec * lap * (i g_d) * scale), d = x, y, z.
+// w0, w1, w2: spec buffers (w0 may alias spec when fwd = 1).
+static int poisson_grad(const Plan& p, const float2* spec, int fwd,
+                        float2* w0, float2* w1, float2* w2, float* fx,
+                        float* fy, float* fz, float scale, hipStream_t s) {
+  XJob xj{};
+  xj.nin = 1;
+  xj.nout = 2;
+  xj.in[0] = spec;
+  xj.out[0] = w0;  // lap
+  xj.out[1] = w1;  // lap * i gx
+  xj.term[0][0] = term(scale, 1, 0, 0, 0);
+  xj.term[0][1] = term(scale, 1, 1, 0, 0);
+  xj.do_fwd = fwd;
+  xj.do_inv = 1;
+  PMX_TRY(launch_xpass(p, xj, s));
+  YJobs yj{};
+  yj.n = 2;
+  yj.j[0].in = w0;
+  yj.j[0].nout = 2;
+  yj.j[0].out[0] = w2;  // for Fy: i gy
+  yj.j[0].ey[0] = 1;
+  yj.j[0].scale[0] = 1.0f;
+  yj.j[0].out[1] = w0;  // for Fz
+  yj.j[0].ey[1] = 0;
+  yj.j[0].scale[1] = 1.0f;
+  yj.j[1].in = w1;
+  yj.j[1].nout = 1;
+  yj.j[1].out[0] = w1;
+  yj.j[1].ey[0] = 0;
+  yj.j[1].scale[0] = 1.0f;
+  PMX_TRY(launch_ypass(p, yj, 1, s));
+  ZJobs zj{};
+  zj.n = 3;
+  zj.j[0] = zc2r(w1, fx, 1.0f, 0);
+  zj.j[1] = zc2r(w2, fy, 1.0f, 0);
+  zj.j[2] = zc2r(w0, fz, 1.0f, 1);
+  PMX_TRY(launch_zpass_c2r(p, zj, s));
+  return kOk;
+}
+
+static int force_meshes(const Plan& p, const float* rho, float2* w, float* fx,
+                        float* fy, float* fz, float rho_scale, hipStream_t s) {
+  float2* w0 = w;
+  float2* w1 = w + p.d.S;
+  float2* w2 = w + 2 * p.d.S;
+  PMX_TRY(r2c_zy(p, &rho, &w0, 1, rho_scale, s));
+  return poisson_grad(p, w0, 1, w0, w1, w2, fx, fy, fz, 1.0f, s);
+}
+
+// Adjoint of the map (g_x, g_y, g_z) <- force_meshes(rho):
+// grho = IFFT3(sum_d conj(lap i g_d) FFT3(g_d) * rho_scale).
+static int force_meshes_adj(const Plan& p, const float* gx, const float* gy,
+                            const float* gz, float2* w, float* grho,
+                            float rho_scale, hipStream_t s) {
+  float2* ws[3] = {w, w + p.d.S, w + 2 * p.d.S};
+  const float* in[3] = {gx, gy, gz};
+  PMX_TRY(r2c_zy(p, in, ws, 3, rho_scale, s));
+  XJob xj{};
+  xj.nin = 3;
+  xj.nout = 1;
+  for (int d = 0; d < 3; ++d) {
+    xj.in[d] = ws[d];
+    xj.term[d][0] = term(-1.0f, 1, d == 0, d == 1, d == 2);
+  }
+  xj.out[0] = ws[0];
+  xj.do_fwd = 1;
+  xj.do_inv = 1;
+  PMX_TRY(launch_xpass(p, xj, s));
+  YJobs yj{};
+  yj.n = 1;
+  yj.j[0].in = ws[0];
+  yj.j[0].nout = 1;
+  yj.j[0].out[0] = ws[0];
+  yj.j[0].scale[0] = 1.0f;
+  PMX_TRY(launch_ypass(p, yj, 1, s));
+  ZJobs zj{};
+  zj.n = 1;
+  zj.j[0] = zc2r(ws[0], grho, 1.0f, 0);
+  PMX_TRY(launch_zpass_c2r(p, zj, s));
+  return kOk;
+}
+
+// Adjoint of poisson_grad(spec, fwd=0): gdk = sum_d conj(lap i g_d) FFT3(g_d)
+// * scale (fully transformed half spectrum).
+static int poisson_grad_adj(const Plan& p, const float* gx, const float* gy,
+                            const float* gz, float2* w, float2* gdk,
+                            float scale, hipStream_t s) {
+  float2* ws[3] = {w, w + p.d.S, w + 2 * p.d.S};
+  const float* in[3] = {gx, gy, gz};
+  PMX_TRY(r2c_zy(p, in, ws, 3, 1.0f, s));
+  XJob xj{};
+  xj.nin = 3;
+  xj.nout = 1;
+  for (int d = 0; d < 3; ++d) {
+    xj.in[d] = ws[d];
+    xj.term[d][0] = term(-scale, 1, d == 0, d == 1, d == 2);
+  }
+  xj.out[0] = gdk;
+  xj.do_fwd = 1;
+  xj.do_inv = 0;
+  PMX_TRY(launch_xpass(p, xj, s));
+  return kOk;
+}
+
+// out (+)= IFFT3(sum_q Term_q FFT3(in_q) * in_scale), nin <= 3.
+static int fourier_apply(const Plan& p, int nin, const float* const* in,
+                         const Term* terms, float in_scale, float2* w,
+                         float* out, int accum, hipStream_t s) {
+  float2* ws[3] = {w, w + p.d.S, w + 2 * p.d.S};
+  PMX_TRY(r2c_zy(p, in, ws, nin, in_scale, s));
+  XJob xj{};
+  xj.nin = nin;
+  xj.nout = 1;
+  for (int q = 0; q < nin; ++q) {
+    xj.in[q] = ws[q];
+    xj.term[q][0] = terms[q];
+  }
+  xj.out[0] = ws[0];
+  xj.do_fwd = 1;
+  xj.do_inv = 1;
+  PMX_TRY(launch_xpass(p, xj, s));
+  YJobs yj{};
+  yj.n = 1;
+  yj.j[0].in = ws[0];
+  yj.j[0].nout = 1;
+  yj.j[0].out[0] = ws[0];
+  yj.j[0].scale[0] = 1.0f;
+  PMX_TRY(launch_ypass(p, yj, 1, s));
+  ZJobs zj{};
+  zj.n = 1;
+  zj.j[0] = zc2r(ws[0], out, 1.0f, 0);
+  zj.j[0].accum = accum;
+  return launch_zpass_c2r(p, zj, s);
+}
+
+// ---------------------------------------------------------- LPT kernels --
+__global__ void k_lpt2_combine(float* __restrict__ out,
+                               const float* __restrict__ xx,
+                               const float* __restrict__ yy,
+                               const float* __restrict__ zz,
+                               const float* __restrict__ xy,
+                               const float* __restrict__ xz,
+                               const float* __restrict__ yz, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    // tfpm.py:130-143: D1 = [1,2,0], D2 = [2,0,1]
+    float s = 0.0f;
+    s = s + yy[i] * zz[i];
+    s = s + zz[i] * xx[i];
+    s = s + xx[i] * yy[i];
+    s = s - yz[i] * yz[i];
+    s = s - xz[i] * xz[i];
+    s = s - xy[i] * xy[i];
+    out[i] = s * (3.0f / 7.0f);
+  }
+}
+
+__global__ void k_lpt_assemble(float* __restrict__ state,
+                               const float* __restrict__ d1x,
+                               const float* __restrict__ d1y,
+                               const float* __restrict__ d1z,
+                               const float* __restrict__ d2x,
+                               const float* __restrict__ d2y,
+                               const float* __restrict__ d2z, int B, int nx,
+                               int ny, int nz, float c0, float c1, float c2,
+                               float c3, float c4, float c5) {
+  const long long N = (long long)nx * ny * nz;
+  const long long tot = (long long)B * N;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long q = i % N;
+    const int qz = (int)(q % nz);
+    const int qy = (int)((q / nz) % ny);
+    const int qx = (int)(q / ((long long)nz * ny));
+    const float Q[3] = {(float)qx, (float)qy, (float)qz};
+    const float a1[3] = {d1x[i], d1y[i], d1z[i]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float DX = c0 * a1[d];
+      float P = c1 * DX;
+      float F = c2 * DX;
+      if (d2x) {
+        const float a2 = d == 0 ? d2x[i] : (d == 1 ? d2y[i] : d2z[i]);
+        const float DX2 = c3 * a2;
+        P = P + c4 * DX2;
+        F = F + c5 * DX2;
+        DX = DX + DX2;
+      }
+      state[3 * i + d] = DX + Q[d];
+      state[3 * tot + 3 * i + d] = P;
+      state[6 * tot + 3 * i + d] = F;
+    }
+  }
+}
+
+// Cotangent update of the KDK scheme run backwards (see nbody_adjoint).
+__global__ void k_adj_update(float* __restrict__ gX, float* __restrict__ gP,
+                             float* __restrict__ gF, long long n, float d,
+                             float k1, float k2prev, int mode) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (mode == 0) {  // closing kick of the last step: gF += k2 gP
+      gF[i] = gF[i] + k2prev * gP[i];
+    } else {  // after the force VJP of a step
+      const float gp = gP[i] + d * gX[i];
+      gP[i] = gp;
+      float gf = k1 * gp;
+      if (mode == 2) gf = gf + k2prev * gp;
+      gF[i] = gf;
+    }
+  }
+}
+
+// Adjoint of k_lpt_assemble: per-component cotangent meshes of dx1 / dx2.
+__global__ void k_lpt_assemble_adj(const float* __restrict__ gstate,
+                                   float* __restrict__ g1,
+                                   float* __restrict__ g2, long long tot,
+                                   float c0, float c1, float c2, float c3,
+                                   float c4, float c5) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot;
+       i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float gX = gstate[3 * i + d];
+      const float gP = gstate[3 * tot + 3 * i + d];
+      const float gF = gstate[6 * tot + 3 * i + d];
+      g1[d * tot + i] = c0 * (gX + c1 * gP + c2 * gF);
+      if (g2) g2[d * tot + i] = c3 * (gX + c4 * gP + c5 * gF);
+    }
+  }
+}
+
+// Adjoint of k_lpt2_combine: cotangents of the six phi_ij from that of
+// the source.  phi order: xx, yy, zz, xy, xz, yz (contiguous meshes).
+__global__ void k_lpt2_combine_adj(const float* __restrict__ phi,
+                                   const float* __restrict__ gs,
+                                   float* __restrict__ gphi, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float g = gs[i] * (3.0f / 7.0f);
+    const float xx = phi[i], yy = phi[n + i], zz = phi[2 * n + i];
+    gphi[i] = g * (yy + zz);
+    gphi[n + i] = g * (zz + xx);
+    gphi[2 * n + i] = g * (xx + yy);
+    gphi[3 * n + i] = -2.0f * g * phi[3 * n + i];
+    gphi[4 * n + i] = -2.0f * g * phi[4 * n + i];
+    gphi[5 * n + i] = -2.0f * g * phi[5 * n + i];
+  }
+}
+
+// spec *= amp (real, batch-independent, half-spectrum layout of one mesh)
+__global__ void k_spec_mul_real(float2* __restrict__ spec,
+                                const float* __restrict__ amp, PlanDev p) {
+  const long long per_main = (long long)p.nx * p.ny * p.H;
+  const long long per_nyq = (long long)p.nx * p.ny;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       e < p.S; e += (long long)gridDim.x * blockDim.x) {
+    float a;
+    if (e < p.nyq0)
+      a = amp[e % per_main];
+    else
+      a = amp[per_main + (e - p.nyq0) % per_nyq];
+    spec[e] = make_float2(spec[e].x * a, spec[e].y * a);
+  }
+}
+
+static inline unsigned gridfor(long long n) {
+  return (unsigned)std::max<long long>(
+      1, std::min<long long>((n + 255) / 256, 16384));
+}
+
+// ---------------------------------------------------------------- nbody --
+struct NbodyWork {
+  float* rho;
+  float* f[3];
+  float2* w;
+};
+static NbodyWork nbody_work(const Plan& p, float* work) {
+  NbodyWork nw;
+  const long long M = p.d.rows * p.d.nz;
+  nw.rho = work;
+  for (int d = 0; d < 3; ++d) nw.f[d] = work + (1 + d) * M;
+  nw.w = reinterpret_cast<float2*>(work + 4 * M);
+  return nw;
+}
+static long long nbody_work_floats(const Plan& p) {
+  const long long M = p.d.rows * p.d.nz;
+  return 4 * M + 2 * 3 * p.d.S;
+}
+
+// force meshes at X (positions * f), density scaled by 1/nbar and 1/M.
+static int force_at(const Plan& p, const float* X, long long N, float f,
+                    const NbodyWork& nw, hipStream_t s) {
+  const long long M = p.d.rows * p.d.nz;
+  const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
+  PMX_TRY(launch_fill(nw.rho, M, 0.0f, s));
+  MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
+  PMX_TRY(launch_paint(nw.rho, X, nullptr, 1.0f, md, f, 0, s));
+  // rho * (1/nbar) with nbar = N / Mb (tfpm.py:346-351), then r2c / Mb
+  const double inv_nbar = (double)Mb / (double)N;
+  const float sc = (float)(inv_nbar / (double)Mb);
+  return force_meshes(p, nw.rho, nw.w, nw.f[0], nw.f[1], nw.f[2], sc, s);
+}
+
+static int nbody_run(const Plan& p, float* state, long long N, float f,
+                     float fac, const double* coef, int b0, int b1,
+                     int jump, float* work, hipStream_t s) {
+  const long long BN3 = (long long)p.d.B * N * 3;
+  float* X = state;
+  float* P = state + BN3;
+  float* F = state + 2 * BN3;
+  MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
+  NbodyWork nw = nbody_work(p, work);
+  KickDrift kd{};
+  kd.fac = fac;
+  kd.f = f;
+  if (jump) {
+    PMX_TRY(force_at(p, X, N, f, nw, s));
+    kd.do_k2 = 0;
+    kd.do_k1d = (b0 < b1);
+    if (kd.do_k1d) {
+      kd.k1 = (float)coef[3 * b0];
+      kd.d = (float)coef[3 * b0 + 1];
+    }
+    PMX_TRY(launch_readout_kick_drift(X, P, F, nw.f[0], nw.f[1], nw.f[2], md,
+                                      kd, s));
+  } else if (b0 < b1) {
+    // opening kick + drift of the first step from the stored force
+    KickDrift c{};
+    c.do_k1d = 1;
+    c.k1 = (float)coef[3 * b0];
+    c.d = (float)coef[3 * b0 + 1];
+    PMX_TRY(launch_kick_drift_only(X, P, F, md, c, s));
+  }
+  for (int i = b0; i < b1; ++i) {
+    PMX_TRY(force_at(p, X, N, f, nw, s));
+    kd.do_k2 = 1;
+    kd.k2 = (float)coef[3 * i + 2];
+    kd.do_k1d = (i + 1 < b1);
+    if (kd.do_k1d) {
+      kd.k1 = (float)coef[3 * (i + 1)];
+      kd.d = (float)coef[3 * (i + 1) + 1];
+    }
+    PMX_TRY(launch_readout_kick_drift(X, P, F, nw.f[0], nw.f[1], nw.f[2], md,
+                                      kd, s));
+  }
+  return kOk;
+}
+
+// X-cotangent contribution of F = force(X) for cotangent gF.
+static int force_vjp(const Plan& p, const float* X, const float* gF,
+                     float* gX, long long N, float f, float fac,
+                     const NbodyWork& nw, float* gm, hipStream_t s) {
+  const long long M = p.d.rows * p.d.nz;
+  const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
+  MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
+  PMX_TRY(force_at(p, X, N, f, nw, s));
+  PMX_TRY(launch_fill(gm, 3 * M, 0.0f, s));
+  PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X,
+                                     nw.f[0], nw.f[1], nw.f[2], md, f,
+                                     fac / f, s));
+  const double inv_nbar = (double)Mb / (double)N;
+  const float sc = (float)(inv_nbar / (double)Mb);
+  PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nw.rho, sc, s));
+  PMX_TRY(launch_paint_adj(gX, nullptr, nw.rho, X, nullptr, md, f, 1.0f, 1,
+                           s));
+  return kOk;
+}
+
+static long long nbody_adj_work_floats(const Plan& p, long long N) {
+  const long long M = p.d.rows * p.d.nz;
+  return nbody_work_floats(p) + 3 * M + 9 * (long long)p.d.B * N;
+}
+
+static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
+                         int nsteps, long long N, float f, float fac,
+                         const double* coef, float* gstate, float* xslots,
+                         float* work, hipStream_t s) {
+  PMX_REQUIRE(every >= 1, "nbody_adjoint: every must be >= 1");
+  const long long M = p.d.rows * p.d.nz;
+  const long long BN3 = (long long)p.d.B * N * 3;
+  NbodyWork nw = nbody_work(p, work);
+  float* gm = work + nbody_work_floats(p);
+  float* st = gm + 3 * M;  // scratch state (3, B, N, 3)
+  float* gX = gstate;
+  float* gP = gstate + BN3;
+  float* gF = gstate + 2 * BN3;
+  const unsigned g = gridfor(BN3);
+  const int nseg = (nsteps + every - 1) / every;
+  if (nsteps > 0)
+    hipLaunchKernelGGL(k_adj_update, dim3(g), dim3(256), 0, s, gX, gP, gF,
+                       BN3, 0.f, 0.f, (float)coef[3 * (nsteps - 1) + 2], 0);
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int b0 = sg * every, b1 = std::min(nsteps, b0 + every);
+    PMX_HIP_CHECK(hipMemcpyAsync(st, ckpt + (long long)sg * 3 * BN3,
+                                 sizeof(float) * 3 * BN3,
+                                 hipMemcpyDeviceToDevice, s));
+    for (int i = b0; i < b1; ++i) {
+      PMX_TRY(nbody_run(p, st, N, f, fac, coef, i, i + 1, sg == 0 && i == 0,
+                        work, s));
+      PMX_HIP_CHECK(hipMemcpyAsync(xslots + (long long)(i - b0) * BN3, st,
+                                   sizeof(float) * BN3,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    for (int i = b1 - 1; i >= b0; --i) {
+      PMX_TRY(force_vjp(p, xslots + (long long)(i - b0) * BN3, gF, gX, N, f,
+                        fac, nw, gm, s));
+      const int mode = i > 0 ? 2 : 1;
+      hipLaunchKernelGGL(k_adj_update, dim3(g), dim3(256), 0, s, gX, gP, gF,
+                         BN3, (float)coef[3 * i + 1], (float)coef[3 * i],
+                         i > 0 ? (float)coef[3 * (i - 1) + 2] : 0.f, mode);
+      PMX_LAUNCH_CHECK();
+    }
+  }
+  // jump-start force at the initial positions; the initial F is discarded
+  PMX_TRY(force_vjp(p, ckpt, gF, gX, N, f, fac, nw, gm, s));
+  PMX_TRY(launch_fill(gF, BN3, 0.0f, s));
+  return kOk;
+}
+
+}  // namespace pmx
+
+// ================================================================ C ABI ==
+using namespace pmx;
+
+struct pmx_plan {
+  Plan p;
+};
+
+static inline const Plan& P(const pmx_plan* h) { return h->p; }
+
+extern "C" {
+
+int pmx_version(void) { return PMX_VERSION; }
+const char* pmx_last_error(void) { return last_error(); }
+
+int pmx_plan_create(pmx_plan** plan, int B, int nx, int ny, int nz) {
+  PMX_REQUIRE(plan != nullptr, "pmx_plan_create: null output");
+  PMX_REQUIRE(nz % 2 == 0, "pmx_plan_create: nz must be even");
+  Plan* p = nullptr;
+  PMX_TRY(plan_create(&p, B, nx, ny, nz));
+  pmx_plan* h = new pmx_plan();
+  h->p = *p;
+  delete p;
+  *plan = h;
+  return 0;
+}
+
+int pmx_plan_destroy(pmx_plan* plan) {
+  if (!plan) return 0;
+  if (plan->p.dev_mem) PMX_HIP_CHECK(hipFree(plan->p.dev_mem));
+  delete plan;
+  return 0;
+}
+
+long long pmx_plan_spec_elems(const pmx_plan* plan) {
+  return plan ? plan->p.d.S : -1;
+}
+
+int pmx_plan_set_tiles(pmx_plan* plan, int ty, int tx) {
+  PMX_REQUIRE(plan, "pmx_plan_set_tiles: null plan");
+  PMX_REQUIRE((ty == 8 || ty == 16) && (tx == 8 || tx == 16),
+              "pmx_plan_set_tiles: tile widths must be 8 or 16");
+  plan->p.tw_y = ty;
+  plan->p.tw_x = tx;
+  return 0;
+}
+
+#define CHECK_MESH(B, N, nx, ny, nz)                                   \
+  PMX_REQUIRE((B) >= 1 && (N) >= 0 && (nx) >= 1 && (ny) >= 1 &&        \
+                  (nz) >= 1,                                           \
+              "bad shape: need B >= 1, N >= 0 and positive mesh sides")
+
+int pmx_cic_paint(float* mesh, const float* pos, const float* weight, int B,
+                  long long N, int nx, int ny, int nz, float ps,
+                  void* stream) {
+  CHECK_MESH(B, N, nx, ny, nz);
+  PMX_REQUIRE(mesh && (pos || N == 0), "pmx_cic_paint: null pointer");
+  MeshDims d{B, nx, ny, nz, N};
+  return launch_paint(mesh, pos, weight, 1.0f, d, ps, 0, S(stream));
+}
+
+int pmx_cic_readout(float* out, const float* mesh, const float* pos, int B,
+                    long long N, int nx, int ny, int nz, float ps,
+                    void* stream) {
+  CHECK_MESH(B, N, nx, ny, nz);
+  PMX_REQUIRE(mesh && ((out && pos) || N == 0),
+              "pmx_cic_readout: null pointer");
+  MeshDims d{B, nx, ny, nz, N};
+  return launch_readout(out, mesh, pos, d, ps, S(stream));
+}
+
+int pmx_cic_paint_adj(float* dpos, float* dweight, const float* gmesh,
+                      const float* pos, const float* weight, int B,
+                      long long N, int nx, int ny, int nz, float ps,
+                      void* stream) {
+  CHECK_MESH(B, N, nx, ny, nz);
+  PMX_REQUIRE(gmesh && (pos || N == 0), "pmx_cic_paint_adj: null pointer");
+  MeshDims d{B, nx, ny, nz, N};
+  return launch_paint_adj(dpos, dweight, gmesh, pos, weight, d, ps, 1.0f, 0,
+                          S(stream));
+}
+
+int pmx_cic_readout_adj(float* gmesh, float* dpos, const float* gout,
+                        const float* mesh, const float* pos, int B,
+                        long long N, int nx, int ny, int nz, float ps,
+                        void* stream) {
+  CHECK_MESH(B, N, nx, ny, nz);
+  PMX_REQUIRE(gout && (pos || N == 0), "pmx_cic_readout_adj: null pointer");
+  PMX_REQUIRE(dpos == nullptr || mesh != nullptr,
+              "pmx_cic_readout_adj: position gradient needs the mesh");
+  MeshDims d{B, nx, ny, nz, N};
+  return launch_readout_adj(gmesh, dpos, gout, mesh, pos, d, ps, 0,
+                            S(stream));
+}
+
+int pmx_fft_r2c(const pmx_plan* plan, const float* in, float* spec,
+                float scale, void* stream) {
+  PMX_REQUIRE(plan && in && spec, "pmx_fft_r2c: null pointer");
+  const Plan& p = P(plan);
+  float2* w = reinterpret_cast<float2*>(spec);
+  PMX_TRY(r2c_zy(p, &in, &w, 1, scale, S(stream)));
+  XJob xj{};
+  xj.nin = xj.nout = 1;
+  xj.in[0] = w;
+  xj.out[0] = w;
+  xj.term[0][0] = term(1.0f, 0, 0, 0, 0);
+  xj.do_fwd = 1;
+  xj.do_inv = 0;
+  return launch_xpass(p, xj, S(stream));
+}
+
+int pmx_fft_c2r(const pmx_plan* plan, float* spec, float* out, float scale,
+                void* stream) {
+  PMX_REQUIRE(plan && out && spec, "pmx_fft_c2r: null pointer");
+  const Plan& p = P(plan);
+  float2* w = reinterpret_cast<float2*>(spec);
+  XJob xj{};
+  xj.nin = xj.nout = 1;
+  xj.in[0] = w;
+  xj.out[0] = w;
+  xj.term[0][0] = term(1.0f, 0, 0, 0, 0);
+  xj.do_fwd = 0;
+  xj.do_inv = 1;
+  PMX_TRY(launch_xpass(p, xj, S(stream)));
+  YJobs yj{};
+  yj.n = 1;
+  yj.j[0].in = w;
+  yj.j[0].nout = 1;
+  yj.j[0].out[0] = w;
+  yj.j[0].scale[0] = 1.0f;
+  PMX_TRY(launch_ypass(p, yj, 1, S(stream)));
+  ZJobs zj{};
+  zj.n = 1;
+  zj.j[0] = zc2r(w, out, scale, 0);
+  return launch_zpass_c2r(p, zj, S(stream));
+}
+
+int pmx_spec_to_full(const pmx_plan* plan, const float* spec, float* full,
+                     void* stream) {
+  PMX_REQUIRE(plan && spec && full, "pmx_spec_to_full: null pointer");
+  return launch_spec_to_full(P(plan), reinterpret_cast<const float2*>(spec),
+                             reinterpret_cast<float2*>(full), S(stream));
+}
+
+int pmx_full_to_spec(const pmx_plan* plan, const float* full, float* spec,
+                     void* stream) {
+  PMX_REQUIRE(plan && spec && full, "pmx_full_to_spec: null pointer");
+  return launch_full_to_spec(P(plan), reinterpret_cast<const float2*>(full),
+                             reinterpret_cast<float2*>(spec), S(stream));
+}
+
+int pmx_fft_c2r_poisson_grad(const pmx_plan* plan, const float* dk,
+                             float* work, float* fx, float* fy, float* fz,
+                             float scale, void* stream) {
+  PMX_REQUIRE(plan && dk && work && fx && fy && fz,
+              "pmx_fft_c2r_poisson_grad: null pointer");
+  const Plan& p = P(plan);
+  float2* w = reinterpret_cast<float2*>(work);
+  return poisson_grad(p, reinterpret_cast<const float2*>(dk), 0, w,
+                      w + p.d.S, w + 2 * p.d.S, fx, fy, fz, scale, S(stream));
+}
+
+int pmx_force_meshes(const pmx_plan* plan, const float* rho, float* work,
+                     float* fx, float* fy, float* fz, float rho_scale,
+                     void* stream) {
+  PMX_REQUIRE(plan && rho && work && fx && fy && fz,
+              "pmx_force_meshes: null pointer");
+  return force_meshes(P(plan), rho, reinterpret_cast<float2*>(work), fx, fy,
+                      fz, rho_scale, S(stream));
+}
+
+int pmx_force_meshes_adj(const pmx_plan* plan, const float* gfx,
+                         const float* gfy, const float* gfz, float* work,
+                         float* grho, float rho_scale, void* stream) {
+  PMX_REQUIRE(plan && gfx && gfy && gfz && work && grho,
+              "pmx_force_meshes_adj: null pointer");
+  return force_meshes_adj(P(plan), gfx, gfy, gfz,
+                          reinterpret_cast<float2*>(work), grho, rho_scale,
+                          S(stream));
+}
+
+int pmx_fft_poisson_grad_adj(const pmx_plan* plan, const float* gfx,
+                             const float* gfy, const float* gfz, float* work,
+                             float* gdk, float scale, void* stream) {
+  PMX_REQUIRE(plan && gfx && gfy && gfz && work && gdk,
+              "pmx_fft_poisson_grad_adj: null pointer");
+  return poisson_grad_adj(P(plan), gfx, gfy, gfz,
+                          reinterpret_cast<float2*>(work),
+                          reinterpret_cast<float2*>(gdk), scale, S(stream));
+}
+
+long long pmx_lpt2_work_floats(const pmx_plan* plan) {
+  if (!plan) return -1;
+  const Plan& p = P(plan);
+  return 6 * 2 * p.d.S + 7 * p.d.rows * p.d.nz;
+}
+
+int pmx_lpt2_source(const pmx_plan* plan, const float* dk, float* work,
+                    float* src_k, void* stream) {
+  PMX_REQUIRE(plan && dk && work && src_k, "pmx_lpt2_source: null pointer");
+  const Plan& p = P(plan);
+  hipStream_t s = S(stream);
+  float2* w = reinterpret_cast<float2*>(work);
+  float2* ws[6];
+  for (int i = 0; i < 6; ++i) ws[i] = w + i * p.d.S;
+  const long long M = p.d.rows * p.d.nz;
+  float* r = work + 6 * 2 * p.d.S;
+  float* phi[7];
+  for (int i = 0; i < 7; ++i) phi[i] = r + i * M;
+  // phi_ij = c2r(dk lap (i g_i)(i g_j)); x pass splits by the x exponent
+  XJob xj{};
+  xj.nin = 1;
+  xj.nout = 3;
+  xj.in[0] = reinterpret_cast<const float2*>(dk);
+  for (int e = 0; e < 3; ++e) {
+    xj.out[e] = ws[e];
+    xj.term[0][e] = term(1.0f, 1, e, 0, 0);
+  }
+  xj.do_fwd = 0;
+  xj.do_inv = 1;
+  PMX_TRY(launch_xpass(p, xj, s));
+  YJobs yj{};
+  yj.n = 3;
+  // from ex=0: yy (ey=2) -> ws3, yz (ey=1) -> ws4, zz (ey=0) -> ws0
+  yj.j[0].in = ws[0];
+  yj.j[0].nout = 3;
+  yj.j[0].out[0] = ws[3];
+  yj.j[0].ey[0] = 2;
+  yj.j[0].out[1] = ws[4];
+  yj.j[0].ey[1] = 1;
+  yj.j[0].out[2] = ws[0];
+  yj.j[0].ey[2] = 0;
+  // from ex=1: xy (ey=1) -> ws5, xz (ey=0) -> ws1
+  yj.j[1].in = ws[1];
+  yj.j[1].nout = 2;
+  yj.j[1].out[0] = ws[5];
+  yj.j[1].ey[0] = 1;
+  yj.j[1].out[1] = ws[1];
+  yj.j[1].ey[1] = 0;
+  // from ex=2: xx -> ws2
+  yj.j[2].in = ws[2];
+  yj.j[2].nout = 1;
+  yj.j[2].out[0] = ws[2];
+  yj.j[2].ey[0] = 0;
+  for (int j = 0; j < 3; ++j)
+    for (int o = 0; o < 3; ++o) yj.j[j].scale[o] = 1.0f;
+  PMX_TRY(launch_ypass(p, yj, 1, s));
+  // phi: 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz
+  ZJobs zj{};
+  zj.n = 3;
+  zj.j[0] = zc2r(ws[2], phi[0], 1.0f, 0);
+  zj.j[1] = zc2r(ws[3], phi[1], 1.0f, 0);
+  zj.j[2] = zc2r(ws[0], phi[2], 1.0f, 2);
+  PMX_TRY(launch_zpass_c2r(p, zj, s));
+  zj.j[0] = zc2r(ws[5], phi[3], 1.0f, 0);
+  zj.j[1] = zc2r(ws[1], phi[4], 1.0f, 1);
+  zj.j[2] = zc2r(ws[4], phi[5], 1.0f, 1);
+  PMX_TRY(launch_zpass_c2r(p, zj, s));
+  hipLaunchKernelGGL(k_lpt2_combine, dim3(gridfor(M)), dim3(256), 0, s,
+                     phi[6], phi[0], phi[1], phi[2], phi[3], phi[4], phi[5],
+                     M);
+  PMX_LAUNCH_CHECK();
+  const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
+  return pmx_fft_r2c(plan, phi[6], src_k, (float)(1.0 / (double)Mb), stream);
+}
+
+int pmx_lpt_assemble(float* state, const float* dx1, const float* dx2,
+                     const double* c, int B, int nx, int ny, int nz,
+                     void* stream) {
+  PMX_REQUIRE(state && dx1 && c, "pmx_lpt_assemble: null pointer");
+  CHECK_MESH(B, 0, nx, ny, nz);
+  const long long M = (long long)nx * ny * nz;
+  const long long BM = (long long)B * M;
+  const float* d2x = dx2;
+  const float* d2y = dx2 ? dx2 + BM : nullptr;
+  const float* d2z = dx2 ? dx2 + 2 * BM : nullptr;
+  hipLaunchKernelGGL(k_lpt_assemble, dim3(gridfor(BM)), dim3(256), 0,
+                     S(stream), state, dx1, dx1 + BM, dx1 + 2 * BM, d2x, d2y,
+                     d2z, B, nx, ny, nz, (float)c[0], (float)c[1],
+                     (float)c[2], (float)c[3], (float)c[4], (float)c[5]);
+  PMX_LAUNCH_CHECK();
+  return 0;
+}
+
+int pmx_lpt_assemble_adj(const float* gstate, const double* c, float* g1,
+                         float* g2, int B, int nx, int ny, int nz,
+                         void* stream) {
+  PMX_REQUIRE(gstate && g1 && c, "pmx_lpt_assemble_adj: null pointer");
+  CHECK_MESH(B, 0, nx, ny, nz);
+  const long long BM = (long long)B * nx * ny * nz;
+  hipLaunchKernelGGL(k_lpt_assemble_adj, dim3(gridfor(BM)), dim3(256), 0,
+                     S(stream), gstate, g1, g2, BM, (float)c[0], (float)c[1],
+                     (float)c[2], (float)c[3], (float)c[4], (float)c[5]);
+  PMX_LAUNCH_CHECK();
+  return 0;
+}
+
+int pmx_lpt2_source_adj(const pmx_plan* plan, const float* work,
+                        const float* gsrc, float* gphi, void* stream) {
+  PMX_REQUIRE(plan && work && gsrc && gphi,
+              "pmx_lpt2_source_adj: null pointer");
+  const Plan& p = P(plan);
+  const long long M = p.d.rows * p.d.nz;
+  const float* phi = work + 6 * 2 * p.d.S;
+  hipLaunchKernelGGL(k_lpt2_combine_adj, dim3(gridfor(M)), dim3(256), 0,
+                     S(stream), phi, gsrc, gphi, M);
+  PMX_LAUNCH_CHECK();
+  return 0;
+}
+
+int pmx_fourier_apply(const pmx_plan* plan, int nin, const float* const* in,
+                      const int* e, const float* scales, int lap,
+                      float in_scale, float* work, float* out, int accumulate,
+                      void* stream) {
+  PMX_REQUIRE(plan && in && e && scales && work && out,
+              "pmx_fourier_apply: null pointer");
+  PMX_REQUIRE(nin >= 1 && nin <= 3, "pmx_fourier_apply: 1 <= nin <= 3");
+  Term t[3];
+  for (int q = 0; q < nin; ++q) {
+    PMX_REQUIRE(in[q] != nullptr, "pmx_fourier_apply: null input");
+    for (int d = 0; d < 3; ++d)
+      PMX_REQUIRE(e[3 * q + d] >= 0 && e[3 * q + d] <= 2,
+                  "pmx_fourier_apply: exponents must be 0, 1 or 2");
+    t[q] = term(scales[q], lap ? 1 : 0, e[3 * q], e[3 * q + 1], e[3 * q + 2]);
+  }
+  return fourier_apply(P(plan), nin, in, t, in_scale,
+                       reinterpret_cast<float2*>(work), out, accumulate,
+                       S(stream));
+}
+
+int pmx_spec_mul_real(const pmx_plan* plan, float* spec, const float* amp,
+                      void* stream) {
+  PMX_REQUIRE(plan && spec && amp, "pmx_spec_mul_real: null pointer");
+  hipLaunchKernelGGL(k_spec_mul_real, dim3(gridfor(P(plan).d.S)), dim3(256),
+                     0, S(stream), reinterpret_cast<float2*>(spec), amp,
+                     P(plan).d);
+  PMX_LAUNCH_CHECK();
+  return 0;
+}
+
+int pmx_readout_kick_drift(float* X, float* P, float* F, const float* fx,
+                           const float* fy, const float* fz, int B,
+                           long long N, int nx, int ny, int nz, float f,
+                           float fac, float k2, float k1, float d, int flags,
+                           void* stream) {
+  CHECK_MESH(B, N, nx, ny, nz);
+  PMX_REQUIRE(X && P && F, "pmx_readout_kick_drift: null state pointer");
+  MeshDims md{B, nx, ny, nz, N};
+  if (flags & 4) {  // kick/drift only, from the stored force
+    KickDrift c{};
+    c.k1 = k1;
+    c.d = d;
+    c.k2 = k2;
+    c.do_k2 = flags & 1;
+    c.do_k1d = (flags >> 1) & 1;
+    return launch_kick_drift_only(X, P, F, md, c, S(stream));
+  }
+  PMX_REQUIRE(fx && fy && fz, "pmx_readout_kick_drift: null mesh pointer");
+  KickDrift c{};
+  c.fac = fac;
+  c.f = f;
+  c.k2 = k2;
+  c.k1 = k1;
+  c.d = d;
+  c.do_k2 = flags & 1;
+  c.do_k1d = (flags >> 1) & 1;
+  return launch_readout_kick_drift(X, P, F, fx, fy, fz, md, c, S(stream));
+}
+
+long long pmx_nbody_work_floats(const pmx_plan* plan) {
+  return plan ? nbody_work_floats(P(plan)) : -1;
+}
+
+int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
+                  float fac, const double* coef, int b0, int b1, int jump,
+                  float* work, void* stream) {
+  PMX_REQUIRE(plan && state && work, "pmx_nbody_run: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_nbody_run: need at least one particle");
+  PMX_REQUIRE(b0 >= 0 && b1 >= b0 && (coef || b1 == b0),
+              "pmx_nbody_run: bad step range");
+  return nbody_run(P(plan), state, N, f, fac, coef, b0, b1, jump, work,
+                   S(stream));
+}
+
+long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N) {
+  return plan ? nbody_adj_work_floats(P(plan), N) : -1;
+}
+
+int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
+                      int nsteps, long long N, float f, float fac,
+                      const double* coef, float* gstate, float* xslots,
+                      float* work, void* stream) {
+  PMX_REQUIRE(plan && ckpt && gstate && xslots && work,
+              "pmx_nbody_adjoint: null pointer");
+  PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
+              "pmx_nbody_adjoint: bad steps");
+  return nbody_adjoint(P(plan), ckpt, every, nsteps, N, f, fac, coef, gstate,
+                       xslots, work, S(stream));
+}
+
+}  // extern "C"

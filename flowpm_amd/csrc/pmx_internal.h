@@ -1,0 +1,193 @@
+// Internal declarations shared by the pmx HIP sources (gfx950 only).
+//
+// Data layout in HBM
+//   real mesh      [B][nx][ny][nz] float, z fastest (flowpm/utils.py:32-91)
+//   particles      [B][N][3] float, Lagrangian C-order (flowpm/tfpm.py:161-164)
+//   state          [3][B][N][3] float = X, P, F (flowpm/tfpm.py:179-180)
+//   spectrum       half spectrum split in two blocks of one allocation:
+//                    main [B][nx][ny][nz/2] float2   (kz = 0 .. nz/2-1)
+//                    nyq  [B][nx][ny]       float2   (kz = nz/2)
+//                  Every line transform then reads/writes whole 128-byte
+//                  segments: y and x passes tile 16 consecutive kz columns,
+//                  the Nyquist plane is tiled along its own contiguous axis.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace pmx {
+
+// ------------------------------------------------------------ errors ------
+void set_error(const std::string& msg);
+const char* last_error();
+
+enum Status : int {
+  kOk = 0,
+  kBadArg = -1,
+  kUnsupported = -2,
+  kHipError = -3,
+  kNoMemory = -4,
+};
+
+#define PMX_HIP_CHECK(expr)                                                 \
+  do {                                                                      \
+    hipError_t _e = (expr);                                                 \
+    if (_e != hipSuccess) {                                                 \
+      ::pmx::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));  \
+      return ::pmx::kHipError;                                              \
+    }                                                                       \
+  } while (0)
+
+#define PMX_LAUNCH_CHECK()                                                  \
+  do {                                                                      \
+    hipError_t _e = hipGetLastError();                                      \
+    if (_e != hipSuccess) {                                                 \
+      ::pmx::set_error(std::string("kernel launch: ") +                    \
+                       hipGetErrorString(_e));                             \
+      return ::pmx::kHipError;                                              \
+    }                                                                       \
+  } while (0)
+
+#define PMX_REQUIRE(cond, msg)                                              \
+  do {                                                                      \
+    if (!(cond)) {                                                          \
+      ::pmx::set_error(msg);                                                \
+      return ::pmx::kBadArg;                                                \
+    }                                                                       \
+  } while (0)
+
+#define PMX_TRY(expr)                                                       \
+  do {                                                                      \
+    int _s = (expr);                                                        \
+    if (_s != 0) return _s;                                                 \
+  } while (0)
+
+// ------------------------------------------------------ plan (device) ------
+// Per-axis tables, computed on the host in float64 and uploaded once.
+struct AxisTab {
+  const float2* tw;  // [n]   exp(-2 pi i t / n)
+  const double* k2;  // [n]   (2 pi fftfreq(n))^2          kernels.py:10-25
+  const float* g;    // [n]   (8 sin w - sin 2w) / 6       kernels.py:51-78
+};
+
+struct PlanDev {
+  int B, nx, ny, nz, H;  // H = nz / 2
+  long long S;           // float2 elements in one spectral buffer
+  long long nyq0;        // offset of the Nyquist block = B*nx*ny*H
+  long long rows;        // B*nx*ny
+  AxisTab ax[3];
+};
+
+struct Plan {
+  PlanDev d;
+  void* dev_mem = nullptr;  // all tables in one allocation
+  int tw_y = 16, tw_x = 16, tw_z = 16;  // column-tile widths (autotuned)
+};
+
+// Multiplier applied in Fourier space:
+//   scale * [lap(k) if lap] * prod_d (i g_d(k_d))^e_d
+// lap(k) = 1/(kx^2+ky^2+kz^2) evaluated in float64 and rounded to float,
+// with the zero mode set to 1 as in kernels.py:42-47.
+struct Term {
+  float scale;
+  int lap;
+  int e[3];
+};
+
+constexpr int kMaxJobs = 3;
+
+// Z pass (contiguous rows): real -> half spectrum, or back.
+struct ZJob {
+  const float* rin;   // r2c input
+  float2* cout;       // r2c output
+  const float2* cin;  // c2r input
+  float* rout;        // c2r output
+  float scale;
+  int ez;             // c2r: exponent of (i g_z) applied before the inverse
+  int accum;          // c2r: rout += result instead of rout = result
+};
+struct ZJobs {
+  ZJob j[kMaxJobs];
+  int n;
+};
+
+// Column pass along y: forward in place, or inverse with fan-out.
+struct YJob {
+  const float2* in;
+  float2* out[kMaxJobs];
+  float scale[kMaxJobs];
+  int ey[kMaxJobs];
+  int nout;
+};
+struct YJobs {
+  YJob j[kMaxJobs];
+  int n;
+};
+
+// Column pass along x, fused with the Fourier-space multiply:
+//   out_o = [IFFT_x] sum_q Term(q,o) * [FFT_x] in_q     (nin == 1 or nout == 1)
+struct XJob {
+  const float2* in[kMaxJobs];
+  float2* out[kMaxJobs];
+  Term term[kMaxJobs][kMaxJobs];  // [q][o]
+  int nin, nout;
+  int do_fwd, do_inv;
+};
+
+// Host launchers (fft.hip).  All take the caller's stream.
+int launch_zpass_r2c(const Plan& p, const ZJobs& jobs, hipStream_t s);
+int launch_zpass_c2r(const Plan& p, const ZJobs& jobs, hipStream_t s);
+int launch_ypass(const Plan& p, const YJobs& jobs, int inverse, hipStream_t s);
+int launch_xpass(const Plan& p, const XJob& job, hipStream_t s);
+int launch_spec_to_full(const Plan& p, const float2* spec, float2* full,
+                        hipStream_t s);
+int launch_full_to_spec(const Plan& p, const float2* full, float2* spec,
+                        hipStream_t s);
+bool fft_size_supported(int n);
+
+// CIC launchers (cic.hip).
+struct MeshDims {
+  int B, nx, ny, nz;
+  long long N;  // particles per batch member
+};
+int launch_fill(float* p, long long n, float v, hipStream_t s);
+int launch_paint(float* mesh, const float* pos, const float* w, float wscale,
+                 const MeshDims& d, float pscale, int mode, hipStream_t s);
+int launch_readout(float* out, const float* mesh, const float* pos,
+                   const MeshDims& d, float pscale, hipStream_t s);
+int launch_paint_adj(float* dpos, float* dw, const float* gmesh,
+                     const float* pos, const float* w, const MeshDims& d,
+                     float pscale, float gscale, int accumulate,
+                     hipStream_t s);
+int launch_readout_adj(float* gmesh, float* dpos, const float* gout,
+                       const float* mesh, const float* pos, const MeshDims& d,
+                       float pscale, int accumulate, hipStream_t s);
+
+// Fused particle update: F = readout(meshes)*fac/f ; P += k2 F ;
+// then (optional) P += k1 F ; X += d P      (tfpm.py:273-366, 475-498)
+struct KickDrift {
+  float fac;      // 1.5 Omega_m
+  float f;        // pm_nc_factor
+  float k2;       // closing kick of this step
+  float k1;       // opening kick of the next step
+  float d;        // drift of the next step
+  int do_k2, do_k1d;
+};
+int launch_readout_kick_drift(float* X, float* P, float* F,
+                              const float* fx, const float* fy,
+                              const float* fz, const MeshDims& d,
+                              const KickDrift& c, hipStream_t s);
+
+int launch_kick_drift_only(float* X, float* P, const float* F,
+                           const MeshDims& d, const KickDrift& c,
+                           hipStream_t s);
+
+// Force VJP pieces.
+int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz,
+                               float* dX, const float* gF, const float* X,
+                               const float* fx, const float* fy,
+                               const float* fz, const MeshDims& d, float f,
+                               float c, hipStream_t s);
+
+}  // namespace pmx

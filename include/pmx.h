@@ -1,0 +1,190 @@
+/* pmx -- MI355X (gfx950) particle-mesh step with FlowPM semantics.
+ *
+ * C ABI of libpmx_hip.so.  Plain pointers and sizes only: every float* is a
+ * device pointer (HBM), every `stream` is a hipStream_t passed as void*.
+ * Every entry point returns 0 on success or a negative status; the message
+ * of the last failure on the calling thread is pmx_last_error().  The
+ * library never allocates memory visible to the caller: scratch comes from
+ * caller-provided workspaces whose sizes the library reports; a plan owns
+ * only its small float64-derived tables.
+ *
+ * The reference (DifferentiableUniverseInitiative/flowpm) has no native
+ * layer or FFI: its path is Python over TensorFlow ops.  Each entry point
+ * below names the reference function / TF op it replaces.
+ *
+ * Layouts: mesh [B][nx][ny][nz] float; particles [B][N][3] float;
+ * state [3][B][N][3] float (X, P, F).  "spec" buffers hold the half
+ * spectrum in the library's layout: pmx_plan_spec_elems() float2 values
+ * (main block [B][nx][ny][nz/2] followed by the kz=nz/2 plane [B][nx][ny]).
+ */
+#ifndef PMX_H_
+#define PMX_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMX_VERSION 1000
+
+typedef struct pmx_plan pmx_plan;
+
+int pmx_version(void);
+const char* pmx_last_error(void);
+
+/* Plan for a batch of B meshes of nx*ny*nz cells (each in
+ * {8,12,16,24,32,48,64,96,128,192,256,384,512,768,1024}, nz even).
+ * Holds the twiddles, (2 pi fftfreq)^2 and 4th-order gradient tables
+ * (flowpm/kernels.py:10-25, :28-78) computed in float64. */
+int pmx_plan_create(pmx_plan** plan, int B, int nx, int ny, int nz);
+int pmx_plan_destroy(pmx_plan* plan);
+long long pmx_plan_spec_elems(const pmx_plan* plan);
+/* Column-tile widths (float2 columns) of the y and x line passes; the
+ * autotuner sets them.  Allowed: 8 or 16 (16 only for lines <= 512). */
+int pmx_plan_set_tiles(pmx_plan* plan, int tile_y, int tile_x);
+
+/* ---- CIC: flowpm/utils.py:32-91 (cic_paint), :163-210 (cic_readout) ---- */
+/* mesh += paint(pos * pos_scale, weight)  (weight may be NULL = ones) */
+int pmx_cic_paint(float* mesh, const float* pos, const float* weight, int B,
+                  long long N, int nx, int ny, int nz, float pos_scale,
+                  void* stream);
+/* out[b][p] = sum_corners mesh * w(pos * pos_scale) */
+int pmx_cic_readout(float* out, const float* mesh, const float* pos, int B,
+                    long long N, int nx, int ny, int nz, float pos_scale,
+                    void* stream);
+/* VJP of cic_paint w.r.t. positions and weights for mesh cotangent gmesh
+ * (dpos or dweight may be NULL). */
+int pmx_cic_paint_adj(float* dpos, float* dweight, const float* gmesh,
+                      const float* pos, const float* weight, int B,
+                      long long N, int nx, int ny, int nz, float pos_scale,
+                      void* stream);
+/* VJP of cic_readout: gmesh += paint(pos, gout) and dpos (either may be
+ * NULL). */
+int pmx_cic_readout_adj(float* gmesh, float* dpos, const float* gout,
+                        const float* mesh, const float* pos, int B,
+                        long long N, int nx, int ny, int nz, float pos_scale,
+                        void* stream);
+
+/* ---- FFT: flowpm/utils.py:213-241 (r2c3d), :244-272 (c2r3d) ---- */
+/* spec = fft3d(in) * scale, half spectrum (r2c3d with scale = 1/norm) */
+int pmx_fft_r2c(const pmx_plan* plan, const float* in, float* spec,
+                float scale, void* stream);
+/* out = real(ifft3d(spec)) * M * scale (c2r3d with scale = norm/M);
+ * spec is overwritten. */
+int pmx_fft_c2r(const pmx_plan* plan, float* spec, float* out, float scale,
+                void* stream);
+/* half spectrum <-> the reference's full (B,nx,ny,nz) complex64 layout;
+ * full_to_spec keeps the Hermitian part, so that c2r of the result equals
+ * the real part of the full inverse. */
+int pmx_spec_to_full(const pmx_plan* plan, const float* spec, float* full,
+                     void* stream);
+int pmx_full_to_spec(const pmx_plan* plan, const float* full, float* spec,
+                     void* stream);
+
+/* ---- Fourier-space Poisson x gradient: kernels.py:28-102 used by
+ * tfpm.py:66-95 (lpt1) and :183-219 (apply_longrange) ----
+ * out_d = c2r3d(dk * laplace_kernel * gradient_kernel(d) * scale) for
+ * d = x, y, z.  dk is a half spectrum (not modified); work holds 3 spec
+ * buffers. */
+int pmx_fft_c2r_poisson_grad(const pmx_plan* plan, const float* dk,
+                             float* work, float* fx, float* fy, float* fz,
+                             float scale, void* stream);
+/* Fused density -> force meshes: r2c3d(rho * rho_scale) followed by the
+ * Poisson x gradient inverse above, with the forward x transform, the
+ * multiply and the inverse x transform in one pass (tfpm.py:344-354). */
+int pmx_force_meshes(const pmx_plan* plan, const float* rho, float* work,
+                     float* fx, float* fy, float* fz, float rho_scale,
+                     void* stream);
+/* Its adjoint: grho = sum_d T_d^T g_d, T_d^T = c2r(conj(lap i g_d) r2c(.)),
+ * the conjugate transform (tfpm.py:183-219 reversed). */
+int pmx_force_meshes_adj(const pmx_plan* plan, const float* gfx,
+                         const float* gfy, const float* gfz, float* work,
+                         float* grho, float rho_scale, void* stream);
+/* Adjoint of pmx_fft_c2r_poisson_grad: gdk (half spectrum, adjoint in the
+ * real-linear sense) from the three mesh cotangents.  work: 3 spec. */
+int pmx_fft_poisson_grad_adj(const pmx_plan* plan, const float* gfx,
+                             const float* gfy, const float* gfz, float* work,
+                             float* gdk, float scale, void* stream);
+
+/* ---- second-order LPT source: tfpm.py:98-145 ----
+ * src_k = r2c3d(3/7 [sum phi_ii phi_jj - sum phi_ij^2]) from dk.
+ * work: 6 spec buffers followed by 7 real meshes. */
+int pmx_lpt2_source(const pmx_plan* plan, const float* dk, float* work,
+                    float* src_k, void* stream);
+long long pmx_lpt2_work_floats(const pmx_plan* plan);
+
+/* ---- lpt_init assembly: tfpm.py:148-180 ----
+ * state[0] = Q + c[0] dx1 + c[3] dx2, state[1] = c[1] dx1 + c[4] dx2,
+ * state[2] = c[2] dx1 + c[5] dx2 (dx2 may be NULL), dx meshes at the
+ * particle grid (B, nx, ny, nz) with Q = np.indices order. */
+int pmx_lpt_assemble(float* state, const float* dx1, const float* dx2,
+                     const double* coef6, int B, int nx, int ny, int nz,
+                     void* stream);
+
+/* Adjoint of pmx_lpt_assemble: g1 (3 meshes, x/y/z) = c[0] (gX + c[1] gP +
+ * c[2] gF) per component, and likewise g2 with c[3..5] (g2 may be NULL). */
+int pmx_lpt_assemble_adj(const float* gstate, const double* coef6, float* g1,
+                         float* g2, int B, int nx, int ny, int nz,
+                         void* stream);
+/* Adjoint of the pointwise source combination inside pmx_lpt2_source:
+ * work is the buffer a pmx_lpt2_source call filled (its phi_ij meshes);
+ * gsrc is the cotangent of the real-space source (before its r2c); gphi
+ * receives 6 meshes (xx, yy, zz, xy, xz, yz). */
+int pmx_lpt2_source_adj(const pmx_plan* plan, const float* work,
+                        const float* gsrc, float* gphi, void* stream);
+
+/* Generic Fourier filter used by the adjoints of lpt_init and force:
+ * out (+)= c2r3d( sum_q scales[q] [laplace_kernel] prod_d
+ *          gradient_kernel(d)^e[3q+d] r2c3d(in[q]) * in_scale ),
+ * nin <= 3 real meshes, exponents 0..2 (kernels.py:28-78).
+ * work: 3 spec buffers. */
+int pmx_fourier_apply(const pmx_plan* plan, int nin, const float* const* in,
+                      const int* e, const float* scales, int lap,
+                      float in_scale, float* work, float* out, int accumulate,
+                      void* stream);
+
+/* spec *= amp, amp a real half-spectrum array of one mesh (nx*ny*nz/2 main
+ * values then nx*ny Nyquist-plane values), applied to every batch member:
+ * the sqrt(P(k)/V) weighting of tfpm.py:56-62. */
+int pmx_spec_mul_real(const pmx_plan* plan, float* spec, const float* amp,
+                      void* stream);
+
+/* ---- fused readout + kick + drift: tfpm.py:273-314, 352-366 ----
+ * F = readout(f{x,y,z}, X*f) * fac / f; then, by flags,
+ *   bit0: P += k2 F      bit1: P += k1 F ; X += d P
+ *   bit2: skip the readout and use F as stored (kick/drift only). */
+int pmx_readout_kick_drift(float* X, float* P, float* F, const float* fx,
+                           const float* fy, const float* fz, int B,
+                           long long N, int nx, int ny, int nz, float f,
+                           float fac, float k2, float k1, float d, int flags,
+                           void* stream);
+
+/* ---- nbody time loop: tfpm.py:425-503 (force :317-366) ----
+ * Runs steps [step_begin, step_end) of the FastPM KDK scheme on state
+ * (3,B,N,3) in place.  coef[3*i + {0,1,2}] = (k1, d, k2) of step i: opening
+ * kick, drift and closing kick factors (float64, from the growth tables).
+ * jump_start != 0 first evaluates force(state) as tfpm.py:470 does.
+ * The plan is the force mesh (nc * pm_nc_factor); f = pm_nc_factor.
+ * work: pmx_nbody_work_floats() floats. */
+long long pmx_nbody_work_floats(const pmx_plan* plan);
+int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
+                  float fac, const double* coef, int step_begin, int step_end,
+                  int jump_start, float* work, void* stream);
+
+/* Reverse sweep (hand-written adjoint with recompute).  ckpt holds the
+ * states at the start of every segment of `every` steps (segment s starts
+ * at step s*every; segment 0 at step 0 *before* the jump-start force);
+ * gstate (3,B,N,3) holds the cotangent of the final state on entry and the
+ * cotangent of the initial state on exit.  xslots: every*B*N*3 floats;
+ * work: pmx_nbody_adj_work_floats() floats. */
+long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N);
+int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
+                      int nsteps, long long N, float f, float fac,
+                      const double* coef, float* gstate, float* xslots,
+                      float* work, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMX_H_ */
