@@ -1,0 +1,232 @@
+#!/usr/bin/env python
+"""PM-step benchmark on one MI355X.
+
+Workload: a single simulation with 256^3 particles on a 256^3 force mesh,
+box 512 Mpc/h, Planck15 Eisenstein-Hu P(k), seeded Gaussian initial
+conditions, 2LPT at a0 = 0.1, 40 linear-in-a KDK steps to a = 1, float32.
+A "step" is one PM step (paint, forward FFT, Poisson x gradient, three
+inverse FFTs, fused readout + kick + drift).
+
+Reports (one JSON line):
+  value / ms_per_step      forward ms per PM step (K timed steps, including
+                           the jump-start force of nbody)
+  fwd_adjoint_ms_per_step  forward with checkpoints every 8 steps + the
+                           hand-written adjoint with recomputation, per step
+  roofline                 dominant kernel: algorithmic bytes per launch /
+                           average launch time from HIP events recorded on
+                           the launch stream during a forward run
+  cpu_baseline             the CPU oracle (torch, float32, the reference's
+                           op graph restated) timed for one PM step at the
+                           same size on this host's cores
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("ms/PM step fwd and fwd+adjoint at 256^3; batched sims/sec at "
+          "64^3; HBM GB/s")
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(kind, n, N):
+  """Bytes each kernel must read + write once, per launch (B = 1)."""
+  M = n**3
+  S8 = 8 * n * n * (n // 2 + 1)  # one half-spectrum buffer, bytes
+  table = {
+      "fill": 4 * M,
+      "paint": 12 * N + 4 * M,
+      "fft_z_r2c": 4 * M + S8,
+      "fft_y_fwd": 2 * S8,
+      "fft_x_fused": S8 + 2 * S8,
+      "fft_y_inv": 2 * S8 + 3 * S8,
+      "fft_z_c2r": 3 * S8 + 3 * 4 * M,
+      "readout_kick_drift": 3 * 4 * M + 24 * N + 36 * N,
+  }
+  return table.get(kind)
+
+
+def main():
+  ap = argparse.ArgumentParser()
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=40)
+  ap.add_argument("--warmup", type=int, default=2)
+  ap.add_argument("--n", type=int, default=256)
+  ap.add_argument("--box", type=float, default=512.0)
+  ap.add_argument("--every", type=int, default=8)
+  ap.add_argument("--no-adjoint", action="store_true")
+  ap.add_argument("--no-cpu-baseline", action="store_true")
+  args = ap.parse_args()
+  if args.gpus != 1:
+    print("bench.py measures one GPU: this build has no multi-GPU path "
+          "(simulations shard by batch member with no communication; see "
+          "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+    sys.exit(2)
+
+  import flowpm_amd as fa
+  from flowpm_amd import _lib
+  from flowpm_amd import _plans
+
+  dev = torch.device("cuda:0")
+  n, L, K, W = args.n, args.box, args.steps, args.warmup
+  N = n**3
+  cosmo = fa.cosmology.Planck15()
+  kk = np.logspace(-4, 1, 256)
+  pk_tab = fa.tfpower.linear_matter_power(cosmo, kk)
+  pk = lambda k: fa.scipy.interpolate.interp_tf(k, kk, pk_tab)
+  gen = torch.Generator().manual_seed(100)
+  white = (torch.randn(1, n, n, n, generator=gen, dtype=torch.float64) *
+           n**1.5).float().to(dev)
+  lin = fa.whitenoise_to_linear(n, L, white, pk)
+  state0 = fa.lpt_init(cosmo, lin, 0.1, order=2)
+  stages = np.linspace(0.1, 1.0, K + 1)
+  coef = fa.tfpm.step_coefficients(cosmo, stages)
+  plan = _plans.get(dev, 1, n, n, n)
+  fac = float(np.float32(1.5 * cosmo.Omega_m))
+  work = torch.empty(_lib.lib().pmx_nbody_work_floats(plan.handle),
+                     dtype=torch.float32, device=dev)
+  cptr = _lib.dbl(coef)
+  stream = _lib.stream_of(work)
+
+  def run(st, b0, b1, jump):
+    _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, 1.0, fac, cptr,
+              b0, b1, jump, _lib.ptr(work), stream)
+
+  # warmup: W steps on a copy
+  st = state0.clone()
+  run(st, 0, min(W, K), 1)
+  torch.cuda.synchronize()
+
+  # timed forward: K steps (jump-start force + K KDK steps)
+  st = state0.clone()
+  torch.cuda.synchronize()
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  e0.record()
+  run(st, 0, K, 1)
+  e1.record()
+  torch.cuda.synchronize()
+  fwd_ms = e0.elapsed_time(e1)
+  ms_step = fwd_ms / K
+
+  # live per-kernel timing on the launch stream (separate forward run)
+  st = state0.clone()
+  torch.cuda.synchronize()
+  _lib.call("pmx_profile_start")
+  run(st, 0, K, 1)
+  nk = _lib.lib().pmx_profile_kinds()
+  ms = (_lib._c.c_double * nk)()
+  cnt = (_lib._ll * nk)()
+  _lib.call("pmx_profile_stop", ms, cnt, nk)
+  kernels = {}
+  for k in range(nk):
+    if cnt[k]:
+      name = _lib.lib().pmx_profile_kind_name(k).decode()
+      kernels[name] = {"launches": int(cnt[k]),
+                       "avg_us": 1e3 * ms[k] / cnt[k],
+                       "total_ms": ms[k]}
+  dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+  alg = algorithmic_bytes(dom, n, N)
+  avg_s = kernels[dom]["avg_us"] * 1e-6
+  achieved = alg / avg_s / 1e9
+  for name, v in kernels.items():
+    b = algorithmic_bytes(name, n, N)
+    if b:
+      v["alg_bytes"] = b
+      v["GBps"] = b / (v["avg_us"] * 1e-6) / 1e9
+
+  # forward + adjoint (checkpoint every k, recompute in backward)
+  fa_ms = None
+  if not args.no_adjoint:
+    x0 = state0.clone().requires_grad_(True)
+    gseed = torch.randn(state0.shape, generator=torch.Generator().manual_seed(
+        101)).to(dev)
+    # warm the adjoint path
+    fin = fa.nbody(cosmo, x0, stages[:W + 1], n, checkpoint_every=args.every)
+    torch.autograd.grad(fin, x0, grad_outputs=gseed)
+    torch.cuda.synchronize()
+    e0.record()
+    fin = fa.nbody(cosmo, x0, stages, n, checkpoint_every=args.every)
+    (gx,) = torch.autograd.grad(fin, x0, grad_outputs=gseed)
+    e1.record()
+    torch.cuda.synchronize()
+    fa_ms = e0.elapsed_time(e1) / K
+    assert torch.isfinite(gx).all()
+
+  cpu = None
+  if not args.no_cpu_baseline:
+    cpu = cpu_baseline(state0, n, cosmo)
+
+  out = {
+      "metric": METRIC,
+      "value": ms_step,
+      "unit": "ms/step",
+      "n_gpus": 1,
+      "steps": K,
+      "warmup": W,
+      "ms_per_step": ms_step,
+      "higher_is_better": False,
+      "scaling": "weak",
+      "vs_baseline": None,
+      "dtype": "f32",
+      "data": "synthetic",
+      "config": {
+          "workload": "single sim, %d^3 particles, %d^3 force mesh, box %g "
+                      "Mpc/h, Planck15 EH P(k), 2LPT a0=0.1, %d KDK steps "
+                      "to a=1" % (n, n, L, K),
+          "mesh": n, "particles": N, "box_mpc_h": L, "lpt_order": 2,
+          "a0": 0.1, "checkpoint_every": args.every,
+      },
+      "fwd_adjoint_ms_per_step": fa_ms,
+      "adjoint_over_forward": (fa_ms / ms_step) if fa_ms else None,
+      "roofline": {
+          "bound": "hbm",
+          "kernel": dom,
+          "achieved": achieved,
+          "peak": HBM_PEAK_GBS,
+          "unit": "GB/s",
+          "frac": achieved / HBM_PEAK_GBS,
+          "traffic": None,
+      },
+      "step_alg_bytes": sum(
+          (algorithmic_bytes(k, n, N) or 0) * v["launches"] / (K + 1)
+          for k, v in kernels.items()),
+      "kernels": kernels,
+      "cpu_baseline": cpu,
+  }
+  print(json.dumps(out))
+
+
+def cpu_baseline(state0, n, cosmo):
+  """One PM step (force + kick + drift) of the same workload with the CPU
+  oracle in float32 on all host threads."""
+  from oracle import background as obg
+  from oracle import pm as opm
+  torch.set_num_threads(os.cpu_count() or 1)
+  g = obg.GrowthTable(obg.planck15(), dtype=np.float64)
+  st = state0.detach().cpu()
+  a0, a1 = 0.1, 0.1225
+  ah = (a0 * a1)**0.5
+  t0 = time.perf_counter()
+  st = opm.force(g, st, [n] * 3)
+  st = opm.kick(g, st, a0, a0, ah)
+  st = opm.drift(g, st, a0, ah, a1)
+  st = opm.force(g, st, [n] * 3)
+  st = opm.kick(g, st, ah, a1, a1)
+  dt = time.perf_counter() - t0
+  # the sample holds two force evaluations (jump start + the step's force)
+  return {"value": 1e3 * dt / 2, "unit": "ms/step",
+          "cores": torch.get_num_threads(), "kind": "port",
+          "sample": "1 KDK step (2 force evaluations) of the %d^3 workload "
+                    "with the float32 torch-CPU oracle" % n}
+
+
+if __name__ == "__main__":
+  main()

@@ -297,6 +297,7 @@ static inline unsigned nblk(long long n, int t) {
 int launch_fill(float* p, long long n, float v, hipStream_t s) {
   if (n <= 0) return kOk;
   const unsigned g = (unsigned)std::min<long long>(nblk(n, 256), 8192);
+  ProfScope prof_(kProfFill, s);
   hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, p, n, v);
   PMX_LAUNCH_CHECK();
   return kOk;
@@ -307,6 +308,7 @@ int launch_paint(float* mesh, const float* pos, const float* w, float wscale,
   (void)mode;
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
+  ProfScope prof_(kProfPaint, s);
   hipLaunchKernelGGL(k_paint, dim3(nblk(tot, 256)), dim3(256), 0, s, mesh,
                      pos, w, wscale, d, ps);
   PMX_LAUNCH_CHECK();
@@ -317,6 +319,7 @@ int launch_readout(float* out, const float* mesh, const float* pos,
                    const MeshDims& d, float ps, hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
+  ProfScope prof_(kProfReadout, s);
   hipLaunchKernelGGL(k_readout, dim3(nblk(tot, 256)), dim3(256), 0, s, out,
                      mesh, pos, d, ps);
   PMX_LAUNCH_CHECK();
@@ -328,6 +331,7 @@ int launch_paint_adj(float* dpos, float* dw, const float* g, const float* pos,
                      int accumulate, hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
+  ProfScope prof_(kProfAdjCic, s);
   hipLaunchKernelGGL(k_paint_adj, dim3(nblk(tot, 256)), dim3(256), 0, s, dpos,
                      dw, g, pos, w, d, ps, gscale, accumulate);
   PMX_LAUNCH_CHECK();
@@ -339,6 +343,7 @@ int launch_readout_adj(float* gmesh, float* dpos, const float* gout,
                        float ps, int accumulate, hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
+  ProfScope prof_(kProfAdjCic, s);
   hipLaunchKernelGGL(k_readout_adj, dim3(nblk(tot, 256)), dim3(256), 0, s,
                      gmesh, dpos, gout, mesh, pos, d, ps, accumulate);
   PMX_LAUNCH_CHECK();
@@ -351,6 +356,7 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const float* fx,
                               hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
+  ProfScope prof_(kProfReadoutKD, s);
   hipLaunchKernelGGL(k_readout_kick_drift, dim3(nblk(tot, 256)), dim3(256), 0,
                      s, X, P, F, fx, fy, fz, d, c);
   PMX_LAUNCH_CHECK();
@@ -362,6 +368,7 @@ int launch_kick_drift_only(float* X, float* P, const float* F,
                            hipStream_t s) {
   const long long n3 = (long long)d.B * d.N * 3;
   if (n3 == 0) return kOk;
+  ProfScope prof_(kProfKickDrift, s);
   hipLaunchKernelGGL(k_kick_drift_only, dim3(nblk(n3, 256)), dim3(256), 0, s,
                      X, P, F, n3, c);
   PMX_LAUNCH_CHECK();
@@ -375,6 +382,7 @@ int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz, float* dX,
                                float c, hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
+  ProfScope prof_(kProfAdjCic, s);
   hipLaunchKernelGGL(k_force_vjp_particles, dim3(nblk(tot, 256)), dim3(256),
                      0, s, gmx, gmy, gmz, dX, gF, X, fx, fy, fz, d, f, c);
   PMX_LAUNCH_CHECK();

@@ -621,6 +621,7 @@ template <int N>
 static int zr2c_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   constexpr int TW = Cfg<N>::ZTW, T = Cfg<N>::T;
   dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)), jobs.n);
+  ProfScope prof_(kProfZr2c, s);
   hipLaunchKernelGGL((k_zpass_r2c<N, TW, T>), grid, dim3(T), 0, s, jobs,
                      pl.d);
   PMX_LAUNCH_CHECK();
@@ -630,6 +631,7 @@ template <int N>
 static int zc2r_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   constexpr int TW = Cfg<N>::ZTW, T = Cfg<N>::T;
   dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)), jobs.n);
+  ProfScope prof_(kProfZc2r, s);
   hipLaunchKernelGGL((k_zpass_c2r<N, TW, T>), grid, dim3(T), 0, s, jobs,
                      pl.d);
   PMX_LAUNCH_CHECK();
@@ -642,6 +644,7 @@ static int y_n(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
   const int nkt = (p.H + TW - 1) / TW;
   const int tiles = p.B * p.nx * nkt + p.B * ((p.nx + TW - 1) / TW);
   dim3 grid(tiles, jobs.n);
+  ProfScope prof_(inv ? kProfYinv : kProfYfwd, s);
   if (inv)
     hipLaunchKernelGGL((k_ypass<N, TW, T, +1>), grid, dim3(T), 0, s, jobs, p);
   else
@@ -655,6 +658,7 @@ static int x_n(const Plan& pl, const XJob& job, hipStream_t s) {
   const PlanDev& p = pl.d;
   const int nkt = (p.H + TW - 1) / TW;
   const int tiles = p.B * p.ny * nkt + p.B * ((p.ny + TW - 1) / TW);
+  ProfScope prof_(kProfX, s);
   hipLaunchKernelGGL((k_xpass<N, TW, T>), dim3(tiles), dim3(T), 0, s, job, p);
   PMX_LAUNCH_CHECK();
   return kOk;

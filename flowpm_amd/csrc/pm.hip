@@ -18,6 +18,32 @@ const char* last_error() { return g_err.c_str(); }
 
 static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// ------------------------------------------------------ kernel timing ----
+bool g_prof_on = false;
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+static std::vector<ProfRec> g_prof;
+static std::mutex g_prof_mu;
+
+void prof_record(int kind, hipStream_t s, hipEvent_t* ev, bool end) {
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, s);
+  if (!end) {
+    *ev = e;
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.push_back({kind, *ev, e});
+}
+
+static const char* kProfNames[kProfKinds] = {
+    "fill", "paint", "fft_z_r2c", "fft_y_fwd", "fft_x_fused", "fft_y_inv",
+    "fft_z_c2r", "readout_kick_drift", "kick_drift", "readout", "cic_adjoint",
+    "other"};
+
 // ------------------------------------------------------------------ plan --
 static int plan_create(Plan** out, int B, int nx, int ny, int nz) {
   PMX_REQUIRE(out != nullptr, "plan_create: null output");
@@ -573,6 +599,47 @@ static inline const Plan& P(const pmx_plan* h) { return h->p; }
 extern "C" {
 
 int pmx_version(void) { return PMX_VERSION; }
+
+int pmx_profile_start(void) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (auto& r : g_prof) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  g_prof.clear();
+  g_prof_on = true;
+  return 0;
+}
+
+int pmx_profile_stop(double* ms, long long* count, int nkinds) {
+  g_prof_on = false;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (int k = 0; k < nkinds; ++k) {
+    if (ms) ms[k] = 0.0;
+    if (count) count[k] = 0;
+  }
+  int status = 0;
+  for (auto& r : g_prof) {
+    float t = 0.f;
+    if (hipEventSynchronize(r.b) != hipSuccess ||
+        hipEventElapsedTime(&t, r.a, r.b) != hipSuccess)
+      status = kHipError;
+    if (r.kind < nkinds) {
+      if (ms) ms[r.kind] += t;
+      if (count) count[r.kind] += 1;
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  g_prof.clear();
+  if (status) set_error("pmx_profile_stop: event query failed");
+  return status;
+}
+
+int pmx_profile_kinds(void) { return kProfKinds; }
+const char* pmx_profile_kind_name(int k) {
+  return (k >= 0 && k < kProfKinds) ? kProfNames[k] : "";
+}
 const char* pmx_last_error(void) { return last_error(); }
 
 int pmx_plan_create(pmx_plan** plan, int B, int nx, int ny, int nz) {

@@ -63,6 +63,38 @@ enum Status : int {
     if (_s != 0) return _s;                                                 \
   } while (0)
 
+// --------------------------------------------------- live kernel timing --
+// When enabled (pmx_profile_start), every launch is bracketed by a pair of
+// hipEvents on its own stream; pmx_profile_stop sums the durations by kind.
+enum ProfKind : int {
+  kProfFill = 0,
+  kProfPaint,
+  kProfZr2c,
+  kProfYfwd,
+  kProfX,
+  kProfYinv,
+  kProfZc2r,
+  kProfReadoutKD,
+  kProfKickDrift,
+  kProfReadout,
+  kProfAdjCic,
+  kProfOther,
+  kProfKinds
+};
+extern bool g_prof_on;
+void prof_record(int kind, hipStream_t s, hipEvent_t* ev, bool end);
+struct ProfScope {
+  int kind;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  ProfScope(int k, hipStream_t st) : kind(k), s(st) {
+    if (g_prof_on) prof_record(kind, s, &a, false);
+  }
+  ~ProfScope() {
+    if (g_prof_on && a) prof_record(kind, s, &a, true);
+  }
+};
+
 // ------------------------------------------------------ plan (device) ------
 // Per-axis tables, computed on the host in float64 and uploaded once.
 struct AxisTab {

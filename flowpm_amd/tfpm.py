@@ -1,0 +1,399 @@
+"""PM solver drop-ins for flowpm/tfpm.py on the HIP path.
+
+Signatures follow the reference (``name=`` accepted and ignored, ``nc`` an
+int or a list).  State tensors are (3, batch, npart, 3) float32 on the GPU:
+X (mesh units, never wrapped), P, F, as tfpm.py:179-180.  The growth and ODE
+coefficients are host float64 (tfbackground); all particle and mesh work runs
+in the HIP library (libpmx_hip.so):
+
+  linear_field   tfpm.py:15-63    r2c, * sqrt(P(k)/V), c2r (3 HIP passes each)
+  lpt1           tfpm.py:66-95    fused Poisson x gradient inverse + readout
+  lpt2_source    tfpm.py:98-145   six phi_ij by one fan-out inverse, source,
+                                  r2c
+  lpt_init       tfpm.py:148-180  differentiable (hand-written adjoint)
+  apply_longrange tfpm.py:183-219 (split = 0, as on the path)
+  kick / drift   tfpm.py:273-314
+  force          tfpm.py:317-366  paint -> fused r2c/Poisson/gradient/c2r ->
+                                  readout
+  nbody          tfpm.py:425-503  C++ time loop; one fused readout + kick +
+                                  kick + drift kernel per step; differentiable
+                                  with checkpoints every k steps and
+                                  recomputation in the backward pass
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from . import _plans
+from . import tfbackground as bg
+from .utils import cic_readout, white_noise, _f32, _half_to_full, \
+    _full_to_real_half, fwd_real, inv_real
+
+__all__ = ["linear_field", "whitenoise_to_linear", "lpt1", "lpt2_source",
+           "lpt_init", "apply_longrange", "kick", "drift", "force", "nbody",
+           "step_coefficients", "set_checkpoint_every"]
+
+_CKPT_EVERY = [8]
+
+
+def set_checkpoint_every(k):
+  """Trajectory checkpoint interval used by differentiable nbody calls."""
+  if int(k) < 1:
+    raise ValueError("checkpoint interval must be >= 1")
+  _CKPT_EVERY[0] = int(k)
+
+
+def _nc3(nc):
+  if isinstance(nc, (int, np.integer)):
+    return [int(nc)] * 3
+  return [int(n) for n in nc]
+
+
+# -------------------------------------------------------- initial field --
+def _half_kmesh(nc, boxsize):
+  """|k| (h/Mpc) on the library's half-spectrum layout of one mesh:
+  nx*ny*(nz/2) main values then the nx*ny values of kz = nz/2."""
+  kx = 2 * np.pi * np.fft.fftfreq(nc[0]) / boxsize[0] * nc[0]
+  ky = 2 * np.pi * np.fft.fftfreq(nc[1]) / boxsize[1] * nc[1]
+  kz = 2 * np.pi * np.fft.fftfreq(nc[2]) / boxsize[2] * nc[2]
+  H = nc[2] // 2
+  kxy2 = kx[:, None]**2 + ky[None, :]**2
+  main = (kxy2[:, :, None] + kz[None, None, :H]**2)**0.5
+  nyq = (kxy2 + kz[H]**2)**0.5
+  return np.concatenate([main.reshape(-1), nyq.reshape(-1)])
+
+
+class _WhiteToLinear(torch.autograd.Function):
+  """linear = c2r(r2c(white) * amp); amp real and even, so the map is
+  symmetric and its adjoint is itself (tfpm.py:56-62)."""
+
+  @staticmethod
+  def forward(ctx, white, amp):
+    B, nx, ny, nz = [int(s) for s in white.shape]
+    plan = _plans.get(white.device, B, nx, ny, nz)
+    ctx.plan = plan
+    ctx.save_for_backward(amp)
+    return _WhiteToLinear.apply_op(plan, _f32(white), amp)
+
+  @staticmethod
+  def apply_op(plan, x, amp):
+    spec = fwd_real(plan, x, 1.0 / plan.M)
+    _lib.call("pmx_spec_mul_real", plan.handle, _lib.ptr(spec), _lib.ptr(amp),
+              _lib.stream_of(spec))
+    return inv_real(plan, spec, 1.0)
+
+  @staticmethod
+  def backward(ctx, g):
+    (amp,) = ctx.saved_tensors
+    return _WhiteToLinear.apply_op(ctx.plan, _f32(g), amp), None
+
+
+def whitenoise_to_linear(nc, boxsize, white, pk, kvec=None, name=None):
+  """Linear field from an injected real white-noise field
+  (scripts/check_grads.py:39-65 with the noise given in real space).
+
+  white: (batch, nx, ny, nz) N(0, sqrt(M)); pk: callable on a float64 numpy
+  |k| array (h/Mpc).  Differentiable w.r.t. ``white``."""
+  nc = _nc3(nc)
+  if np.isscalar(boxsize):
+    boxsize = [float(boxsize)] * 3
+  kmesh = _half_kmesh(nc, boxsize)
+  pkv = np.asarray(pk(kmesh), dtype=np.float64).reshape(-1)
+  amp = (pkv / (boxsize[0] * boxsize[1] * boxsize[2]))**0.5
+  amp = torch.as_tensor(amp.astype(np.float32), device=white.device)
+  _lib.require_cuda(white)
+  return _WhiteToLinear.apply(white, amp)
+
+
+def linear_field(nc, boxsize, pk, kvec=None, batch_size=1, seed=None,
+                 dtype=torch.float32, name="LinearField", device=None):
+  """Gaussian linear field with power spectrum pk (tfpm.py:15-63)."""
+  nc = _nc3(nc)
+  white = white_noise(nc, batch_size=batch_size, seed=seed, type="real",
+                      device=device)
+  return whitenoise_to_linear(nc, boxsize, white, pk)
+
+
+# -------------------------------------------------------------------- LPT --
+def _poisson_grad_meshes(plan, spec):
+  """3 meshes c2r(dk * lap * i g_d), as one (3, B, nx, ny, nz) tensor."""
+  out = torch.empty((3, plan.B, plan.nx, plan.ny, plan.nz),
+                    dtype=torch.float32, device=spec.device)
+  work = plan.spec(3)
+  _lib.call("pmx_fft_c2r_poisson_grad", plan.handle, _lib.ptr(spec),
+            _lib.ptr(work), _lib.ptr(out[0]), _lib.ptr(out[1]),
+            _lib.ptr(out[2]), 1.0, _lib.stream_of(spec))
+  return out
+
+
+def lpt1(dlin_k, pos, kvec=None, name="LTP1"):
+  """First-order LPT displacement read out at ``pos`` (tfpm.py:66-95)."""
+  B, nx, ny, nz = [int(s) for s in dlin_k.shape]
+  plan = _plans.get(dlin_k.device, B, nx, ny, nz)
+  meshes = _poisson_grad_meshes(plan, _full_to_real_half(plan, dlin_k))
+  return torch.stack([cic_readout(meshes[d], pos) for d in range(3)], dim=2)
+
+
+def _lpt2_src(plan, spec):
+  work = torch.empty(_lib.lib().pmx_lpt2_work_floats(plan.handle),
+                     dtype=torch.float32, device=spec.device)
+  src = plan.spec()
+  _lib.call("pmx_lpt2_source", plan.handle, _lib.ptr(spec), _lib.ptr(work),
+            _lib.ptr(src), _lib.stream_of(spec))
+  return src, work
+
+
+def lpt2_source(dlin_k, kvec=None, name="LPT2Source"):
+  """Second-order LPT source term, full complex spectrum (tfpm.py:98-145)."""
+  B, nx, ny, nz = [int(s) for s in dlin_k.shape]
+  plan = _plans.get(dlin_k.device, B, nx, ny, nz)
+  src, _ = _lpt2_src(plan, _full_to_real_half(plan, dlin_k))
+  return _half_to_full(plan, src)
+
+
+def lpt_coefficients(cosmo, a, order):
+  """(D1, a^2 f1 E, a^2 E gf / D1, D2, a^2 f2 E, a^2 E gf2 / D2) in float64
+  (tfpm.py:168-177)."""
+  a = float(a)
+  E = float(bg.E(cosmo, a))
+  d1 = float(bg.D1(cosmo, a))
+  c = [d1, a**2 * float(bg.f1(cosmo, a)) * E,
+       a**2 * E * float(bg.gf(cosmo, a)) / d1, 0.0, 0.0, 0.0]
+  if order == 2:
+    d2 = float(bg.D2(cosmo, a))
+    c[3:] = [d2, a**2 * float(bg.f2(cosmo, a)) * E,
+             a**2 * E * float(bg.gf2(cosmo, a)) / d2]
+  return c
+
+
+class _LptInit(torch.autograd.Function):
+
+  @staticmethod
+  def forward(ctx, linear, coef, order):
+    B, nx, ny, nz = [int(s) for s in linear.shape]
+    plan = _plans.get(linear.device, B, nx, ny, nz)
+    M = plan.M
+    dk = fwd_real(plan, _f32(linear), 1.0 / M)
+    dx1 = _poisson_grad_meshes(plan, dk)
+    dx2 = None
+    work2 = None
+    if order == 2:
+      src, work2 = _lpt2_src(plan, dk)
+      dx2 = _poisson_grad_meshes(plan, src)
+    del dk
+    state = torch.empty((3, B, M, 3), dtype=torch.float32,
+                        device=linear.device)
+    _lib.call("pmx_lpt_assemble", _lib.ptr(state), _lib.ptr(dx1),
+              _lib.ptr(dx2), _lib.dbl(coef), B, nx, ny, nz,
+              _lib.stream_of(state))
+    ctx.plan, ctx.coef, ctx.order = plan, coef, order
+    if order == 2:
+      ctx.save_for_backward(work2)
+    return state
+
+  @staticmethod
+  def backward(ctx, g):
+    plan, coef, order = ctx.plan, ctx.coef, ctx.order
+    B, nx, ny, nz, M = plan.B, plan.nx, plan.ny, plan.nz, plan.M
+    g = _f32(g)
+    dev = g.device
+    st = _lib.stream_of(g)
+    g1 = torch.empty((3, B, nx, ny, nz), dtype=torch.float32, device=dev)
+    g2 = torch.empty_like(g1) if order == 2 else None
+    _lib.call("pmx_lpt_assemble_adj", _lib.ptr(g), _lib.dbl(coef),
+              _lib.ptr(g1), _lib.ptr(g2), B, nx, ny, nz, st)
+    work = plan.spec(3)
+    glin = torch.empty((B, nx, ny, nz), dtype=torch.float32, device=dev)
+    _lib.call("pmx_force_meshes_adj", plan.handle, _lib.ptr(g1[0]),
+              _lib.ptr(g1[1]), _lib.ptr(g1[2]), _lib.ptr(work),
+              _lib.ptr(glin), 1.0 / M, st)
+    if order == 2:
+      (work2,) = ctx.saved_tensors
+      gsrc = torch.empty_like(glin)
+      _lib.call("pmx_force_meshes_adj", plan.handle, _lib.ptr(g2[0]),
+                _lib.ptr(g2[1]), _lib.ptr(g2[2]), _lib.ptr(work),
+                _lib.ptr(gsrc), 1.0 / M, st)
+      gphi = torch.empty((6, B, nx, ny, nz), dtype=torch.float32, device=dev)
+      _lib.call("pmx_lpt2_source_adj", plan.handle, _lib.ptr(work2),
+                _lib.ptr(gsrc), _lib.ptr(gphi), st)
+      # phi_ij^T = c2r(lap (i g_i)(i g_j) r2c(.)/M): diagonal, then off-diag
+      for idx, exps in (((0, 1, 2), (2, 0, 0, 0, 2, 0, 0, 0, 2)),
+                        ((3, 4, 5), (1, 1, 0, 1, 0, 1, 0, 1, 1))):
+        ins = (_lib._vp * 3)(*[gphi[i].data_ptr() for i in idx])
+        e = (_lib._i * 9)(*exps)
+        sc = (_lib._f * 3)(1.0, 1.0, 1.0)
+        _lib.call("pmx_fourier_apply", plan.handle, 3, ins, e, sc, 1,
+                  1.0 / M, _lib.ptr(work), _lib.ptr(glin), 1, st)
+    return glin, None, None
+
+
+def lpt_init(cosmo, linear, a, order=2, kvec=None, name="LPTInit"):
+  """LPT initial state (3, batch, npart, 3) from a linear field
+  (tfpm.py:148-180); differentiable w.r.t. ``linear``."""
+  assert order in (1, 2)
+  _lib.require_cuda(linear)
+  coef = lpt_coefficients(cosmo, a, order)
+  return _LptInit.apply(linear, coef, order)
+
+
+# ------------------------------------------------------------------ nbody --
+def apply_longrange(x, delta_k, split=0, factor=1, kvec=None,
+                    name="ApplyLongrange"):
+  """Long-range force at positions x from delta_k (tfpm.py:183-219)."""
+  if split != 0:
+    raise NotImplementedError("only split=0 (the path's setting) is built")
+  B, nx, ny, nz = [int(s) for s in delta_k.shape]
+  plan = _plans.get(delta_k.device, B, nx, ny, nz)
+  meshes = _poisson_grad_meshes(plan, _full_to_real_half(plan, delta_k))
+  f = torch.stack([cic_readout(meshes[d], x) for d in range(3)], dim=2)
+  return f * factor
+
+
+def _kick_fac(cosmo, ai, ac, af):
+  return float(1 / (ac**2 * bg.E(cosmo, ac)) *
+               (bg.Gf(cosmo, af) - bg.Gf(cosmo, ai)) / bg.gf(cosmo, ac))
+
+
+def _drift_fac(cosmo, ai, ac, af):
+  return float(1. / (ac**3 * bg.E(cosmo, ac)) *
+               (bg.D1(cosmo, af) - bg.D1(cosmo, ai)) / bg.D1f(cosmo, ac))
+
+
+def _state_dims(state):
+  if state.dim() != 4 or state.shape[0] != 3 or state.shape[3] != 3:
+    raise ValueError("state must be (3, batch, npart, 3), got %s" %
+                     (tuple(state.shape),))
+  return int(state.shape[1]), int(state.shape[2])
+
+
+def _kick_drift(state, k2=0.0, k1=0.0, d=0.0, flags=0):
+  _lib.require_cuda(state)
+  B, N = _state_dims(state)
+  st = _f32(state).clone()
+  _lib.call("pmx_readout_kick_drift", _lib.ptr(st[0]), _lib.ptr(st[1]),
+            _lib.ptr(st[2]), None, None, None, B, N, 1, 1, 1, 1.0, 0.0,
+            float(k2), float(k1), float(d), flags | 4, _lib.stream_of(st))
+  return st
+
+
+def kick(cosmo, state, ai, ac, af, dtype=torch.float32, name="Kick", **kw):
+  """P += fac F (tfpm.py:273-292)."""
+  return _kick_drift(state, k2=_kick_fac(cosmo, ai, ac, af), flags=1)
+
+
+def drift(cosmo, state, ai, ac, af, dtype=torch.float32, name="Drift", **kw):
+  """X += fac P (tfpm.py:295-314)."""
+  return _kick_drift(state, d=_drift_fac(cosmo, ai, ac, af), flags=2)
+
+
+def _force_setup(cosmo, state, nc, pm_nc_factor):
+  B, N = _state_dims(state)
+  nc = _nc3(nc)
+  if nc[0] * nc[1] * nc[2] != N:
+    raise ValueError("npart (%d) must equal prod(nc) (%d)" %
+                     (N, nc[0] * nc[1] * nc[2]))
+  f = int(pm_nc_factor)
+  plan = _plans.get(state.device, B, nc[0] * f, nc[1] * f, nc[2] * f)
+  fac = float(np.float32(1.5 * cosmo.Omega_m))
+  return plan, B, N, float(f), fac
+
+
+def _work(plan):
+  return torch.empty(_lib.lib().pmx_nbody_work_floats(plan.handle),
+                     dtype=torch.float32, device=plan.device)
+
+
+def force(cosmo, state, nc, pm_nc_factor=1, dtype=torch.float32, name="Force",
+          **kw):
+  """Replaces F by the PM force at X (tfpm.py:317-366)."""
+  _lib.require_cuda(state)
+  plan, B, N, f, fac = _force_setup(cosmo, state, nc, pm_nc_factor)
+  st = _f32(state).clone()
+  _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, None, 0, 0,
+            1, _lib.ptr(_work(plan)), _lib.stream_of(st))
+  return st
+
+
+def step_coefficients(cosmo, stages):
+  """Per-step (k1, d, k2) of the KDK scheme of tfpm.py:472-498 (float64):
+  opening kick (a0 -> ah), drift (a0 -> a1 at ah), closing kick
+  (ah -> a1 at a1), ah = sqrt(a0 a1)."""
+  coef = []
+  for i in range(len(stages) - 1):
+    a0, a1 = float(stages[i]), float(stages[i + 1])
+    ah = (a0 * a1)**0.5
+    coef += [_kick_fac(cosmo, a0, a0, ah), _drift_fac(cosmo, a0, ah, a1),
+             _kick_fac(cosmo, ah, a1, a1)]
+  return coef
+
+
+class _Nbody(torch.autograd.Function):
+
+  @staticmethod
+  def forward(ctx, state, plan, N, f, fac, coef, every, need_grad):
+    B = plan.B
+    nsteps = len(coef) // 3
+    st = _f32(state).clone()
+    work = _work(plan)
+    cptr = _lib.dbl(coef)
+    stream = _lib.stream_of(st)
+    if not need_grad:
+      _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
+                0, nsteps, 1, _lib.ptr(work), stream)
+      return st
+    nseg = max(1, (nsteps + every - 1) // every)
+    ckpt = torch.empty((nseg, 3, B, N, 3), dtype=torch.float32,
+                       device=st.device)
+    for s in range(nseg):
+      ckpt[s].copy_(st)
+      b0, b1 = s * every, min(nsteps, (s + 1) * every)
+      _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
+                b0, b1, 1 if s == 0 else 0, _lib.ptr(work), stream)
+    ctx.save_for_backward(ckpt)
+    ctx.args = (plan, N, f, fac, coef, every)
+    return st
+
+  @staticmethod
+  def backward(ctx, g):
+    (ckpt,) = ctx.saved_tensors
+    plan, N, f, fac, coef, every = ctx.args
+    nsteps = len(coef) // 3
+    gs = _f32(g).clone()
+    xslots = torch.empty((min(every, max(nsteps, 1)), plan.B, N, 3),
+                         dtype=torch.float32, device=gs.device)
+    work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N),
+                       dtype=torch.float32, device=gs.device)
+    _lib.call("pmx_nbody_adjoint", plan.handle, _lib.ptr(ckpt), every, nsteps,
+              N, f, fac, _lib.dbl(coef), _lib.ptr(gs), _lib.ptr(xslots),
+              _lib.ptr(work), _lib.stream_of(gs))
+    return gs, None, None, None, None, None, None, None
+
+
+def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
+          return_intermediate_states=False, pgdparams=None, name="NBody",
+          checkpoint_every=None):
+  """Integrates the state across ``stages`` with the FastPM KDK scheme
+  (tfpm.py:425-503).  Differentiable w.r.t. ``state``: the backward pass
+  recomputes each segment of ``checkpoint_every`` steps from a stored
+  checkpoint and runs the hand-written adjoint kernels in reverse."""
+  if pgdparams is not None:
+    raise NotImplementedError("PGD correction (tfpm.py:369-422) is not built")
+  _lib.require_cuda(state)
+  if len(stages) == 0:
+    return state
+  plan, B, N, f, fac = _force_setup(cosmo, state, nc, pm_nc_factor)
+  coef = step_coefficients(cosmo, stages)
+  if return_intermediate_states:
+    st = _f32(state).clone()
+    work = _work(plan)
+    cptr = _lib.dbl(coef)
+    out = []
+    for i in range(len(stages) - 1):
+      _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
+                i, i + 1, 1 if i == 0 else 0, _lib.ptr(work),
+                _lib.stream_of(st))
+      out.append((stages[i + 1], st.clone()))
+    return out
+  every = int(checkpoint_every or _CKPT_EVERY[0])
+  need_grad = torch.is_grad_enabled() and state.requires_grad
+  return _Nbody.apply(state, plan, N, f, fac, coef, every, need_grad)

@@ -33,6 +33,17 @@ typedef struct pmx_plan pmx_plan;
 int pmx_version(void);
 const char* pmx_last_error(void);
 
+/* Live kernel timing (no reference counterpart; the reference's scripts
+ * time whole runs with nvprof / the TF profiler,
+ * scripts/Modi2020/fft_benchmark.py:126-168).  While started, every launch
+ * is bracketed by hipEvents on its stream; stop synchronises and returns the
+ * summed milliseconds and launch counts per kernel kind
+ * (pmx_profile_kind_name). */
+int pmx_profile_start(void);
+int pmx_profile_stop(double* ms, long long* count, int nkinds);
+int pmx_profile_kinds(void);
+const char* pmx_profile_kind_name(int kind);
+
 /* Plan for a batch of B meshes of nx*ny*nz cells (each in
  * {8,12,16,24,32,48,64,96,128,192,256,384,512,768,1024}, nz even).
  * Holds the twiddles, (2 pi fftfreq)^2 and 4th-order gradient tables

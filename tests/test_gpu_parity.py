@@ -1,0 +1,250 @@
+"""GPU parity of the HIP path against the CPU oracle (float64 unless noted).
+
+Tolerances: FFTs and CIC at float32 rounding (relative RMS 1e-6 .. 1e-5);
+force / LPT / nbody final positions, velocities and painted density at the
+1e-4 relative RMS that the PM-step parity bar states; gradients by cosine
+>= 0.999 against the float64 oracle's autograd.
+"""
+import numpy as np
+import pytest
+import torch
+
+import flowpm_amd as fa
+from oracle import background as obg
+from oracle import pm as opm
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+  a = a.detach().cpu().to(torch.complex128 if a.is_complex() else
+                          torch.float64)
+  b = b.detach().cpu().to(a.dtype)
+  return float(((a - b).abs()**2).mean().sqrt() / (b.abs()**2).mean().sqrt())
+
+
+def close(a, b, tol, what=""):
+  e = rel_rms(a, b)
+  assert e < tol, "%s rel RMS %.3g >= %.3g" % (what, e, tol)
+  return e
+
+
+def cosine(a, b):
+  a = a.detach().double().cpu().reshape(-1)
+  b = b.detach().double().cpu().reshape(-1)
+  return float(a @ b / (a.norm() * b.norm()))
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 16, 16), (2, 32, 32, 32),
+                                   (1, 16, 16, 48), (1, 96, 96, 96),
+                                   (1, 24, 48, 12), (1, 128, 64, 256)])
+def test_r2c_c2r_against_numpy(gpu, shape):
+  rng = np.random.default_rng(sum(shape))
+  x = rng.standard_normal(shape)
+  M = np.prod(shape[1:])
+  c = fa.r2c3d(torch.tensor(x, dtype=torch.float32, device=gpu))
+  ref = np.fft.fftn(x, axes=(1, 2, 3)) / M
+  assert c.dtype == torch.complex64 and tuple(c.shape) == shape
+  err = np.sqrt(np.mean(np.abs(c.cpu().numpy() - ref)**2) /
+                np.mean(np.abs(ref)**2))
+  assert err < 2e-6, err
+  back = fa.c2r3d(c)
+  close(back, torch.tensor(x), 2e-6)
+  # c2r of a non-Hermitian spectrum = real part of the full inverse
+  z = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+  r = fa.c2r3d(torch.tensor(z, dtype=torch.complex64, device=gpu), norm=7.0)
+  refr = np.fft.ifftn(z, axes=(1, 2, 3)).real * 7.0
+  close(r, torch.tensor(refr), 2e-6)
+
+
+def test_cic_against_oracle(gpu):
+  rng = np.random.default_rng(0)
+  nc = [16, 16, 16]
+  pos = rng.random((2, 5000, 3)) * 16
+  pos[:, :50] -= 17.5  # wrap through negative coordinates
+  w = rng.random((2, 5000))
+  base = rng.random([2] + nc) * 100
+  zero = torch.zeros([2] + nc, dtype=torch.float64)
+  ref = opm.cic_paint(zero, torch.tensor(pos), torch.tensor(w))
+  out = fa.cic_paint(zero.float().to(gpu), torch.tensor(pos).float().to(gpu),
+                     torch.tensor(w).float().to(gpu))
+  e = rel_rms(out, ref)
+  assert e < 1e-6, e
+  # onto a non-zero mesh: float32 sum of base (~100) and increments (~1)
+  mesh0 = torch.tensor(base)
+  ref = opm.cic_paint(mesh0, torch.tensor(pos), torch.tensor(w))
+  out = fa.cic_paint(mesh0.float().to(gpu), torch.tensor(pos).float().to(gpu),
+                     torch.tensor(w).float().to(gpu))
+  e = rel_rms(out, ref)
+  assert e < 1e-7, e
+  refv = opm.cic_readout(torch.tensor(base), torch.tensor(pos))
+  val = fa.cic_readout(torch.tensor(base).float().to(gpu),
+                       torch.tensor(pos).float().to(gpu))
+  e = rel_rms(val, refv)
+  assert e < 1e-6, e
+  # mass conservation
+  m = fa.cic_paint(torch.zeros(2, *nc, device=gpu),
+                   torch.tensor(pos).float().to(gpu))
+  np.testing.assert_allclose(m.sum((1, 2, 3)).cpu().numpy(), 5000, rtol=1e-5)
+
+
+def test_cic_adjoints_against_oracle_autograd(gpu):
+  rng = np.random.default_rng(1)
+  nc = [8, 16, 12]
+  pos = torch.tensor(rng.random((1, 700, 3)) * np.array(nc) * 1.3 - 2)
+  w = torch.tensor(rng.random((1, 700)))
+  gm = torch.tensor(rng.standard_normal([1] + nc))
+  gv = torch.tensor(rng.standard_normal((1, 700)))
+  mesh = torch.tensor(rng.standard_normal([1] + nc))
+  # oracle
+  p_o = pos.clone().requires_grad_(True)
+  w_o = w.clone().requires_grad_(True)
+  m_o = mesh.clone().requires_grad_(True)
+  L = (opm.cic_paint(m_o, p_o, w_o) * gm).sum() + \
+      (opm.cic_readout(m_o, p_o) * gv).sum()
+  L.backward()
+  # HIP
+  p_g = pos.float().to(gpu).requires_grad_(True)
+  w_g = w.float().to(gpu).requires_grad_(True)
+  m_g = mesh.float().to(gpu).requires_grad_(True)
+  Lg = (fa.cic_paint(m_g, p_g, w_g) * gm.float().to(gpu)).sum() + \
+      (fa.cic_readout(m_g, p_g) * gv.float().to(gpu)).sum()
+  Lg.backward()
+  close(p_g.grad, p_o.grad, 1e-5)
+  close(w_g.grad, w_o.grad, 1e-6)
+  close(m_g.grad, m_o.grad, 1e-6)
+
+
+def test_fft_autograd_against_oracle(gpu):
+  rng = np.random.default_rng(2)
+  x = torch.tensor(rng.standard_normal((1, 16, 8, 24)))
+  gc = torch.tensor(rng.standard_normal((1, 16, 8, 24)) +
+                    1j * rng.standard_normal((1, 16, 8, 24)))
+  xo = x.clone().requires_grad_(True)
+  Lo = (opm.r2c3d(xo, norm=3.0) * gc.conj()).real.sum()
+  Lo = Lo + (opm.c2r3d(opm.r2c3d(xo)) * x).sum()
+  Lo.backward()
+  xg = x.float().to(gpu).requires_grad_(True)
+  Lg = (fa.r2c3d(xg, norm=3.0) * gc.to(torch.complex64).to(gpu).conj()
+        ).real.sum()
+  Lg = Lg + (fa.c2r3d(fa.r2c3d(xg)) * x.float().to(gpu)).sum()
+  Lg.backward()
+  close(xg.grad, xo.grad, 2e-6)
+
+
+def _displaced_state(n, B=1, amp=1.5, seed=3, dtype=torch.float64):
+  gen = torch.Generator().manual_seed(seed)
+  x = opm.lagrangian_grid([n] * 3, B, dtype)
+  x = x + amp * torch.randn(x.shape, generator=gen, dtype=dtype)
+  p = torch.randn(x.shape, generator=gen, dtype=dtype)
+  return torch.stack([x, p, torch.zeros_like(x)])
+
+
+@pytest.mark.parametrize("n,f,B", [(32, 1, 1), (16, 2, 2), (64, 1, 1)])
+def test_force_against_oracle(gpu, n, f, B):
+  g = obg.GrowthTable(obg.planck15())
+  st = _displaced_state(n, B)
+  ref = opm.force(g, st, [n] * 3, pm_nc_factor=f)[2]
+  out = fa.tfpm.force(fa.cosmology.Planck15(), st.float().to(gpu), n,
+                      pm_nc_factor=f)
+  close(out[2], ref, 1e-4)
+  assert float((out[0].double().cpu() - st[0].float().double()).abs().max()) == 0.0
+
+
+def test_kick_drift(gpu):
+  c = fa.cosmology.Planck15()
+  g = obg.GrowthTable(obg.planck15())
+  st = _displaced_state(8)
+  st[2] = torch.randn_like(st[2])
+  k = fa.tfpm.kick(c, st.float().to(gpu), 0.1, 0.12, 0.15)
+  close(k, opm.kick(g, st, 0.1, 0.12, 0.15), 1e-6)
+  d = fa.tfpm.drift(c, st.float().to(gpu), 0.1, 0.12, 0.15)
+  close(d, opm.drift(g, st, 0.1, 0.12, 0.15), 1e-6)
+
+
+def _ics(n, L, seed, B=1):
+  gen = torch.Generator().manual_seed(seed)
+  white = torch.randn(B, n, n, n, generator=gen, dtype=torch.float64) * n**1.5
+  from oracle import power as opw
+  k = np.logspace(-4, 1, 256)
+  pkt = opw.linear_matter_power(obg.planck15(), k)
+  pk = lambda x: opw.interp_tf(x.reshape(-1), k, pkt).reshape(x.shape)
+  return white, pk
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_linear_field_and_lpt_init(gpu, order):
+  n, L = 32, 100.0
+  white, pk = _ics(n, L, 100)
+  g = obg.GrowthTable(obg.planck15())
+  lin_o = opm.whitenoise_to_linear(white, L, pk)
+  lin_g = fa.whitenoise_to_linear(n, L, white.float().to(gpu), pk)
+  close(lin_g, lin_o, 2e-6)
+  st_o = opm.lpt_init(g, lin_o, 0.1, order=order)
+  st_g = fa.lpt_init(fa.cosmology.Planck15(), lin_g, 0.1, order=order)
+  q = opm.lagrangian_grid([n] * 3, 1, torch.float64)
+  close(st_g[0] - q.float().to(gpu), st_o[0] - q, 1e-5)
+  close(st_g[1], st_o[1], 1e-5)
+  close(st_g[2], st_o[2], 1e-5)
+
+
+def test_lpt_ops_dropins(gpu):
+  n = 16
+  white, pk = _ics(n, 50.0, 7)
+  lin = opm.whitenoise_to_linear(white, 50.0, pk)
+  dk_o = opm.r2c3d(lin)
+  dk_g = fa.r2c3d(lin.float().to(gpu))
+  q = opm.lagrangian_grid([n] * 3, 1, torch.float64) + 0.3
+  close(fa.tfpm.lpt1(dk_g, q.float().to(gpu)), opm.lpt1(dk_o, q), 1e-5)
+  close(fa.tfpm.lpt2_source(dk_g), opm.lpt2_source(dk_o), 1e-5)
+  f_o = opm.apply_longrange(q, dk_o, factor=0.4)
+  f_g = fa.tfpm.apply_longrange(q.float().to(gpu), dk_g, factor=0.4)
+  close(f_g, f_o, 1e-5)
+
+
+def test_nbody_readme_workflow_32(gpu):
+  """The reference README workflow at 32^3, box 100, 1LPT at 0.1,
+  stages linspace(0.1, 1, 10): final X, P and painted density."""
+  n, L = 32, 100.0
+  white, pk = _ics(n, L, 100)
+  g = obg.GrowthTable(obg.planck15())
+  stages = np.linspace(0.1, 1.0, 10)
+  lin = opm.whitenoise_to_linear(white, L, pk)
+  st_o = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=1), stages, [n] * 3)
+  c = fa.cosmology.Planck15()
+  st_g = fa.nbody(c, fa.lpt_init(c, lin.float().to(gpu), 0.1, order=1),
+                  stages, n)
+  q = opm.lagrangian_grid([n] * 3, 1, torch.float64)
+  ex = rel_rms(st_g[0] - q.float().to(gpu), st_o[0] - q)
+  ep = rel_rms(st_g[1], st_o[1])
+  rho_o = opm.cic_paint(torch.zeros(1, n, n, n, dtype=torch.float64), st_o[0])
+  rho_g = fa.cic_paint(torch.zeros(1, n, n, n, device=gpu), st_g[0])
+  er = rel_rms(rho_g - 1, rho_o - 1)
+  print("32^3 nbody rel RMS: dX %.3g P %.3g delta %.3g" % (ex, ep, er))
+  assert ex < 1e-4 and ep < 1e-4 and er < 1e-4
+
+
+def test_nbody_gradient_against_oracle(gpu):
+  n, L = 16, 64.0
+  white, pk = _ics(n, L, 11)
+  tgt = torch.rand(1, n, n, n, dtype=torch.float64,
+                   generator=torch.Generator().manual_seed(12)) * 2
+  g = obg.GrowthTable(obg.planck15())
+  stages = np.linspace(0.1, 1.0, 6)
+  wo = white.clone().requires_grad_(True)
+  lin = opm.whitenoise_to_linear(wo, L, pk)
+  st = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=2), stages, [n] * 3)
+  rho = opm.cic_paint(torch.zeros_like(lin), st[0])
+  ((rho - tgt)**2).sum().backward()
+  c = fa.cosmology.Planck15()
+  for every in (1, 2, 8):
+    wg = white.float().to(gpu).requires_grad_(True)
+    ling = fa.whitenoise_to_linear(n, L, wg, pk)
+    stg = fa.nbody(c, fa.lpt_init(c, ling, 0.1, order=2), stages, n,
+                   checkpoint_every=every)
+    rhog = fa.cic_paint(torch.zeros_like(ling), stg[0])
+    ((rhog - tgt.float().to(gpu))**2).sum().backward()
+    cs = cosine(wg.grad, wo.grad)
+    print("gradient cosine (every=%d): %.7f, rel %.3g" %
+          (every, cs, rel_rms(wg.grad, wo.grad)))
+    assert cs > 0.999
