@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_force_vjp_particles(
     float* __restrict__ dX, const float* __restrict__ gF,
     const float* __restrict__ X, const float* __restrict__ fxm,
     const float* __restrict__ fym, const float* __restrict__ fzm, MeshDims d,
-    float f, float cf) {
+    float f, float cf, int scatter) {
   const long long tot = (long long)d.B * d.N;
   const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (p >= tot) return;
@@ -274,10 +274,12 @@ __global__ __launch_bounds__(256) void k_force_vjp_particles(
   if (pos_ok(x, y, z)) {
     const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
     PMX_CORNERS({
-      const float k = (wx * wy) * wz;
-      atomicAdd(gmx + cell, gfx * k);
-      atomicAdd(gmy + cell, gfy * k);
-      atomicAdd(gmz + cell, gfz * k);
+      if (scatter) {
+        const float k = (wx * wy) * wz;
+        atomicAdd(gmx + cell, gfx * k);
+        atomicAdd(gmy + cell, gfy * k);
+        atomicAdd(gmz + cell, gfz * k);
+      }
       const float v = gfx * fxm[cell] + gfy * fym[cell] + gfz * fzm[cell];
       sx += v * ((dx * wy) * wz);
       sy += v * ((wx * dy) * wz);
@@ -379,12 +381,13 @@ int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz, float* dX,
                                const float* gF, const float* X,
                                const float* fx, const float* fy,
                                const float* fz, const MeshDims& d, float f,
-                               float c, hipStream_t s) {
+                               float c, int scatter, hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   ProfScope prof_(kProfAdjCic, s);
   hipLaunchKernelGGL(k_force_vjp_particles, dim3(nblk(tot, 256)), dim3(256),
-                     0, s, gmx, gmy, gmz, dX, gF, X, fx, fy, fz, d, f, c);
+                     0, s, gmx, gmy, gmz, dX, gF, X, fx, fy, fz, d, f, c,
+                     scatter);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

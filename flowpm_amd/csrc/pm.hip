@@ -40,7 +40,7 @@ void prof_record(int kind, hipStream_t s, hipEvent_t* ev, bool end) {
 }
 
 static const char* kProfNames[kProfKinds] = {
-    "fill", "paint", "fft_z_r2c", "fft_y_fwd", "fft_x_fused", "fft_y_inv",
+    "fill", "paint", "paint_gather", "fft_z_r2c", "fft_y_fwd", "fft_x_fused", "fft_y_inv",
     "fft_z_c2r", "readout_kick_drift", "kick_drift", "readout", "cic_adjoint",
     "other"};
 
@@ -441,18 +441,61 @@ struct NbodyWork {
   float* rho;
   float* f[3];
   float2* w;
+  float* brick;  // brick-paint arena (C = 1 and C = 3 share it)
 };
+// Brick paint needs the particle grid: the mesh divided by pm_nc_factor.
+static MeshDims paint_dims(const Plan& p, long long N) {
+  return MeshDims{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
+}
+static long long brick_floats(const Plan& p) {
+  // worst case over f in {1, 2, 4} and C in {1, 3}
+  long long m = 0;
+  for (int f = 1; f <= 4; f *= 2) {
+    if (p.d.nx % f || p.d.ny % f || p.d.nz % f) continue;
+    const long long N = (long long)(p.d.nx / f) * (p.d.ny / f) * (p.d.nz / f);
+    const MeshDims d = paint_dims(p, N);
+    for (int C = 1; C <= 3; C += 2) {
+      const BrickGrid g =
+          make_brick_grid(p.d.nx / f, p.d.ny / f, p.d.nz / f, f, C);
+      m = std::max(m, brick_work_floats(d, g, C));
+    }
+  }
+  return m;
+}
 static NbodyWork nbody_work(const Plan& p, float* work) {
   NbodyWork nw;
   const long long M = p.d.rows * p.d.nz;
   nw.rho = work;
   for (int d = 0; d < 3; ++d) nw.f[d] = work + (1 + d) * M;
   nw.w = reinterpret_cast<float2*>(work + 4 * M);
+  nw.brick = work + 4 * M + 2 * 3 * p.d.S;
   return nw;
 }
 static long long nbody_work_floats(const Plan& p) {
   const long long M = p.d.rows * p.d.nz;
-  return 4 * M + 2 * 3 * p.d.S;
+  return 4 * M + 2 * 3 * p.d.S + brick_floats(p);
+}
+
+// Paint C meshes (zeroed here) with the brick paint when the particles form
+// the grid mesh/f, else with the plain atomic scatter.
+static int paint_meshes(const Plan& p, float* const* mesh, int C,
+                        const float* X, const float* w, int wstride,
+                        float wscale, long long N, float f, float* brick,
+                        hipStream_t s) {
+  const long long M = p.d.rows * p.d.nz;
+  for (int c = 0; c < C; ++c) PMX_TRY(launch_fill(mesh[c], M, 0.0f, s));
+  const int fi = (int)f;
+  const MeshDims md = paint_dims(p, N);
+  if ((float)fi == f && fi >= 1 && p.d.nx % fi == 0 && p.d.ny % fi == 0 &&
+      p.d.nz % fi == 0) {
+    const BrickGrid g =
+        make_brick_grid(p.d.nx / fi, p.d.ny / fi, p.d.nz / fi, fi, C);
+    if (brick_paint_ok(md, g) && brick_work_floats(md, g, C) <= brick_floats(p))
+      return launch_brick_paint(mesh, C, X, w, wstride, wscale, f, md, g,
+                                brick, s);
+  }
+  if (C == 1) return launch_paint(mesh[0], X, w, wscale, md, f, 0, s);
+  return kUnsupported;  // C = 3 only arises on the grid path
 }
 
 // force meshes at X (positions * f), density scaled by 1/nbar and 1/M.
@@ -460,9 +503,11 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
                     const NbodyWork& nw, hipStream_t s) {
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
-  PMX_TRY(launch_fill(nw.rho, M, 0.0f, s));
-  MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
-  PMX_TRY(launch_paint(nw.rho, X, nullptr, 1.0f, md, f, 0, s));
+  (void)M;
+  {
+    float* r[1] = {nw.rho};
+    PMX_TRY(paint_meshes(p, r, 1, X, nullptr, 1, 1.0f, N, f, nw.brick, s));
+  }
   // rho * (1/nbar) with nbar = N / Mb (tfpm.py:346-351), then r2c / Mb
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
@@ -522,10 +567,15 @@ static int force_vjp(const Plan& p, const float* X, const float* gF,
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
   PMX_TRY(force_at(p, X, N, f, nw, s));
-  PMX_TRY(launch_fill(gm, 3 * M, 0.0f, s));
+  float* gms[3] = {gm, gm + M, gm + 2 * M};
+  const bool grid = paint_meshes(p, gms, 3, X, gF, 3, fac / f, N, f,
+                                 nw.brick, s) == kOk;
+  if (!grid) {
+    for (int c = 0; c < 3; ++c) PMX_TRY(launch_fill(gms[c], M, 0.0f, s));
+  }
   PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X,
                                      nw.f[0], nw.f[1], nw.f[2], md, f,
-                                     fac / f, s));
+                                     fac / f, grid ? 0 : 1, s));
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
   PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nw.rho, sc, s));

@@ -69,6 +69,7 @@ enum Status : int {
 enum ProfKind : int {
   kProfFill = 0,
   kProfPaint,
+  kProfPaintGather,
   kProfZr2c,
   kProfYfwd,
   kProfX,
@@ -215,11 +216,29 @@ int launch_kick_drift_only(float* X, float* P, const float* F,
                            const MeshDims& d, const KickDrift& c,
                            hipStream_t s);
 
+// Lagrangian-brick paint (paint.hip).
+struct BrickGrid {
+  int gx, gy, gz;     // particle grid (N = gx*gy*gz per batch member)
+  int ex, ey, ez;     // brick size in particles
+  int nbx, nby, nbz;  // bricks per dimension
+  int nbricks;
+  int f;              // mesh cells per particle spacing (pm_nc_factor)
+};
+BrickGrid make_brick_grid(int gx, int gy, int gz, int f, int C);
+bool brick_paint_ok(const MeshDims& d, const BrickGrid& g);
+long long brick_work_floats(const MeshDims& d, const BrickGrid& g, int C);
+// mesh[c] (zeroed by the caller) += sum_p w[p*wstride+c]*wscale * CIC(X*ps)
+int launch_brick_paint(float* const* mesh, int C, const float* X,
+                       const float* w, int wstride, float wscale, float ps,
+                       const MeshDims& d, const BrickGrid& g, float* work,
+                       hipStream_t s);
+
 // Force VJP pieces.
+// scatter = 0 skips the mesh-cotangent atomics (done by the brick paint)
 int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz,
                                float* dX, const float* gF, const float* X,
                                const float* fx, const float* fy,
                                const float* fz, const MeshDims& d, float f,
-                               float c, hipStream_t s);
+                               float c, int scatter, hipStream_t s);
 
 }  // namespace pmx

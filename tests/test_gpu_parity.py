@@ -140,10 +140,14 @@ def _displaced_state(n, B=1, amp=1.5, seed=3, dtype=torch.float64):
   return torch.stack([x, p, torch.zeros_like(x)])
 
 
-@pytest.mark.parametrize("n,f,B", [(32, 1, 1), (16, 2, 2), (64, 1, 1)])
-def test_force_against_oracle(gpu, n, f, B):
+@pytest.mark.parametrize("n,f,B,amp", [(32, 1, 1, 1.5), (16, 2, 2, 1.5),
+                                       (64, 1, 1, 1.5), (32, 1, 2, 9.0),
+                                       (24, 1, 1, 40.0)])
+def test_force_against_oracle(gpu, n, f, B, amp):
+  """Small displacements use the LDS brick paint; amp 9 makes some bricks
+  overflow to device atomics; amp 40 wraps particles many boxes away."""
   g = obg.GrowthTable(obg.planck15())
-  st = _displaced_state(n, B)
+  st = _displaced_state(n, B, amp=amp)
   ref = opm.force(g, st, [n] * 3, pm_nc_factor=f)[2]
   out = fa.tfpm.force(fa.cosmology.Planck15(), st.float().to(gpu), n,
                       pm_nc_factor=f)
