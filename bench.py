@@ -42,7 +42,9 @@ def algorithmic_bytes(kind, n, N):
   S8 = 8 * n * n * (n // 2 + 1)  # one half-spectrum buffer, bytes
   table = {
       "fill": 4 * M,
-      "paint": 12 * N + 4 * M,
+      # brick paint: deposit reads the particles, gather writes the mesh
+      "paint": 12 * N,
+      "paint_gather": 4 * M,
       "fft_z_r2c": 4 * M + S8,
       "fft_y_fwd": 2 * S8,
       "fft_x_fused": S8 + 2 * S8,
@@ -89,6 +91,8 @@ def main():
   stages = np.linspace(0.1, 1.0, K + 1)
   coef = fa.tfpm.step_coefficients(cosmo, stages)
   plan = _plans.get(dev, 1, n, n, n)
+  from flowpm_amd import autotune
+  tiles, tile_ms = autotune.tune(dev, 1, n, n, n)  # untimed
   fac = float(np.float32(1.5 * cosmo.Omega_m))
   work = torch.empty(_lib.lib().pmx_nbody_work_floats(plan.handle),
                      dtype=torch.float32, device=dev)
@@ -183,7 +187,9 @@ def main():
                       "to a=1" % (n, n, L, K),
           "mesh": n, "particles": N, "box_mpc_h": L, "lpt_order": 2,
           "a0": 0.1, "checkpoint_every": args.every,
+          "fft_tiles": {"y": tiles[0], "x": tiles[1]},
       },
+      "autotune_force_mesh_ms": {"%dx%d" % k: v for k, v in tile_ms.items()},
       "fwd_adjoint_ms_per_step": fa_ms,
       "adjoint_over_forward": (fa_ms / ms_step) if fa_ms else None,
       "roofline": {

@@ -1,0 +1,88 @@
+"""Many simulations of uneven mesh sizes advanced together.
+
+The reference batches only simulations of one size (a leading batch axis,
+flowpm/tfpm.py:15-63); here each size class is one batched plan and all
+classes advance step by step on one stream.  The whole KDK loop of every
+class is captured once into a HIP graph (the library's launches are plain
+stream work, so torch's graph capture records them) and replayed: one
+launch of the graph runs every class for every step with no host
+round trip.  Coefficients are per class (each class may have its own
+cosmology and stages).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from . import _plans
+from . import tfpm
+
+
+class _Group:
+
+  def __init__(self, cosmo, state, stages, nc, pm_nc_factor):
+    self.plan, self.B, self.N, self.f, self.fac = tfpm._force_setup(
+        cosmo, state, nc, pm_nc_factor)
+    self.state = tfpm._f32(state).clone()
+    self.coef = tfpm.step_coefficients(cosmo, stages)
+    self.cptr = _lib.dbl(self.coef)
+    self.nsteps = len(self.coef) // 3
+    self.work = torch.empty(_lib.lib().pmx_nbody_work_floats(
+        self.plan.handle), dtype=torch.float32, device=state.device)
+
+  def step(self, i, stream):
+    if i >= self.nsteps and not (i == 0 and self.nsteps == 0):
+      return
+    _lib.call("pmx_nbody_run", self.plan.handle, _lib.ptr(self.state), self.N,
+              self.f, self.fac, self.cptr, i, min(i + 1, self.nsteps),
+              1 if i == 0 else 0, _lib.ptr(self.work), stream)
+
+
+class BatchedNbody:
+  """Advances several (cosmo, state, stages, nc) groups together.
+
+  ``groups``: list of dicts with keys cosmo, state (3, B, N, 3), stages, nc
+  and optionally pm_nc_factor.  ``run()`` integrates every group to its last
+  stage and returns the final states; with ``graph=True`` the loop is
+  captured once and replayed on later ``run()`` calls from the same initial
+  states (``reset(states)`` loads new ones in place)."""
+
+  def __init__(self, groups, graph=True):
+    self.groups = [_Group(g["cosmo"], g["state"], g["stages"], g["nc"],
+                          g.get("pm_nc_factor", 1)) for g in groups]
+    self.init = [g.state.clone() for g in self.groups]
+    self.use_graph = graph
+    self.graph = None
+    self.nsteps = max(max(g.nsteps for g in self.groups), 1)
+
+  def _loop(self, stream):
+    for i in range(self.nsteps):
+      for g in self.groups:
+        g.step(i, stream)
+
+  def reset(self, states=None):
+    for k, g in enumerate(self.groups):
+      src = self.init[k] if states is None else tfpm._f32(states[k])
+      g.state.copy_(src)
+
+  def run(self):
+    if not self.use_graph:
+      self.reset()
+      self._loop(_lib.stream_of(self.groups[0].state))
+      return [g.state for g in self.groups]
+    if self.graph is None:
+      # warm once eagerly (loads kernels), then capture
+      self.reset()
+      self._loop(_lib.stream_of(self.groups[0].state))
+      torch.cuda.synchronize()
+      self.graph = torch.cuda.CUDAGraph()
+      s = torch.cuda.Stream()
+      s.wait_stream(torch.cuda.current_stream())
+      with torch.cuda.stream(s):
+        self.reset()
+        with torch.cuda.graph(self.graph, stream=s):
+          self._loop(_lib.stream_of(self.groups[0].state))
+      torch.cuda.current_stream().wait_stream(s)
+      torch.cuda.synchronize()
+    self.reset()
+    self.graph.replay()
+    return [g.state for g in self.groups]

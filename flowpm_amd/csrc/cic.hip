@@ -76,7 +76,8 @@ __global__ __launch_bounds__(256) void k_paint(
     float* __restrict__ mesh, const float* __restrict__ pos,
     const float* __restrict__ w, float wscale, MeshDims d, float ps) {
   const long long tot = (long long)d.B * d.N;
-  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long p =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
   if (p >= tot) return;
   const int b = (int)(p / d.N);
   const float x = pos[3 * p] * ps, y = pos[3 * p + 1] * ps,
@@ -94,7 +95,8 @@ __global__ __launch_bounds__(256) void k_readout(
     float* __restrict__ out, const float* __restrict__ mesh,
     const float* __restrict__ pos, MeshDims d, float ps) {
   const long long tot = (long long)d.B * d.N;
-  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long p =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
   if (p >= tot) return;
   const int b = (int)(p / d.N);
   const float x = pos[3 * p] * ps, y = pos[3 * p + 1] * ps,
@@ -116,7 +118,8 @@ __global__ __launch_bounds__(256) void k_paint_adj(
     const float* __restrict__ w, MeshDims d, float ps, float gscale,
     int accumulate) {
   const long long tot = (long long)d.B * d.N;
-  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long p =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
   if (p >= tot) return;
   const int b = (int)(p / d.N);
   const float x = pos[3 * p] * ps, y = pos[3 * p + 1] * ps,
@@ -153,7 +156,8 @@ __global__ __launch_bounds__(256) void k_readout_adj(
     const float* __restrict__ gout, const float* __restrict__ mesh,
     const float* __restrict__ pos, MeshDims d, float ps, int accumulate) {
   const long long tot = (long long)d.B * d.N;
-  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long p =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
   if (p >= tot) return;
   const int b = (int)(p / d.N);
   const float x = pos[3 * p] * ps, y = pos[3 * p + 1] * ps,
@@ -195,7 +199,8 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
     const float* __restrict__ fxm, const float* __restrict__ fym,
     const float* __restrict__ fzm, MeshDims d, KickDrift c) {
   const long long tot = (long long)d.B * d.N;
-  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long p =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
   if (p >= tot) return;
   const int b = (int)(p / d.N);
   float px = X[3 * p], py = X[3 * p + 1], pz = X[3 * p + 2];
@@ -264,7 +269,8 @@ __global__ __launch_bounds__(256) void k_force_vjp_particles(
     const float* __restrict__ fym, const float* __restrict__ fzm, MeshDims d,
     float f, float cf, int scatter) {
   const long long tot = (long long)d.B * d.N;
-  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long p =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
   if (p >= tot) return;
   const int b = (int)(p / d.N);
   const float x = X[3 * p] * f, y = X[3 * p + 1] * f, z = X[3 * p + 2] * f;

@@ -612,8 +612,7 @@ bool fft_size_supported(int n) {
 // threads per block and tile width per line length
 template <int N>
 struct Cfg {
-  static constexpr int TW = (N >= 768) ? 8 : 16;
-  static constexpr int T = (N * TW >= 4096) ? 256 : 128;
+  static constexpr int T = (N * 16 >= 4096) ? 256 : 128;
   static constexpr int ZTW = (N >= 1024) ? 4 : (N >= 512 ? 8 : 16);
 };
 
@@ -637,9 +636,9 @@ static int zc2r_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   PMX_LAUNCH_CHECK();
   return kOk;
 }
-template <int N>
-static int y_n(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
-  constexpr int TW = Cfg<N>::TW, T = Cfg<N>::T;
+template <int N, int TW>
+static int y_nt(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
+  constexpr int T = (N * TW >= 4096) ? 256 : 128;
   const PlanDev& p = pl.d;
   const int nkt = (p.H + TW - 1) / TW;
   const int tiles = p.B * p.nx * nkt + p.B * ((p.nx + TW - 1) / TW);
@@ -652,9 +651,9 @@ static int y_n(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
   PMX_LAUNCH_CHECK();
   return kOk;
 }
-template <int N>
-static int x_n(const Plan& pl, const XJob& job, hipStream_t s) {
-  constexpr int TW = Cfg<N>::TW, T = Cfg<N>::T;
+template <int N, int TW>
+static int x_nt(const Plan& pl, const XJob& job, hipStream_t s) {
+  constexpr int T = (N * TW >= 4096) ? 256 : 128;
   const PlanDev& p = pl.d;
   const int nkt = (p.H + TW - 1) / TW;
   const int tiles = p.B * p.ny * nkt + p.B * ((p.ny + TW - 1) / TW);
@@ -662,6 +661,25 @@ static int x_n(const Plan& pl, const XJob& job, hipStream_t s) {
   hipLaunchKernelGGL((k_xpass<N, TW, T>), dim3(tiles), dim3(T), 0, s, job, p);
   PMX_LAUNCH_CHECK();
   return kOk;
+}
+// Column-tile width: 16 float2 (128-byte rows) or 8 (more workgroups per CU,
+// 64-byte rows); lines of 768 and more only fit LDS with 8.
+template <int N>
+static int y_n(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
+  if constexpr (N >= 768) {
+    return y_nt<N, 8>(pl, jobs, inv, s);
+  } else {
+    return pl.tw_y == 8 ? y_nt<N, 8>(pl, jobs, inv, s)
+                        : y_nt<N, 16>(pl, jobs, inv, s);
+  }
+}
+template <int N>
+static int x_n(const Plan& pl, const XJob& job, hipStream_t s) {
+  if constexpr (N >= 768) {
+    return x_nt<N, 8>(pl, job, s);
+  } else {
+    return pl.tw_x == 8 ? x_nt<N, 8>(pl, job, s) : x_nt<N, 16>(pl, job, s);
+  }
 }
 
 int launch_zpass_r2c(const Plan& p, const ZJobs& jobs, hipStream_t s) {

@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
   const int b = blockIdx.y;
   const int ntx = (d.nx + kTile - 1) / kTile, nty = (d.ny + kTile - 1) / kTile,
             ntz = (d.nz + kTile - 1) / kTile;
-  const int tt = blockIdx.x;
+  const int tt = (int)xcd_block(blockIdx.x, gridDim.x);
   const int tz = tt % ntz, ty = (tt / ntz) % nty, tx = tt / (ntz * nty);
   const int t0[3] = {tx * kTile, ty * kTile, tz * kTile};
   const int n[3] = {d.nx, d.ny, d.nz};

@@ -514,19 +514,31 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
   return force_meshes(p, nw.rho, nw.w, nw.f[0], nw.f[1], nw.f[2], sc, s);
 }
 
+// fjump / fstep (3 meshes each, may be null): where the force meshes of the
+// jump-start force and of the first step's force are left (the adjoint keeps
+// them instead of recomputing them).
 static int nbody_run(const Plan& p, float* state, long long N, float f,
                      float fac, const double* coef, int b0, int b1,
-                     int jump, float* work, hipStream_t s) {
+                     int jump, float* work, hipStream_t s,
+                     float* fjump = nullptr, float* fstep = nullptr) {
   const long long BN3 = (long long)p.d.B * N * 3;
+  const long long M = p.d.rows * p.d.nz;
   float* X = state;
   float* P = state + BN3;
   float* F = state + 2 * BN3;
   MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
-  NbodyWork nw = nbody_work(p, work);
+  const NbodyWork nw0 = nbody_work(p, work);
+  auto at = [&](float* dst) {
+    NbodyWork w = nw0;
+    if (dst)
+      for (int d = 0; d < 3; ++d) w.f[d] = dst + d * M;
+    return w;
+  };
   KickDrift kd{};
   kd.fac = fac;
   kd.f = f;
   if (jump) {
+    const NbodyWork nw = at(fjump);
     PMX_TRY(force_at(p, X, N, f, nw, s));
     kd.do_k2 = 0;
     kd.do_k1d = (b0 < b1);
@@ -545,6 +557,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     PMX_TRY(launch_kick_drift_only(X, P, F, md, c, s));
   }
   for (int i = b0; i < b1; ++i) {
+    const NbodyWork nw = at(i == b0 ? fstep : nullptr);
     PMX_TRY(force_at(p, X, N, f, nw, s));
     kd.do_k2 = 1;
     kd.k2 = (float)coef[3 * i + 2];
@@ -559,23 +572,28 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   return kOk;
 }
 
-// X-cotangent contribution of F = force(X) for cotangent gF.
+// X-cotangent contribution of F = force(X) for cotangent gF.  fm: the force
+// meshes at X from the forward pass (recomputed when null).
 static int force_vjp(const Plan& p, const float* X, const float* gF,
                      float* gX, long long N, float f, float fac,
-                     const NbodyWork& nw, float* gm, hipStream_t s) {
+                     const NbodyWork& nw, float* gm, const float* fm,
+                     hipStream_t s) {
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
-  PMX_TRY(force_at(p, X, N, f, nw, s));
+  const float* fx = fm ? fm : nw.f[0];
+  const float* fy = fm ? fm + M : nw.f[1];
+  const float* fz = fm ? fm + 2 * M : nw.f[2];
+  if (!fm) PMX_TRY(force_at(p, X, N, f, nw, s));
   float* gms[3] = {gm, gm + M, gm + 2 * M};
   const bool grid = paint_meshes(p, gms, 3, X, gF, 3, fac / f, N, f,
                                  nw.brick, s) == kOk;
   if (!grid) {
     for (int c = 0; c < 3; ++c) PMX_TRY(launch_fill(gms[c], M, 0.0f, s));
   }
-  PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X,
-                                     nw.f[0], nw.f[1], nw.f[2], md, f,
-                                     fac / f, grid ? 0 : 1, s));
+  PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X, fx,
+                                     fy, fz, md, f, fac / f, grid ? 0 : 1,
+                                     s));
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
   PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nw.rho, sc, s));
@@ -584,44 +602,58 @@ static int force_vjp(const Plan& p, const float* X, const float* gF,
   return kOk;
 }
 
-static long long nbody_adj_work_floats(const Plan& p, long long N) {
+// Adjoint workspace: nbody work, 3 cotangent meshes, a scratch state, X of
+// every step of a segment and the force meshes of every step of a segment
+// (plus the jump-start force).
+static long long nbody_adj_work_floats(const Plan& p, long long N, int every) {
   const long long M = p.d.rows * p.d.nz;
-  return nbody_work_floats(p) + 3 * M + 9 * (long long)p.d.B * N;
+  const long long BN3 = 3LL * p.d.B * N;
+  return nbody_work_floats(p) + 3 * M + 3 * BN3 + (long long)every * BN3 +
+         (long long)(every + 1) * 3 * M;
 }
 
 static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
                          int nsteps, long long N, float f, float fac,
-                         const double* coef, float* gstate, float* xslots,
-                         float* work, hipStream_t s) {
+                         const double* coef, float* gstate, float* work,
+                         hipStream_t s) {
   PMX_REQUIRE(every >= 1, "nbody_adjoint: every must be >= 1");
   const long long M = p.d.rows * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
   NbodyWork nw = nbody_work(p, work);
   float* gm = work + nbody_work_floats(p);
-  float* st = gm + 3 * M;  // scratch state (3, B, N, 3)
+  float* st = gm + 3 * M;            // scratch state (3, B, N, 3)
+  float* xslots = st + 3 * BN3;      // every * BN3
+  float* fslots = xslots + (long long)every * BN3;  // (every + 1) * 3M
+  float* fjump = fslots + (long long)every * 3 * M;
   float* gX = gstate;
   float* gP = gstate + BN3;
   float* gF = gstate + 2 * BN3;
   const unsigned g = gridfor(BN3);
   const int nseg = (nsteps + every - 1) / every;
-  if (nsteps > 0)
+  if (nsteps > 0) {
     hipLaunchKernelGGL(k_adj_update, dim3(g), dim3(256), 0, s, gX, gP, gF,
                        BN3, 0.f, 0.f, (float)coef[3 * (nsteps - 1) + 2], 0);
+    PMX_LAUNCH_CHECK();
+  }
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int b0 = sg * every, b1 = std::min(nsteps, b0 + every);
     PMX_HIP_CHECK(hipMemcpyAsync(st, ckpt + (long long)sg * 3 * BN3,
                                  sizeof(float) * 3 * BN3,
                                  hipMemcpyDeviceToDevice, s));
+    // recompute the segment, keeping X and the force meshes of every step
     for (int i = b0; i < b1; ++i) {
-      PMX_TRY(nbody_run(p, st, N, f, fac, coef, i, i + 1, sg == 0 && i == 0,
-                        work, s));
+      const bool jump = sg == 0 && i == 0;
+      PMX_TRY(nbody_run(p, st, N, f, fac, coef, i, i + 1, jump, work, s,
+                        jump ? fjump : nullptr,
+                        fslots + (long long)(i - b0) * 3 * M));
       PMX_HIP_CHECK(hipMemcpyAsync(xslots + (long long)(i - b0) * BN3, st,
                                    sizeof(float) * BN3,
                                    hipMemcpyDeviceToDevice, s));
     }
     for (int i = b1 - 1; i >= b0; --i) {
       PMX_TRY(force_vjp(p, xslots + (long long)(i - b0) * BN3, gF, gX, N, f,
-                        fac, nw, gm, s));
+                        fac, nw, gm, fslots + (long long)(i - b0) * 3 * M,
+                        s));
       const int mode = i > 0 ? 2 : 1;
       hipLaunchKernelGGL(k_adj_update, dim3(g), dim3(256), 0, s, gX, gP, gF,
                          BN3, (float)coef[3 * i + 1], (float)coef[3 * i],
@@ -630,7 +662,8 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
     }
   }
   // jump-start force at the initial positions; the initial F is discarded
-  PMX_TRY(force_vjp(p, ckpt, gF, gX, N, f, fac, nw, gm, s));
+  PMX_TRY(force_vjp(p, ckpt, gF, gX, N, f, fac, nw, gm,
+                    nsteps > 0 ? fjump : nullptr, s));
   PMX_TRY(launch_fill(gF, BN3, 0.0f, s));
   return kOk;
 }
@@ -1061,20 +1094,21 @@ int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
                    S(stream));
 }
 
-long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N) {
-  return plan ? nbody_adj_work_floats(P(plan), N) : -1;
+long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N,
+                                    int every) {
+  return (plan && every >= 1) ? nbody_adj_work_floats(P(plan), N, every) : -1;
 }
 
 int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
                       int nsteps, long long N, float f, float fac,
-                      const double* coef, float* gstate, float* xslots,
-                      float* work, void* stream) {
-  PMX_REQUIRE(plan && ckpt && gstate && xslots && work,
+                      const double* coef, float* gstate, float* work,
+                      void* stream) {
+  PMX_REQUIRE(plan && ckpt && gstate && work,
               "pmx_nbody_adjoint: null pointer");
   PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
               "pmx_nbody_adjoint: bad steps");
   return nbody_adjoint(P(plan), ckpt, every, nsteps, N, f, fac, coef, gstate,
-                       xslots, work, S(stream));
+                       work, S(stream));
 }
 
 }  // extern "C"

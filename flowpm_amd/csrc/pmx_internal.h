@@ -96,6 +96,16 @@ struct ProfScope {
   }
 };
 
+// XCD-aware block order: the hardware deals workgroups round-robin to the 8
+// XCDs (block i -> XCD i % 8).  Return a logical block index such that each
+// XCD walks one contiguous chunk of the grid, so neighbouring particle rows
+// (which gather the same mesh rows) share one L2.
+__device__ __forceinline__ long long xcd_block(long long i, long long nb) {
+  const long long q = nb / 8, r = nb % 8;
+  const long long c = i % 8, slot = i / 8;
+  return c * q + (c < r ? c : r) + slot;
+}
+
 // ------------------------------------------------------ plan (device) ------
 // Per-axis tables, computed on the host in float64 and uploaded once.
 struct AxisTab {

@@ -359,13 +359,13 @@ class _Nbody(torch.autograd.Function):
     plan, N, f, fac, coef, every = ctx.args
     nsteps = len(coef) // 3
     gs = _f32(g).clone()
-    xslots = torch.empty((min(every, max(nsteps, 1)), plan.B, N, 3),
-                         dtype=torch.float32, device=gs.device)
-    work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N),
+    k = min(every, max(nsteps, 1))
+    work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
+                                                            k),
                        dtype=torch.float32, device=gs.device)
-    _lib.call("pmx_nbody_adjoint", plan.handle, _lib.ptr(ckpt), every, nsteps,
-              N, f, fac, _lib.dbl(coef), _lib.ptr(gs), _lib.ptr(xslots),
-              _lib.ptr(work), _lib.stream_of(gs))
+    _lib.call("pmx_nbody_adjoint", plan.handle, _lib.ptr(ckpt), k, nsteps,
+              N, f, fac, _lib.dbl(coef), _lib.ptr(gs), _lib.ptr(work),
+              _lib.stream_of(gs))
     return gs, None, None, None, None, None, None, None
 
 

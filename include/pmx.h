@@ -187,13 +187,15 @@ int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
  * states at the start of every segment of `every` steps (segment s starts
  * at step s*every; segment 0 at step 0 *before* the jump-start force);
  * gstate (3,B,N,3) holds the cotangent of the final state on entry and the
- * cotangent of the initial state on exit.  xslots: every*B*N*3 floats;
- * work: pmx_nbody_adj_work_floats() floats. */
-long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N);
+ * cotangent of the initial state on exit.  Each segment is recomputed from
+ * its checkpoint keeping X and the force meshes of every step, then swept
+ * in reverse.  work: pmx_nbody_adj_work_floats(plan, N, every) floats. */
+long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N,
+                                    int every);
 int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
                       int nsteps, long long N, float f, float fac,
-                      const double* coef, float* gstate, float* xslots,
-                      float* work, void* stream);
+                      const double* coef, float* gstate, float* work,
+                      void* stream);
 
 #ifdef __cplusplus
 }
