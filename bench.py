@@ -65,6 +65,11 @@ def main():
   ap.add_argument("--every", type=int, default=8)
   ap.add_argument("--no-adjoint", action="store_true")
   ap.add_argument("--no-cpu-baseline", action="store_true")
+  ap.add_argument("--no-batched", action="store_true")
+  ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                  help="first run a plain device copy of this many MiB "
+                  "(the known-size kernel that scales rocprofv3 "
+                  "FETCH_SIZE / WRITE_SIZE counters)")
   args = ap.parse_args()
   if args.gpus != 1:
     print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -77,6 +82,13 @@ def main():
   from flowpm_amd import _plans
 
   dev = torch.device("cuda:0")
+  if args.copy_calibration_mb:
+    src = torch.ones(args.copy_calibration_mb << 18, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+      dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
   n, L, K, W = args.n, args.box, args.steps, args.warmup
   N = n**3
   cosmo = fa.cosmology.Planck15()
@@ -164,6 +176,10 @@ def main():
     fa_ms = e0.elapsed_time(e1) / K
     assert torch.isfinite(gx).all()
 
+  batched = None
+  if not args.no_batched:
+    batched = batched_runs(fa, cosmo, dev)
+
   cpu = None
   if not args.no_cpu_baseline:
     cpu = cpu_baseline(state0, n, cosmo)
@@ -205,9 +221,67 @@ def main():
           (algorithmic_bytes(k, n, N) or 0) * v["launches"] / (K + 1)
           for k, v in kernels.items()),
       "kernels": kernels,
+      "batched": batched,
       "cpu_baseline": cpu,
   }
   print(json.dumps(out))
+
+
+def _ic_states(fa, cosmo, n, L, B, seed, dev):
+  kk = np.logspace(-4, 1, 256)
+  pk_tab = fa.tfpower.linear_matter_power(cosmo, kk)
+  pk = lambda k: fa.scipy.interpolate.interp_tf(k, kk, pk_tab)
+  gen = torch.Generator().manual_seed(seed)
+  white = (torch.randn(B, n, n, n, generator=gen, dtype=torch.float64) *
+           n**1.5).float().to(dev)
+  return fa.lpt_init(cosmo, fa.whitenoise_to_linear(n, L, white, pk), 0.1,
+                     order=1)
+
+
+def _time(fn, reps):
+  fn()
+  torch.cuda.synchronize()
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(reps):
+    fn()
+  e1.record()
+  torch.cuda.synchronize()
+  return e0.elapsed_time(e1) / reps
+
+
+def batched_runs(fa, cosmo, dev):
+  """Forward nbody throughput of many simulations (ICs excluded):
+  64^3 sims (box 200, 1LPT a0=0.1, 10 steps) batched in one plan under a
+  captured graph vs one at a time; and the uneven mix of 8 each of
+  32/64/96/128^3 (boxes 100/200/300/400) under one graph."""
+  from flowpm_amd.batched import BatchedNbody
+  stages = np.linspace(0.1, 1.0, 11)
+  B = 64
+  st = _ic_states(fa, cosmo, 64, 200.0, B, 1000, dev)
+  runner = BatchedNbody([dict(cosmo=cosmo, state=st, stages=stages, nc=64)])
+  t_b = _time(runner.run, 3)
+  one = BatchedNbody([dict(cosmo=cosmo, state=st[:, :1].contiguous(),
+                           stages=stages, nc=64)], graph=False)
+  t_1 = _time(one.run, 5)
+  groups = []
+  for i, (n, L) in enumerate(((32, 100.), (64, 200.), (96, 300.),
+                              (128, 400.))):
+    groups.append(dict(cosmo=cosmo, state=_ic_states(fa, cosmo, n, L, 8,
+                                                     2000 + i, dev),
+                       stages=stages, nc=n))
+  mix = BatchedNbody(groups)
+  t_mix = _time(mix.run, 3)
+  return {
+      "sims64_batch": B,
+      "sims64_per_s_batched": B / (t_b * 1e-3),
+      "sims64_per_s_sequential": 1.0 / (t_1 * 1e-3),
+      "speedup_vs_sequential": (B / t_b) / (1.0 / t_1),
+      "uneven_mix_ms": t_mix,
+      "uneven_mix_sims_per_s": 32 / (t_mix * 1e-3),
+      "note": "10-step forward nbody per sim, ICs excluded; graph replay",
+  }
 
 
 def cpu_baseline(state0, n, cosmo):

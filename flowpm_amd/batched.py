@@ -1,13 +1,15 @@
 """Many simulations of uneven mesh sizes advanced together.
 
 The reference batches only simulations of one size (a leading batch axis,
-flowpm/tfpm.py:15-63); here each size class is one batched plan and all
-classes advance step by step on one stream.  The whole KDK loop of every
+flowpm/tfpm.py:15-63); here each size class is one batched plan.  The whole KDK loop of every
 class is captured once into a HIP graph (the library's launches are plain
 stream work, so torch's graph capture records them) and replayed: one
 launch of the graph runs every class for every step with no host
-round trip.  Coefficients are per class (each class may have its own
-cosmology and stages).
+round trip.  Each class runs on its own stream (default priority) forked
+from and joined back into the capture stream, so the graph holds one branch
+per class and small meshes execute concurrently with large ones instead of
+queueing behind them.  Coefficients are per class (each class may have its
+own cosmology and stages).
 """
 import numpy as np
 import torch
@@ -50,14 +52,22 @@ class BatchedNbody:
     self.groups = [_Group(g["cosmo"], g["state"], g["stages"], g["nc"],
                           g.get("pm_nc_factor", 1)) for g in groups]
     self.init = [g.state.clone() for g in self.groups]
+    self.streams = [torch.cuda.Stream() for _ in self.groups]
     self.use_graph = graph
     self.graph = None
     self.nsteps = max(max(g.nsteps for g in self.groups), 1)
 
-  def _loop(self, stream):
-    for i in range(self.nsteps):
-      for g in self.groups:
-        g.step(i, stream)
+  def _loop(self):
+    main = torch.cuda.current_stream()
+    for st in self.streams:
+      st.wait_stream(main)
+    for g, st in zip(self.groups, self.streams):
+      with torch.cuda.stream(st):
+        sp = _lib._vp(st.cuda_stream)
+        for i in range(self.nsteps):
+          g.step(i, sp)
+    for st in self.streams:
+      main.wait_stream(st)
 
   def reset(self, states=None):
     for k, g in enumerate(self.groups):
@@ -67,12 +77,12 @@ class BatchedNbody:
   def run(self):
     if not self.use_graph:
       self.reset()
-      self._loop(_lib.stream_of(self.groups[0].state))
+      self._loop()
       return [g.state for g in self.groups]
     if self.graph is None:
       # warm once eagerly (loads kernels), then capture
       self.reset()
-      self._loop(_lib.stream_of(self.groups[0].state))
+      self._loop()
       torch.cuda.synchronize()
       self.graph = torch.cuda.CUDAGraph()
       s = torch.cuda.Stream()
@@ -80,7 +90,7 @@ class BatchedNbody:
       with torch.cuda.stream(s):
         self.reset()
         with torch.cuda.graph(self.graph, stream=s):
-          self._loop(_lib.stream_of(self.groups[0].state))
+          self._loop()
       torch.cuda.current_stream().wait_stream(s)
       torch.cuda.synchronize()
     self.reset()
