@@ -194,53 +194,73 @@ __global__ __launch_bounds__(256) void k_readout_adj(
 // F_d = (readout(mesh_d, f X) * fac) / f          tfpm.py:352-356
 // P  += k2 F                                      tfpm.py:284-291
 // P  += k1 F ; X += d P                           tfpm.py:306-313
+// The (B,N,3) state rows are moved through LDS so that every global load
+// and store of X, P and F is a contiguous, coalesced dword stream instead
+// of three 12-byte-strided accesses per particle.
 __global__ __launch_bounds__(256) void k_readout_kick_drift(
     float* __restrict__ X, float* __restrict__ P, float* __restrict__ F,
     const float* __restrict__ fxm, const float* __restrict__ fym,
     const float* __restrict__ fzm, MeshDims d, KickDrift c) {
+  __shared__ float sx[768], sp[768], sf[768];
   const long long tot = (long long)d.B * d.N;
-  const long long p =
-      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
-  if (p >= tot) return;
-  const int b = (int)(p / d.N);
-  float px = X[3 * p], py = X[3 * p + 1], pz = X[3 * p + 2];
-  const float x = px * c.f, y = py * c.f, z = pz * c.f;
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  if (pos_ok(x, y, z)) {
-    const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
-    PMX_CORNERS({
-      const float k = (wx * wy) * wz;
-      ax += fxm[cell] * k;
-      ay += fym[cell] * k;
-      az += fzm[cell] * k;
-    })
+  const long long p0 = xcd_block(blockIdx.x, gridDim.x) * 256;
+  const int np = (int)min((long long)256, tot - p0);
+  const int nf = 3 * np;
+  const bool kick = c.do_k2 || c.do_k1d;
+  const int t = threadIdx.x;
+  for (int i = t; i < nf; i += 256) {
+    sx[i] = X[3 * p0 + i];
+    if (kick) sp[i] = P[3 * p0 + i];
   }
-  const float Fx = (ax * c.fac) / c.f, Fy = (ay * c.fac) / c.f,
-              Fz = (az * c.fac) / c.f;
-  F[3 * p] = Fx;
-  F[3 * p + 1] = Fy;
-  F[3 * p + 2] = Fz;
-  if (c.do_k2 || c.do_k1d) {
-    float vx = P[3 * p], vy = P[3 * p + 1], vz = P[3 * p + 2];
-    if (c.do_k2) {
-      vx = vx + c.k2 * Fx;
-      vy = vy + c.k2 * Fy;
-      vz = vz + c.k2 * Fz;
+  __syncthreads();
+  if (t < np) {
+    const long long p = p0 + t;
+    const int b = (int)(p / d.N);
+    float px = sx[3 * t], py = sx[3 * t + 1], pz = sx[3 * t + 2];
+    const float x = px * c.f, y = py * c.f, z = pz * c.f;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    if (pos_ok(x, y, z)) {
+      const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+      PMX_CORNERS({
+        const float k = (wx * wy) * wz;
+        ax += fxm[cell] * k;
+        ay += fym[cell] * k;
+        az += fzm[cell] * k;
+      })
     }
-    if (c.do_k1d) {
-      vx = vx + c.k1 * Fx;
-      vy = vy + c.k1 * Fy;
-      vz = vz + c.k1 * Fz;
-      px = px + c.d * vx;
-      py = py + c.d * vy;
-      pz = pz + c.d * vz;
-      X[3 * p] = px;
-      X[3 * p + 1] = py;
-      X[3 * p + 2] = pz;
+    const float Fx = (ax * c.fac) / c.f, Fy = (ay * c.fac) / c.f,
+                Fz = (az * c.fac) / c.f;
+    sf[3 * t] = Fx;
+    sf[3 * t + 1] = Fy;
+    sf[3 * t + 2] = Fz;
+    if (kick) {
+      float vx = sp[3 * t], vy = sp[3 * t + 1], vz = sp[3 * t + 2];
+      if (c.do_k2) {
+        vx = vx + c.k2 * Fx;
+        vy = vy + c.k2 * Fy;
+        vz = vz + c.k2 * Fz;
+      }
+      if (c.do_k1d) {
+        vx = vx + c.k1 * Fx;
+        vy = vy + c.k1 * Fy;
+        vz = vz + c.k1 * Fz;
+        px = px + c.d * vx;
+        py = py + c.d * vy;
+        pz = pz + c.d * vz;
+        sx[3 * t] = px;
+        sx[3 * t + 1] = py;
+        sx[3 * t + 2] = pz;
+      }
+      sp[3 * t] = vx;
+      sp[3 * t + 1] = vy;
+      sp[3 * t + 2] = vz;
     }
-    P[3 * p] = vx;
-    P[3 * p + 1] = vy;
-    P[3 * p + 2] = vz;
+  }
+  __syncthreads();
+  for (int i = t; i < nf; i += 256) {
+    F[3 * p0 + i] = sf[i];
+    if (kick) P[3 * p0 + i] = sp[i];
+    if (c.do_k1d) X[3 * p0 + i] = sx[i];
   }
 }
 

@@ -7,12 +7,14 @@
 //   particles (e.g. 8x16x16).  Particles that start close stay close, so
 //   the brick's deposits fit a small box.  The workgroup finds the box,
 //   accumulates all 8*C deposits per particle with LDS float atomics, and
-//   stores the dense box to a scratch arena (bump-allocated, plain stores).
+//   stores the dense box to its own arena slot (plain stores).
 //   A brick whose box does not fit the LDS budget or the arena falls back
 //   to device-scope float atomics straight into the mesh.
-// phase 2 (k_brick_gather): one workgroup per 16^3 mesh tile sums, in a
-//   fixed brick order, the parts of every box that overlap it and writes
-//   the tile with plain stores.
+// phase 2 (k_brick_gather): one workgroup per 16^3 mesh tile evaluates the
+//   candidate bricks in parallel, compacts those whose boxes meet the tile
+//   (in candidate order), and every thread adds them, in that order, to the
+//   16 cells it owns; the tile is written with plain stores, no barrier per
+//   brick.
 //
 // Device-scope float atomics cannot complete inside one XCD's L2 (the eight
 // L2s are not coherent with each other), so the plain scatter pays a far
@@ -33,8 +35,7 @@ struct BrickArgs {
   float* arena;          // box arena
   long long arena_cap;   // floats
   int* meta;             // 8 ints per brick: lo[3], ext[3], off lo/hi
-  int* ctr;              // [0..1] arena bump (64-bit), [2] overflow count,
-                         // [3] box radius
+  int* ctr;              // [2] overflow count, [3] box radius
   const float* X;        // positions [B][N][3]
   const float* w;        // weights [B][N][wstride] or null
   int wstride;
@@ -137,28 +138,25 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
       if (small) vol = (long long)ext[0] * ext[1] * ext[2];
     }
     int mode = 0;  // 0: nothing, 1: LDS box, 2: device atomics
-    long long off = 0;
+    const long long off =
+        ((long long)b * g.nbricks + bid) * (long long)kBoxFloats;
     if (any) {
-      bool fits = small && vol * C <= kBoxFloats;
-      if (fits) {
-        // bump-allocate the arena (one atomic per workgroup)
-        off = (long long)atomicAdd(reinterpret_cast<unsigned long long*>(
-                                       a.ctr),
-                                   (unsigned long long)(vol * C));
-        fits = off + vol * C <= a.arena_cap;
-      }
-      mode = fits ? 1 : 2;
+      // every brick owns a fixed arena slot: no shared allocator
+      mode = (small && vol * C <= kBoxFloats) ? 1 : 2;
       if (mode == 2) atomicAdd(a.ctr + 2, 1);
-      // box radius around the brick's own mesh interval, for the gather
-      const int q0[3] = {q0x, q0y, q0z}, cc[3] = {cx, cy, cz};
       if (mode == 1) {
+        // box radius around the brick's own mesh interval, for the gather;
+        // read first so that only a new maximum costs a device atomic
+        const int q0[3] = {q0x, q0y, q0z}, cc[3] = {cx, cy, cz};
         int R = 0;
         for (int d = 0; d < 3; ++d) {
           const int f = g.f;
           R = max(R, q0[d] * f - lo[d]);
           R = max(R, lo[d] + ext[d] - (q0[d] + cc[d]) * f);
         }
-        atomicMax(a.ctr + 3, R);
+        if (R > __hip_atomic_load(a.ctr + 3, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT))
+          atomicMax(a.ctr + 3, R);
       }
     }
     int* m = a.meta + 8LL * ((long long)b * g.nbricks + bid);
@@ -168,8 +166,8 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     m[3] = mode == 1 ? ext[0] : 0;
     m[4] = mode == 1 ? ext[1] : 0;
     m[5] = mode == 1 ? ext[2] : 0;
-    m[6] = (int)(off & 0x7fffffff);
-    m[7] = (int)(off >> 31);
+    m[6] = 0;
+    m[7] = 0;
     sh[0] = mode;
     sh[1] = lo[0];
     sh[2] = lo[1];
@@ -178,8 +176,8 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     sh[5] = ext[1];
     sh[6] = ext[2];
     sh[7] = (int)vol;
-    sh[8] = m[6];
-    sh[9] = m[7];
+    sh[8] = (int)(off & 0x7fffffff);
+    sh[9] = (int)(off >> 31);
   }
   __syncthreads();
   const int mode = sh[0];
@@ -283,11 +281,20 @@ __device__ __forceinline__ bool brick_hits(int k, int e, int gdim, int f,
   return (s < t0 + T && s1 > t0) || (s < t0 + T + n && s1 > t0 + n);
 }
 
+// does the wrapped box [am, am + ext) (0 <= am < n, ext <= n) meet the tile
+// [t0, t0 + T) (t0 + T <= n)?
+__device__ __forceinline__ bool box_meets(int am, int ext, int t0, int T,
+                                          int n) {
+  return (am < t0 + T && am + ext > t0) || (am + ext - n > t0);
+}
+
 template <int C>
 __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
-  __shared__ float acc[C * kTile * kTile * kTile];
+  constexpr int PER = kTile;  // owned cells per thread: (k, iy, iz), k < 16
   __shared__ short cand[3][1024];
   __shared__ int ncand[3];
+  __shared__ int lst[256][8];  // overlapping bricks of one chunk
+  __shared__ int wcnt[4];
   const BrickGrid& g = a.g;
   const MeshDims& d = a.d;
   const int tid = threadIdx.x;
@@ -296,23 +303,23 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
             ntz = (d.nz + kTile - 1) / kTile;
   const int tt = (int)xcd_block(blockIdx.x, gridDim.x);
   const int tz = tt % ntz, ty = (tt / ntz) % nty, tx = tt / (ntz * nty);
-  const int t0[3] = {tx * kTile, ty * kTile, tz * kTile};
-  const int n[3] = {d.nx, d.ny, d.nz};
-  const int T[3] = {min(kTile, d.nx - t0[0]), min(kTile, d.ny - t0[1]),
-                    min(kTile, d.nz - t0[2])};
-  for (int i = tid; i < C * kTile * kTile * kTile; i += 256) acc[i] = 0.0f;
+  const int t0x = tx * kTile, t0y = ty * kTile, t0z = tz * kTile;
+  const int Tx = min(kTile, d.nx - t0x), Ty = min(kTile, d.ny - t0y),
+            Tz = min(kTile, d.nz - t0z);
   const int R = a.ctr[3];
-  // candidate bricks per dimension (wave d builds list d)
+  // candidate bricks per dimension (wave w builds list w)
   const int wv = tid >> 6, ln = tid & 63;
   if (wv < 3) {
     const int e = wv == 0 ? g.ex : (wv == 1 ? g.ey : g.ez);
     const int gd = wv == 0 ? g.gx : (wv == 1 ? g.gy : g.gz);
     const int nb = wv == 0 ? g.nbx : (wv == 1 ? g.nby : g.nbz);
+    const int t0 = wv == 0 ? t0x : (wv == 1 ? t0y : t0z);
+    const int T = wv == 0 ? Tx : (wv == 1 ? Ty : Tz);
+    const int n = wv == 0 ? d.nx : (wv == 1 ? d.ny : d.nz);
     int base = 0;
     for (int k0 = 0; k0 < nb; k0 += 64) {
       const int k = k0 + ln;
-      const bool hit =
-          k < nb && brick_hits(k, e, gd, g.f, R, t0[wv], T[wv], n[wv]);
+      const bool hit = k < nb && brick_hits(k, e, gd, g.f, R, t0, T, n);
       const unsigned long long mask = __ballot(hit);
       if (hit) {
         const int pos = base + __popcll(mask & ((1ULL << ln) - 1ULL));
@@ -324,54 +331,95 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
   }
   __syncthreads();
   const int nc0 = ncand[0], nc1 = ncand[1], nc2 = ncand[2];
-  for (int i0 = 0; i0 < nc0; ++i0)
-    for (int i1 = 0; i1 < nc1; ++i1)
-      for (int i2 = 0; i2 < nc2; ++i2) {
-        const int bid =
-            (cand[0][i0] * g.nby + cand[1][i1]) * g.nbz + cand[2][i2];
-        const int* m = a.meta + 8LL * ((long long)b * g.nbricks + bid);
-        const int ex0 = m[3], ex1 = m[4], ex2 = m[5];
-        if (ex0 == 0) continue;
-        const Pieces px = overlap(m[0], ex0, t0[0], T[0], n[0]);
-        if (px.n == 0) continue;
-        const Pieces py = overlap(m[1], ex1, t0[1], T[1], n[1]);
-        if (py.n == 0) continue;
-        const Pieces pz = overlap(m[2], ex2, t0[2], T[2], n[2]);
-        if (pz.n == 0) continue;
-        const long long off = (long long)m[6] | ((long long)m[7] << 31);
-        const float* src = a.arena + off;
-        const int vol = ex0 * ex1 * ex2;
-        for (int jx = 0; jx < px.n; ++jx)
-          for (int jy = 0; jy < py.n; ++jy)
-            for (int jz = 0; jz < pz.n; ++jz) {
-              const int lx = px.len[jx], ly = py.len[jy], lz = pz.len[jz];
-              const int cnt = lx * ly * lz;
-              for (int i = tid; i < cnt; i += 256) {
-                const int iz = i % lz, r = i / lz, iy = r % ly, ix = r / ly;
-                const int ti = ((px.ts[jx] + ix) * kTile + py.ts[jy] + iy) *
-                                   kTile + pz.ts[jz] + iz;
-                const int bi = ((px.bo[jx] + ix) * ex1 + py.bo[jy] + iy) * ex2 +
-                               pz.bo[jz] + iz;
+  const int total = nc0 * nc1 * nc2;
+  // owned cells: (x = t0x + k, y = t0y + iy, z = t0z + iz), k = 0..15
+  const int iy = tid / kTile, iz = tid % kTile;
+  const int cy = t0y + iy, cz = t0z + iz;
+  const bool yz_ok = iy < Ty && iz < Tz;
+  float acc[C][PER];
 #pragma unroll
-                for (int c = 0; c < C; ++c)
-                  acc[c * kTile * kTile * kTile + ti] += src[c * vol + bi];
-              }
-            }
-        __syncthreads();
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int k = 0; k < PER; ++k) acc[c][k] = 0.0f;
+  for (int base = 0; base < total; base += 256) {
+    // 1. evaluate one candidate per thread, compact the overlapping ones in
+    //    candidate order (ballot within a wave, wave counts across)
+    const int ci = base + tid;
+    bool ok = false;
+    int rec[7];
+    if (ci < total) {
+      const int i2 = ci % nc2, r = ci / nc2, i1 = r % nc1, i0 = r / nc1;
+      const int bid = (cand[0][i0] * g.nby + cand[1][i1]) * g.nbz + cand[2][i2];
+      const int* m = a.meta + 8LL * ((long long)b * g.nbricks + bid);
+      const int e0 = m[3], e1 = m[4], e2 = m[5];
+      if (e0 > 0) {
+        const int a0 = imod(m[0], d.nx), a1 = imod(m[1], d.ny),
+                  a2 = imod(m[2], d.nz);
+        ok = box_meets(a0, e0, t0x, Tx, d.nx) &&
+             box_meets(a1, e1, t0y, Ty, d.ny) &&
+             box_meets(a2, e2, t0z, Tz, d.nz);
+        rec[0] = bid;
+        rec[1] = a0;
+        rec[2] = a1;
+        rec[3] = a2;
+        rec[4] = e0;
+        rec[5] = e1;
+        rec[6] = e2;
       }
-  __syncthreads();
-  const bool rmw = a.ctr[2] != 0;  // some bricks went straight to the mesh
-  const int cnt = T[0] * T[1] * T[2];
-  for (int i = tid; i < cnt; i += 256) {
-    const int iz = i % T[2], r = i / T[2], iy = r % T[1], ix = r / T[1];
-    const long long cell =
-        (((long long)b * d.nx + t0[0] + ix) * d.ny + t0[1] + iy) * d.nz +
-        t0[2] + iz;
-    const int ti = (ix * kTile + iy) * kTile + iz;
+    }
+    const unsigned long long mask = __ballot(ok);
+    if (ln == 0) wcnt[wv] = __popcll(mask);
+    __syncthreads();
+    int off = 0, nl = 0;
+    for (int w = 0; w < 4; ++w) {
+      if (w < wv) off += wcnt[w];
+      nl += wcnt[w];
+    }
+    if (ok) {
+      const int pos = off + __popcll(mask & ((1ULL << ln) - 1ULL));
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const float v = acc[c * kTile * kTile * kTile + ti];
-      a.mesh[c][cell] = rmw ? a.mesh[c][cell] + v : v;
+      for (int q = 0; q < 7; ++q) lst[pos][q] = rec[q];
+    }
+    __syncthreads();
+    // 2. every thread adds the bricks, in list order, to its own cells
+    if (yz_ok) {
+      for (int j = 0; j < nl; ++j) {
+        const int e1 = lst[j][5], e2 = lst[j][6];
+        int oy = cy - lst[j][2];
+        if (oy < 0) oy += d.ny;
+        int oz = cz - lst[j][3];
+        if (oz < 0) oz += d.nz;
+        if (oy >= e1 || oz >= e2) continue;
+        const int e0 = lst[j][4];
+        const int vol = e0 * e1 * e2;
+        const float* src =
+            a.arena + ((long long)b * g.nbricks + lst[j][0]) *
+                          (long long)kBoxFloats;
+        const int a0 = lst[j][1];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+          int ox = t0x + k - a0;
+          if (ox < 0) ox += d.nx;
+          if (k < Tx && ox < e0) {
+            const int bi = (ox * e1 + oy) * e2 + oz;
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c][k] += src[c * vol + bi];
+          }
+        }
+      }
+    }
+    __syncthreads();  // lst is rewritten by the next chunk
+  }
+  if (!yz_ok) return;
+  const bool rmw = a.ctr[2] != 0;  // some bricks went straight to the mesh
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    if (k < Tx) {
+      const long long cell =
+          (((long long)b * d.nx + t0x + k) * d.ny + cy) * d.nz + cz;
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        a.mesh[c][cell] = rmw ? a.mesh[c][cell] + acc[c][k] : acc[c][k];
     }
   }
 }
@@ -401,8 +449,8 @@ bool brick_paint_ok(const MeshDims& d, const BrickGrid& g) {
 }
 
 long long brick_work_floats(const MeshDims& d, const BrickGrid& g, int C) {
-  const long long M = (long long)d.B * d.nx * d.ny * d.nz;
-  const long long arena = 3 * M * C;
+  (void)C;
+  const long long arena = (long long)d.B * g.nbricks * kBoxFloats;
   const long long meta = 8LL * d.B * g.nbricks;
   return arena + meta + 16;
 }
@@ -412,11 +460,10 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
                        const MeshDims& d, const BrickGrid& g, float* work,
                        hipStream_t s) {
   if ((long long)d.B * d.N == 0) return kOk;
-  const long long M = (long long)d.B * d.nx * d.ny * d.nz;
   BrickArgs a{};
   for (int c = 0; c < C; ++c) a.mesh[c] = mesh[c];
   a.arena = work;
-  a.arena_cap = 3 * M * C;
+  a.arena_cap = (long long)d.B * g.nbricks * kBoxFloats;
   a.meta = reinterpret_cast<int*>(work + a.arena_cap);
   a.ctr = a.meta + 8LL * d.B * g.nbricks;
   a.X = X;
