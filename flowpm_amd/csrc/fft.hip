@@ -226,10 +226,21 @@ __global__ __launch_bounds__(T) void k_zpass_r2c(ZJobs jobs, PlanDev p) {
   const int nr = (int)min((long long)(2 * TW), p.rows - r0);
   const float* __restrict__ src = J.rin + r0 * N;
   float* sf = reinterpret_cast<float*>(s);
-  for (int q = threadIdx.x; q < 2 * TW * N; q += T) {
-    const int row = q / N, z = q % N;
-    const float v = row < nr ? src[q] : 0.0f;
-    sf[2 * (z * P + (row >> 1)) + (row & 1)] = v;
+  {
+    constexpr int NE = 2 * TW * N;
+    constexpr int E = (NE + T - 1) / T;
+    float v[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
+      v[u] = ((NE % T == 0 || q < NE) && q / N < nr) ? src[q] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
+      const int row = q / N, z = q % N;
+      if (NE % T == 0 || q < NE) sf[2 * (z * P + (row >> 1)) + (row & 1)] = v[u];
+    }
   }
   __syncthreads();
   fft_tile<N, TW, T, -1>(s, tw);
@@ -274,14 +285,29 @@ __global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
   const float2* __restrict__ src = J.cin + r0 * H;
   const float* __restrict__ gz = p.ax[2].g;
   const int ez = J.ez;
-  for (int q = threadIdx.x; q < 2 * TW * H; q += T) {
-    const int row = q / H, k = q % H;
-    float2 v = row < nr ? src[q] : make_float2(0.f, 0.f);
-    if (ez) {
-      const float g = gz[k];
-      v = rot_i(cscale(v, ez == 2 ? g * g : g), ez);
+  {
+    constexpr int NE = 2 * TW * H;
+    constexpr int E = (NE + T - 1) / T;
+    float2 v[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
+      v[u] = ((NE % T == 0 || q < NE) && q / H < nr) ? src[q]
+                                                     : make_float2(0.f, 0.f);
     }
-    h[row * HP + k] = v;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
+      if (NE % T == 0 || q < NE) {
+        const int row = q / H, k = q % H;
+        float2 x = v[u];
+        if (ez) {
+          const float g = gz[k];
+          x = rot_i(cscale(x, ez == 2 ? g * g : g), ez);
+        }
+        h[row * HP + k] = x;
+      }
+    }
   }
   for (int row = threadIdx.x; row < 2 * TW; row += T) {
     float2 v = row < nr ? J.cin[p.nyq0 + r0 + row] : make_float2(0.f, 0.f);
@@ -398,22 +424,45 @@ __device__ __forceinline__ int main_kz0(const Tile& tl, const PlanDev& p) {
   return (int)(tl.base % p.H);
 }
 
+// Tile moves between HBM and LDS.  The per-thread element count is a
+// compile-time constant and the loops are unrolled in two phases (all global
+// loads into registers, then all LDS stores), so every thread keeps E loads
+// in flight instead of one load-wait-store chain per element.
 template <int N, int TW, int T>
 __device__ __forceinline__ void load_tile(float2* __restrict__ s,
                                           const float2* __restrict__ g,
                                           const Tile& tl) {
   constexpr int P = Pitch<TW>::P;
+  constexpr int NE = N * TW;
+  constexpr int E = (NE + T - 1) / T;
+  float2 v[E];
   if (tl.cs == 1) {
-    for (int q = threadIdx.x; q < N * TW; q += T) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
       const int i = q / TW, c = q % TW;
-      s[i * P + c] = c < tl.ncol ? g[tl.base + i * tl.ls + c]
-                                 : make_float2(0.f, 0.f);
+      v[u] = ((NE % T == 0 || q < NE) && c < tl.ncol)
+                 ? g[tl.base + i * tl.ls + c]
+                 : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
+      if (NE % T == 0 || q < NE) s[(q / TW) * P + q % TW] = v[u];
     }
   } else {
-    for (int q = threadIdx.x; q < N * TW; q += T) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
       const int i = q % N, c = q / N;
-      s[i * P + c] = c < tl.ncol ? g[tl.base + c * tl.cs + i]
-                                 : make_float2(0.f, 0.f);
+      v[u] = ((NE % T == 0 || q < NE) && c < tl.ncol)
+                 ? g[tl.base + c * tl.cs + i]
+                 : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
+      if (NE % T == 0 || q < NE) s[(q % N) * P + q / N] = v[u];
     }
   }
 }
@@ -423,15 +472,34 @@ __device__ __forceinline__ void store_tile(float2* __restrict__ g,
                                            const float2* __restrict__ s,
                                            const Tile& tl, float scale) {
   constexpr int P = Pitch<TW>::P;
+  constexpr int NE = N * TW;
+  constexpr int E = (NE + T - 1) / T;
+  float2 v[E];
   if (tl.cs == 1) {
-    for (int q = threadIdx.x; q < N * TW; q += T) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
+      if (NE % T == 0 || q < NE) v[u] = s[(q / TW) * P + q % TW];
+    }
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
       const int i = q / TW, c = q % TW;
-      if (c < tl.ncol) g[tl.base + i * tl.ls + c] = cscale(s[i * P + c], scale);
+      if ((NE % T == 0 || q < NE) && c < tl.ncol)
+        g[tl.base + i * tl.ls + c] = cscale(v[u], scale);
     }
   } else {
-    for (int q = threadIdx.x; q < N * TW; q += T) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
+      if (NE % T == 0 || q < NE) v[u] = s[(q % N) * P + q / N];
+    }
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int q = threadIdx.x + u * T;
       const int i = q % N, c = q / N;
-      if (c < tl.ncol) g[tl.base + c * tl.cs + i] = cscale(s[i * P + c], scale);
+      if ((NE % T == 0 || q < NE) && c < tl.ncol)
+        g[tl.base + c * tl.cs + i] = cscale(v[u], scale);
     }
   }
 }

@@ -69,6 +69,8 @@ __device__ __forceinline__ int imod(int a, int n) {
   return r < 0 ? r + n : r;
 }
 
+constexpr int kPPT = 8;  // particles per thread: bricks hold <= 2048
+
 template <int C>
 __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
   __shared__ float box[kBoxFloats];
@@ -86,19 +88,44 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
   const long long pb = (long long)b * a.d.N;
   const float ps = a.ps;
 
-  int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
-  for (int t = tid; t < nloc; t += 256) {
-    const int lz = t % cz, r = t / cz, ly = r % cy, lx = r / cy;
-    const long long p =
-        pb + ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz + q0z + lz;
-    const float x = a.X[3 * p] * ps, y = a.X[3 * p + 1] * ps,
-                z = a.X[3 * p + 2] * ps;
-    if (!ok_pos(x, y, z)) continue;
-    const int f[3] = {(int)floorf(x), (int)floorf(y), (int)floorf(z)};
+  // one pass over the brick's particles: every load issued up front, the
+  // positions and weights stay in registers for the deposit
+  float px[kPPT], py[kPPT], pz[kPPT], wc[kPPT][C];
+  bool ok[kPPT];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      mn[d] = min(mn[d], f[d]);
-      mx[d] = max(mx[d], f[d]);
+  for (int u = 0; u < kPPT; ++u) {
+    const int t = tid + u * 256;
+    ok[u] = t < nloc;
+    px[u] = py[u] = pz[u] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) wc[u][c] = 0.0f;
+    if (ok[u]) {
+      const int lz = t % cz, r = t / cz, ly = r % cy, lx = r / cy;
+      const long long p =
+          pb + ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz + q0z + lz;
+      px[u] = a.X[3 * p];
+      py[u] = a.X[3 * p + 1];
+      pz[u] = a.X[3 * p + 2];
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        wc[u][c] = a.w ? a.w[p * a.wstride + c] * a.wscale : a.wscale;
+    }
+  }
+  int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+  for (int u = 0; u < kPPT; ++u) {
+    px[u] *= ps;
+    py[u] *= ps;
+    pz[u] *= ps;
+    ok[u] = ok[u] && ok_pos(px[u], py[u], pz[u]);
+    if (ok[u]) {
+      const int f[3] = {(int)floorf(px[u]), (int)floorf(py[u]),
+                        (int)floorf(pz[u])};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        mn[d] = min(mn[d], f[d]);
+        mx[d] = max(mx[d], f[d]);
+      }
     }
   }
 #pragma unroll
@@ -191,21 +218,14 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
   constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};
   constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};
   constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};
-  for (int t = tid; t < nloc; t += 256) {
-    const int lz = t % cz, r = t / cz, ly = r % cy, lx = r / cy;
-    const long long p =
-        pb + ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz + q0z + lz;
-    const float x = a.X[3 * p] * ps, y = a.X[3 * p + 1] * ps,
-                z = a.X[3 * p + 2] * ps;
-    if (!ok_pos(x, y, z)) continue;
+#pragma unroll
+  for (int u = 0; u < kPPT; ++u) {
+    if (!ok[u]) continue;
+    const float x = px[u], y = py[u], z = pz[u];
     const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
     const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
     const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
     const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
-    float wc[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-      wc[c] = a.w ? a.w[p * a.wstride + c] * a.wscale : a.wscale;
     const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
     if (mode == 1) {
       const int bx = ix - lo0, by = iy - lo1, bz = iz - lo2;
@@ -215,7 +235,8 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
                          (OZ[k] ? w1z : w0z);
         const int li = ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
 #pragma unroll
-        for (int c = 0; c < C; ++c) atomicAdd(&box[c * vol + li], wc[c] * kw);
+        for (int c = 0; c < C; ++c)
+          atomicAdd(&box[c * vol + li], wc[u][c] * kw);
       }
     } else {
       const MeshDims& d = a.d;
@@ -227,7 +248,8 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
             (((long long)b * d.nx + imod(ix + OX[k], d.nx)) * d.ny +
              imod(iy + OY[k], d.ny)) * d.nz + imod(iz + OZ[k], d.nz);
 #pragma unroll
-        for (int c = 0; c < C; ++c) atomicAdd(a.mesh[c] + cell, wc[c] * kw);
+        for (int c = 0; c < C; ++c)
+          atomicAdd(a.mesh[c] + cell, wc[u][c] * kw);
       }
     }
   }
@@ -443,7 +465,8 @@ BrickGrid make_brick_grid(int gx, int gy, int gz, int f, int C) {
 }
 
 bool brick_paint_ok(const MeshDims& d, const BrickGrid& g) {
-  return g.gx * g.f == d.nx && g.gy * g.f == d.ny && g.gz * g.f == d.nz &&
+  return g.ex * g.ey * g.ez <= 256 * kPPT && g.gx * g.f == d.nx &&
+         g.gy * g.f == d.ny && g.gz * g.f == d.nz &&
          (long long)g.gx * g.gy * g.gz == d.N && g.nbx <= 1024 &&
          g.nby <= 1024 && g.nbz <= 1024;
 }
