@@ -269,13 +269,14 @@ __global__ __launch_bounds__(T) void k_zpass_r2c(ZJobs jobs, PlanDev p) {
 // Inverse: two half-spectrum rows -> Hermitian-extended Z = X1 + i X2 ->
 // inverse FFT -> row pair (re, im).  The imaginary parts of the kz=0 and
 // kz=N/2 entries are dropped (real part of the inverse, utils.py:270-271).
+// A thread that loads (X1[k], X2[k]) writes both Z[k] = X1 + i X2 and
+// Z[N-k] = conj(X1) + i conj(X2), so the half rows never need their own LDS
+// staging buffer.
 template <int N, int TW, int T>
 __global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
   constexpr int P = Pitch<TW>::P;
   constexpr int H = N / 2;
-  constexpr int HP = H + 1;
   __shared__ float2 s[N * P];
-  __shared__ float2 h[2 * TW * HP];
   __shared__ float2 tw[N];
   const ZJob J = blockIdx.y == 0 ? jobs.j[0]
                  : (blockIdx.y == 1 ? jobs.j[1] : jobs.j[2]);
@@ -286,55 +287,44 @@ __global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
   const float* __restrict__ gz = p.ax[2].g;
   const int ez = J.ez;
   {
-    constexpr int NE = 2 * TW * H;
+    // element q = (line l, k), k = 0..H (k = H is the Nyquist plane)
+    constexpr int NE = TW * (H + 1);
     constexpr int E = (NE + T - 1) / T;
-    float2 v[E];
+    float2 x1[E], x2[E];
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int q = threadIdx.x + u * T;
-      v[u] = ((NE % T == 0 || q < NE) && q / H < nr) ? src[q]
-                                                     : make_float2(0.f, 0.f);
+      const int l = q / (H + 1), k = q % (H + 1);
+      const int ra = 2 * l, rb = 2 * l + 1;
+      const bool in = NE % T == 0 || q < NE;
+      x1[u] = make_float2(0.f, 0.f);
+      x2[u] = make_float2(0.f, 0.f);
+      if (in && ra < nr)
+        x1[u] = k < H ? src[ra * H + k] : J.cin[p.nyq0 + r0 + ra];
+      if (in && rb < nr)
+        x2[u] = k < H ? src[rb * H + k] : J.cin[p.nyq0 + r0 + rb];
     }
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int q = threadIdx.x + u * T;
       if (NE % T == 0 || q < NE) {
-        const int row = q / H, k = q % H;
-        float2 x = v[u];
+        const int l = q / (H + 1), k = q % (H + 1);
+        float2 a = x1[u], b = x2[u];
         if (ez) {
           const float g = gz[k];
-          x = rot_i(cscale(x, ez == 2 ? g * g : g), ez);
+          const float c = ez == 2 ? g * g : g;
+          a = rot_i(cscale(a, c), ez);
+          b = rot_i(cscale(b, c), ez);
         }
-        h[row * HP + k] = x;
+        if (k == 0 || k == H) {
+          a.y = 0.0f;
+          b.y = 0.0f;
+        }
+        s[k * P + l] = make_float2(a.x - b.y, a.y + b.x);
+        if (k > 0 && k < H)  // conj(X1) + i conj(X2)
+          s[(N - k) * P + l] = make_float2(a.x + b.y, -a.y + b.x);
       }
     }
-  }
-  for (int row = threadIdx.x; row < 2 * TW; row += T) {
-    float2 v = row < nr ? J.cin[p.nyq0 + r0 + row] : make_float2(0.f, 0.f);
-    if (ez) {
-      const float g = gz[H];
-      v = rot_i(cscale(v, ez == 2 ? g * g : g), ez);
-    }
-    h[row * HP + H] = v;
-  }
-  __syncthreads();
-  for (int q = threadIdx.x; q < TW * N; q += T) {
-    const int k = q / TW, l = q % TW;
-    float2 x1, x2;
-    if (k <= H) {
-      x1 = h[(2 * l) * HP + k];
-      x2 = h[(2 * l + 1) * HP + k];
-      if (k == 0 || k == H) {
-        x1.y = 0.0f;
-        x2.y = 0.0f;
-      }
-    } else {
-      x1 = h[(2 * l) * HP + (N - k)];
-      x2 = h[(2 * l + 1) * HP + (N - k)];
-      x1.y = -x1.y;
-      x2.y = -x2.y;
-    }
-    s[k * P + l] = make_float2(x1.x - x2.y, x1.y + x2.x);
   }
   __syncthreads();
   fft_tile<N, TW, T, +1>(s, tw);
@@ -506,11 +496,13 @@ __device__ __forceinline__ void store_tile(float2* __restrict__ g,
 
 // Y pass.  Forward: in place (out[0] may equal in).  Inverse: each output o
 // gets IFFT_y((i g_y)^ey[o] * in) * scale[o].
-template <int N, int TW, int T, int D>
+// FAN = false: every job has one output and no multiplier (one LDS tile,
+// twice the workgroups per CU); FAN = true: fan-out with multipliers.
+template <int N, int TW, int T, int D, bool FAN>
 __global__ __launch_bounds__(T) void k_ypass(YJobs jobs, PlanDev p) {
   constexpr int P = Pitch<TW>::P;
   __shared__ float2 a[N * P];
-  __shared__ float2 w[N * P];
+  __shared__ float2 w[FAN ? N * P : 1];
   __shared__ float2 tw[N];
   const YJob J = blockIdx.y == 0 ? jobs.j[0]
                  : (blockIdx.y == 1 ? jobs.j[1] : jobs.j[2]);
@@ -518,7 +510,7 @@ __global__ __launch_bounds__(T) void k_ypass(YJobs jobs, PlanDev p) {
   const Tile tl = make_tile<1, TW>(p, blockIdx.x);
   load_tile<N, TW, T>(a, J.in, tl);
   __syncthreads();
-  if (J.nout == 1 && J.ey[0] == 0) {
+  if (!FAN || (J.nout == 1 && J.ey[0] == 0)) {
     fft_tile<N, TW, T, D>(a, tw);
     store_tile<N, TW, T>(J.out[0], a, tl, J.scale[0]);
     return;
@@ -712,10 +704,21 @@ static int y_nt(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
   const int tiles = p.B * p.nx * nkt + p.B * ((p.nx + TW - 1) / TW);
   dim3 grid(tiles, jobs.n);
   ProfScope prof_(inv ? kProfYinv : kProfYfwd, s);
-  if (inv)
-    hipLaunchKernelGGL((k_ypass<N, TW, T, +1>), grid, dim3(T), 0, s, jobs, p);
+  bool fan = false;
+  for (int j = 0; j < jobs.n; ++j)
+    fan = fan || jobs.j[j].nout != 1 || jobs.j[j].ey[0] != 0;
+  if (inv && fan)
+    hipLaunchKernelGGL((k_ypass<N, TW, T, +1, true>), grid, dim3(T), 0, s,
+                       jobs, p);
+  else if (inv)
+    hipLaunchKernelGGL((k_ypass<N, TW, T, +1, false>), grid, dim3(T), 0, s,
+                       jobs, p);
+  else if (fan)
+    hipLaunchKernelGGL((k_ypass<N, TW, T, -1, true>), grid, dim3(T), 0, s,
+                       jobs, p);
   else
-    hipLaunchKernelGGL((k_ypass<N, TW, T, -1>), grid, dim3(T), 0, s, jobs, p);
+    hipLaunchKernelGGL((k_ypass<N, TW, T, -1, false>), grid, dim3(T), 0, s,
+                       jobs, p);
   PMX_LAUNCH_CHECK();
   return kOk;
 }
