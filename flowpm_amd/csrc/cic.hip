@@ -317,6 +317,66 @@ __global__ __launch_bounds__(256) void k_force_vjp_particles(
   dX[3 * p + 2] += f * sz;
 }
 
+// Last particle pass of one reverse step (force VJP + KDK cotangents):
+//   gX += f [ sum_d (cf gF_d) dR_d + dPaint(rbar) ]     (readout and paint
+//         position gradients: dR_d = sum_cell mesh_d[cell] dK_cell/dx)
+//   then, by mode: 1: gP += d gX ; gF = k1 gP
+//                  2: as 1, plus gF += k2prev gP
+//                  3: gF = 0 (the jump-start force replaced F)
+__global__ __launch_bounds__(256) void k_force_vjp_finish(
+    float* __restrict__ gX, float* __restrict__ gP, float* __restrict__ gF,
+    const float* __restrict__ X, const float* __restrict__ fxm,
+    const float* __restrict__ fym, const float* __restrict__ fzm,
+    const float* __restrict__ rbar, MeshDims d, float f, float cf,
+    AdjUpdate u) {
+  const long long tot = (long long)d.B * d.N;
+  const long long p =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
+  if (p >= tot) return;
+  const int b = (int)(p / d.N);
+  const float x = X[3 * p] * f, y = X[3 * p + 1] * f, z = X[3 * p + 2] * f;
+  const float gfx = gF[3 * p] * cf, gfy = gF[3 * p + 1] * cf,
+              gfz = gF[3 * p + 2] * cf;
+  float sx = 0.f, sy = 0.f, sz = 0.f, rx = 0.f, ry = 0.f, rz = 0.f;
+  if (pos_ok(x, y, z)) {
+    const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+    PMX_CORNERS({
+      const float v = gfx * fxm[cell] + gfy * fym[cell] + gfz * fzm[cell];
+      const float r = rbar[cell];
+      const float kx = (dx * wy) * wz;
+      const float ky = (wx * dy) * wz;
+      const float kz = (wx * wy) * dz;
+      sx += v * kx;
+      sy += v * ky;
+      sz += v * kz;
+      rx += r * kx;
+      ry += r * ky;
+      rz += r * kz;
+    })
+  }
+  float g[3] = {gX[3 * p], gX[3 * p + 1], gX[3 * p + 2]};
+  const float s3[3] = {sx, sy, sz}, r3[3] = {rx, ry, rz};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    g[c] = g[c] + f * s3[c];
+    g[c] = g[c] + f * r3[c];
+    gX[3 * p + c] = g[c];
+  }
+  if (u.mode == 1 || u.mode == 2) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float gp = gP[3 * p + c] + u.d * g[c];
+      gP[3 * p + c] = gp;
+      float gf = u.k1 * gp;
+      if (u.mode == 2) gf = gf + u.k2prev * gp;
+      gF[3 * p + c] = gf;
+    }
+  } else if (u.mode == 3) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gF[3 * p + c] = 0.0f;
+  }
+}
+
 // ------------------------------------------------------------ launchers --
 static inline unsigned nblk(long long n, int t) {
   return (unsigned)((n + t - 1) / t);
@@ -399,6 +459,19 @@ int launch_kick_drift_only(float* X, float* P, const float* F,
   ProfScope prof_(kProfKickDrift, s);
   hipLaunchKernelGGL(k_kick_drift_only, dim3(nblk(n3, 256)), dim3(256), 0, s,
                      X, P, F, n3, c);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
+                            const float* fx, const float* fy, const float* fz,
+                            const float* rbar, const MeshDims& d, float f,
+                            float cf, const AdjUpdate& u, hipStream_t s) {
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  ProfScope prof_(kProfAdjCic, s);
+  hipLaunchKernelGGL(k_force_vjp_finish, dim3(nblk(tot, 256)), dim3(256), 0,
+                     s, gX, gP, gF, X, fx, fy, fz, rbar, d, f, cf, u);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

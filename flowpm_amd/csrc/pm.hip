@@ -572,14 +572,16 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   return kOk;
 }
 
-// X-cotangent contribution of F = force(X) for cotangent gF.  fm: the force
-// meshes at X from the forward pass (recomputed when null).
-static int force_vjp(const Plan& p, const float* X, const float* gF,
-                     float* gX, long long N, float f, float fac,
+// X-cotangent contribution of F = force(X) for cotangent gF, followed by
+// the KDK cotangent update u.  fm: the force meshes at X from the forward
+// pass (recomputed when null).
+static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
+                     float* gP, long long N, float f, float fac,
                      const NbodyWork& nw, float* gm, const float* fm,
-                     hipStream_t s) {
+                     const AdjUpdate& u, hipStream_t s) {
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
+  const long long BN3 = (long long)p.d.B * N * 3;
   MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
   const float* fx = fm ? fm : nw.f[0];
   const float* fy = fm ? fm + M : nw.f[1];
@@ -588,17 +590,27 @@ static int force_vjp(const Plan& p, const float* X, const float* gF,
   float* gms[3] = {gm, gm + M, gm + 2 * M};
   const bool grid = paint_meshes(p, gms, 3, X, gF, 3, fac / f, N, f,
                                  nw.brick, s) == kOk;
-  if (!grid) {
-    for (int c = 0; c < 3; ++c) PMX_TRY(launch_fill(gms[c], M, 0.0f, s));
-  }
-  PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X, fx,
-                                     fy, fz, md, f, fac / f, grid ? 0 : 1,
-                                     s));
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
+  if (grid) {
+    // mesh cotangents by the brick paint; one fused particle pass at the end
+    PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nw.rho, sc,
+                             s));
+    return launch_force_vjp_finish(gX, gP, gF, X, fx, fy, fz, nw.rho, md, f,
+                                   fac / f, u, s);
+  }
+  for (int c = 0; c < 3; ++c) PMX_TRY(launch_fill(gms[c], M, 0.0f, s));
+  PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X, fx,
+                                     fy, fz, md, f, fac / f, 1, s));
   PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nw.rho, sc, s));
   PMX_TRY(launch_paint_adj(gX, nullptr, nw.rho, X, nullptr, md, f, 1.0f, 1,
                            s));
+  if (u.mode == 3) return launch_fill(gF, BN3, 0.0f, s);
+  if (u.mode == 1 || u.mode == 2) {
+    hipLaunchKernelGGL(k_adj_update, dim3(gridfor(BN3)), dim3(256), 0, s, gX,
+                       gP, gF, BN3, u.d, u.k1, u.k2prev, u.mode);
+    PMX_LAUNCH_CHECK();
+  }
   return kOk;
 }
 
@@ -651,20 +663,20 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
                                    hipMemcpyDeviceToDevice, s));
     }
     for (int i = b1 - 1; i >= b0; --i) {
-      PMX_TRY(force_vjp(p, xslots + (long long)(i - b0) * BN3, gF, gX, N, f,
-                        fac, nw, gm, fslots + (long long)(i - b0) * 3 * M,
-                        s));
-      const int mode = i > 0 ? 2 : 1;
-      hipLaunchKernelGGL(k_adj_update, dim3(g), dim3(256), 0, s, gX, gP, gF,
-                         BN3, (float)coef[3 * i + 1], (float)coef[3 * i],
-                         i > 0 ? (float)coef[3 * (i - 1) + 2] : 0.f, mode);
-      PMX_LAUNCH_CHECK();
+      AdjUpdate u;
+      u.d = (float)coef[3 * i + 1];
+      u.k1 = (float)coef[3 * i];
+      u.k2prev = i > 0 ? (float)coef[3 * (i - 1) + 2] : 0.f;
+      u.mode = i > 0 ? 2 : 1;
+      PMX_TRY(force_vjp(p, xslots + (long long)(i - b0) * BN3, gF, gX, gP, N,
+                        f, fac, nw, gm, fslots + (long long)(i - b0) * 3 * M,
+                        u, s));
     }
   }
   // jump-start force at the initial positions; the initial F is discarded
-  PMX_TRY(force_vjp(p, ckpt, gF, gX, N, f, fac, nw, gm,
-                    nsteps > 0 ? fjump : nullptr, s));
-  PMX_TRY(launch_fill(gF, BN3, 0.0f, s));
+  AdjUpdate u0{0.f, 0.f, 0.f, 3};
+  PMX_TRY(force_vjp(p, ckpt, gF, gX, gP, N, f, fac, nw, gm,
+                    nsteps > 0 ? fjump : nullptr, u0, s));
   return kOk;
 }
 

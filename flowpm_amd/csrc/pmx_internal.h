@@ -244,6 +244,16 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
                        hipStream_t s);
 
 // Force VJP pieces.
+// KDK cotangent update applied after a force VJP (see k_force_vjp_finish)
+struct AdjUpdate {
+  float d, k1, k2prev;
+  int mode;  // 0 none, 1 update, 2 update + k2prev, 3 zero gF
+};
+int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
+                            const float* fx, const float* fy, const float* fz,
+                            const float* rbar, const MeshDims& d, float f,
+                            float cf, const AdjUpdate& u, hipStream_t s);
+
 // scatter = 0 skips the mesh-cotangent atomics (done by the brick paint)
 int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz,
                                float* dX, const float* gF, const float* X,
