@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   }
   __syncthreads();
   for (int i = t; i < nf; i += 256) {
-    F[3 * p0 + i] = sf[i];
+    if (!c.skip_f) F[3 * p0 + i] = sf[i];
     if (kick) P[3 * p0 + i] = sp[i];
     if (c.do_k1d) X[3 * p0 + i] = sx[i];
   }

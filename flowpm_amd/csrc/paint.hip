@@ -35,7 +35,8 @@ struct BrickArgs {
   float* arena;          // box arena
   long long arena_cap;   // floats
   int* meta;             // 8 ints per brick: lo[3], ext[3], off lo/hi
-  int* ctr;              // [2] overflow count, [3] box radius
+  int* ctr;              // [2] overflow count, [3] box radius (written by
+                         // k_brick_reduce between the two phases)
   const float* X;        // positions [B][N][3]
   const float* w;        // weights [B][N][wstride] or null
   int wstride;
@@ -167,23 +168,19 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     int mode = 0;  // 0: nothing, 1: LDS box, 2: device atomics
     const long long off =
         ((long long)b * g.nbricks + bid) * (long long)kBoxFloats;
+    int R = 0;
     if (any) {
-      // every brick owns a fixed arena slot: no shared allocator
+      // every brick owns a fixed arena slot: no shared allocator, and no
+      // shared counter either (k_brick_reduce takes the radius maximum)
       mode = (small && vol * C <= kBoxFloats) ? 1 : 2;
-      if (mode == 2) atomicAdd(a.ctr + 2, 1);
       if (mode == 1) {
-        // box radius around the brick's own mesh interval, for the gather;
-        // read first so that only a new maximum costs a device atomic
+        // box radius around the brick's own mesh interval, for the gather
         const int q0[3] = {q0x, q0y, q0z}, cc[3] = {cx, cy, cz};
-        int R = 0;
         for (int d = 0; d < 3; ++d) {
           const int f = g.f;
           R = max(R, q0[d] * f - lo[d]);
           R = max(R, lo[d] + ext[d] - (q0[d] + cc[d]) * f);
         }
-        if (R > __hip_atomic_load(a.ctr + 3, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT))
-          atomicMax(a.ctr + 3, R);
       }
     }
     int* m = a.meta + 8LL * ((long long)b * g.nbricks + bid);
@@ -193,8 +190,8 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     m[3] = mode == 1 ? ext[0] : 0;
     m[4] = mode == 1 ? ext[1] : 0;
     m[5] = mode == 1 ? ext[2] : 0;
-    m[6] = 0;
-    m[7] = 0;
+    m[6] = R;
+    m[7] = mode == 2 ? 1 : 0;
     sh[0] = mode;
     sh[1] = lo[0];
     sh[2] = lo[1];
@@ -258,6 +255,38 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     const long long off = (long long)sh[8] | ((long long)sh[9] << 31);
     float* dst = a.arena + off;
     for (int i = tid; i < vol * C; i += 256) dst[i] = box[i];
+  }
+}
+
+// Radius maximum and overflow count over all bricks (one workgroup), read by
+// the gather: ctr[2] = bricks that went to device atomics, ctr[3] = max R.
+__global__ __launch_bounds__(1024) void k_brick_reduce(const int* __restrict__ meta,
+                                                       long long nb,
+                                                       int* __restrict__ ctr) {
+  __shared__ int smax[16], ssum[16];
+  int r = 0, o = 0;
+  for (long long i = threadIdx.x; i < nb; i += 1024) {
+    r = max(r, meta[8 * i + 6]);
+    o += meta[8 * i + 7];
+  }
+#pragma unroll
+  for (int k = 32; k > 0; k >>= 1) {
+    r = max(r, __shfl_xor(r, k));
+    o += __shfl_xor(o, k);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    smax[threadIdx.x >> 6] = r;
+    ssum[threadIdx.x >> 6] = o;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int rr = 0, oo = 0;
+    for (int w = 0; w < 16; ++w) {
+      rr = max(rr, smax[w]);
+      oo += ssum[w];
+    }
+    ctr[2] = oo;
+    ctr[3] = rr;
   }
 }
 
@@ -496,8 +525,6 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
   a.ps = ps;
   a.d = d;
   a.g = g;
-  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, a.ctr, 8);
-  PMX_LAUNCH_CHECK();
   {
     ProfScope prof_(kProfPaint, s);
     dim3 grid(g.nbricks, d.B);
@@ -507,6 +534,9 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
       hipLaunchKernelGGL(k_brick_deposit<3>, grid, dim3(256), 0, s, a);
     PMX_LAUNCH_CHECK();
   }
+  hipLaunchKernelGGL(k_brick_reduce, dim3(1), dim3(1024), 0, s, a.meta,
+                     (long long)d.B * g.nbricks, a.ctr);
+  PMX_LAUNCH_CHECK();
   {
     ProfScope prof_(kProfPaintGather, s);
     const int nt = ((d.nx + kTile - 1) / kTile) * ((d.ny + kTile - 1) / kTile) *

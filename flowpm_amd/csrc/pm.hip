@@ -566,6 +566,8 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
       kd.k1 = (float)coef[3 * (i + 1)];
       kd.d = (float)coef[3 * (i + 1) + 1];
     }
+    // only the last force of the run has to stay in the state
+    kd.skip_f = kd.do_k1d;
     PMX_TRY(launch_readout_kick_drift(X, P, F, nw.f[0], nw.f[1], nw.f[2], md,
                                       kd, s));
   }

@@ -216,6 +216,7 @@ struct KickDrift {
   float k1;       // opening kick of the next step
   float d;        // drift of the next step
   int do_k2, do_k1d;
+  int skip_f;     // F is consumed in registers by the next kick: not stored
 };
 int launch_readout_kick_drift(float* X, float* P, float* F,
                               const float* fx, const float* fy,
