@@ -38,6 +38,15 @@ __device__ __forceinline__ Cic1 cic1(float x, int n) {
   return c;
 }
 
+template <bool I4>
+__device__ __forceinline__ float3 load3(const Mesh3& m, long long cell) {
+  if (I4) {
+    const float4 v = m.f4[cell];
+    return make_float3(v.x, v.y, v.z);
+  }
+  return make_float3(m.x[cell], m.y[cell], m.z[cell]);
+}
+
 __device__ __forceinline__ bool pos_ok(float x, float y, float z) {
   // beyond 2^30 the int conversion of floor would overflow
   return fabsf(x) < 1.0e9f && fabsf(y) < 1.0e9f && fabsf(z) < 1.0e9f;
@@ -197,10 +206,10 @@ __global__ __launch_bounds__(256) void k_readout_adj(
 // The (B,N,3) state rows are moved through LDS so that every global load
 // and store of X, P and F is a contiguous, coalesced dword stream instead
 // of three 12-byte-strided accesses per particle.
+template <bool I4>
 __global__ __launch_bounds__(256) void k_readout_kick_drift(
     float* __restrict__ X, float* __restrict__ P, float* __restrict__ F,
-    const float* __restrict__ fxm, const float* __restrict__ fym,
-    const float* __restrict__ fzm, MeshDims d, KickDrift c) {
+    Mesh3 m, MeshDims d, KickDrift c) {
   __shared__ float sx[768], sp[768], sf[768];
   const long long tot = (long long)d.B * d.N;
   const long long p0 = xcd_block(blockIdx.x, gridDim.x) * 256;
@@ -223,9 +232,10 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
       const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
       PMX_CORNERS({
         const float k = (wx * wy) * wz;
-        ax += fxm[cell] * k;
-        ay += fym[cell] * k;
-        az += fzm[cell] * k;
+        const float3 v = load3<I4>(m, cell);
+        ax += v.x * k;
+        ay += v.y * k;
+        az += v.z * k;
       })
     }
     const float Fx = (ax * c.fac) / c.f, Fy = (ay * c.fac) / c.f,
@@ -282,12 +292,12 @@ __global__ __launch_bounds__(256) void k_kick_drift_only(
 // Force VJP, particle side.  With F_d = c * readout(mesh_d, f X):
 //   gmesh_d[cell] += c gF_d K_cell           (mesh cotangents, 24 atomics)
 //   dX += c f sum_d gF_d sum_cell mesh_d[cell] dK_cell/dx
+template <bool I4>
 __global__ __launch_bounds__(256) void k_force_vjp_particles(
     float* __restrict__ gmx, float* __restrict__ gmy, float* __restrict__ gmz,
     float* __restrict__ dX, const float* __restrict__ gF,
-    const float* __restrict__ X, const float* __restrict__ fxm,
-    const float* __restrict__ fym, const float* __restrict__ fzm, MeshDims d,
-    float f, float cf, int scatter) {
+    const float* __restrict__ X, Mesh3 m, MeshDims d, float f, float cf,
+    int scatter) {
   const long long tot = (long long)d.B * d.N;
   const long long p =
       xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
@@ -306,7 +316,8 @@ __global__ __launch_bounds__(256) void k_force_vjp_particles(
         atomicAdd(gmy + cell, gfy * k);
         atomicAdd(gmz + cell, gfz * k);
       }
-      const float v = gfx * fxm[cell] + gfy * fym[cell] + gfz * fzm[cell];
+      const float3 fm = load3<I4>(m, cell);
+      const float v = gfx * fm.x + gfy * fm.y + gfz * fm.z;
       sx += v * ((dx * wy) * wz);
       sy += v * ((wx * dy) * wz);
       sz += v * ((wx * wy) * dz);
@@ -323,12 +334,11 @@ __global__ __launch_bounds__(256) void k_force_vjp_particles(
 //   then, by mode: 1: gP += d gX ; gF = k1 gP
 //                  2: as 1, plus gF += k2prev gP
 //                  3: gF = 0 (the jump-start force replaced F)
+template <bool I4>
 __global__ __launch_bounds__(256) void k_force_vjp_finish(
     float* __restrict__ gX, float* __restrict__ gP, float* __restrict__ gF,
-    const float* __restrict__ X, const float* __restrict__ fxm,
-    const float* __restrict__ fym, const float* __restrict__ fzm,
-    const float* __restrict__ rbar, MeshDims d, float f, float cf,
-    AdjUpdate u) {
+    const float* __restrict__ X, Mesh3 m, const float* __restrict__ rbar,
+    MeshDims d, float f, float cf, AdjUpdate u) {
   const long long tot = (long long)d.B * d.N;
   const long long p =
       xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
@@ -341,7 +351,8 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
   if (pos_ok(x, y, z)) {
     const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
     PMX_CORNERS({
-      const float v = gfx * fxm[cell] + gfy * fym[cell] + gfz * fzm[cell];
+      const float3 fm = load3<I4>(m, cell);
+      const float v = gfx * fm.x + gfy * fm.y + gfz * fm.z;
       const float r = rbar[cell];
       const float kx = (dx * wy) * wz;
       const float ky = (wx * dy) * wz;
@@ -438,15 +449,18 @@ int launch_readout_adj(float* gmesh, float* dpos, const float* gout,
   return kOk;
 }
 
-int launch_readout_kick_drift(float* X, float* P, float* F, const float* fx,
-                              const float* fy, const float* fz,
+int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
                               const MeshDims& d, const KickDrift& c,
                               hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   ProfScope prof_(kProfReadoutKD, s);
-  hipLaunchKernelGGL(k_readout_kick_drift, dim3(nblk(tot, 256)), dim3(256), 0,
-                     s, X, P, F, fx, fy, fz, d, c);
+  if (m.f4)
+    hipLaunchKernelGGL(k_readout_kick_drift<true>, dim3(nblk(tot, 256)),
+                       dim3(256), 0, s, X, P, F, m, d, c);
+  else
+    hipLaunchKernelGGL(k_readout_kick_drift<false>, dim3(nblk(tot, 256)),
+                       dim3(256), 0, s, X, P, F, m, d, c);
   PMX_LAUNCH_CHECK();
   return kOk;
 }
@@ -464,29 +478,37 @@ int launch_kick_drift_only(float* X, float* P, const float* F,
 }
 
 int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
-                            const float* fx, const float* fy, const float* fz,
-                            const float* rbar, const MeshDims& d, float f,
-                            float cf, const AdjUpdate& u, hipStream_t s) {
+                            const Mesh3& m, const float* rbar,
+                            const MeshDims& d, float f, float cf,
+                            const AdjUpdate& u, hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   ProfScope prof_(kProfAdjCic, s);
-  hipLaunchKernelGGL(k_force_vjp_finish, dim3(nblk(tot, 256)), dim3(256), 0,
-                     s, gX, gP, gF, X, fx, fy, fz, rbar, d, f, cf, u);
+  if (m.f4)
+    hipLaunchKernelGGL(k_force_vjp_finish<true>, dim3(nblk(tot, 256)),
+                       dim3(256), 0, s, gX, gP, gF, X, m, rbar, d, f, cf, u);
+  else
+    hipLaunchKernelGGL(k_force_vjp_finish<false>, dim3(nblk(tot, 256)),
+                       dim3(256), 0, s, gX, gP, gF, X, m, rbar, d, f, cf, u);
   PMX_LAUNCH_CHECK();
   return kOk;
 }
 
 int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz, float* dX,
                                const float* gF, const float* X,
-                               const float* fx, const float* fy,
-                               const float* fz, const MeshDims& d, float f,
+                               const Mesh3& m, const MeshDims& d, float f,
                                float c, int scatter, hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   ProfScope prof_(kProfAdjCic, s);
-  hipLaunchKernelGGL(k_force_vjp_particles, dim3(nblk(tot, 256)), dim3(256),
-                     0, s, gmx, gmy, gmz, dX, gF, X, fx, fy, fz, d, f, c,
-                     scatter);
+  if (m.f4)
+    hipLaunchKernelGGL(k_force_vjp_particles<true>, dim3(nblk(tot, 256)),
+                       dim3(256), 0, s, gmx, gmy, gmz, dX, gF, X, m, d, f, c,
+                       scatter);
+  else
+    hipLaunchKernelGGL(k_force_vjp_particles<false>, dim3(nblk(tot, 256)),
+                       dim3(256), 0, s, gmx, gmy, gmz, dX, gF, X, m, d, f, c,
+                       scatter);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

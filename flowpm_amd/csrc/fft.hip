@@ -278,10 +278,13 @@ __global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
   constexpr int H = N / 2;
   __shared__ float2 s[N * P];
   __shared__ float2 tw[N];
-  const ZJob J = blockIdx.y == 0 ? jobs.j[0]
-                 : (blockIdx.y == 1 ? jobs.j[1] : jobs.j[2]);
+  // 1-D grid: the jobs of one row chunk are consecutive logical blocks on
+  // one XCD, so interleaved outputs (ostride > 1) merge in that L2
+  const long long lb = xcd_block(blockIdx.x, gridDim.x);
+  const int jb = (int)(lb % jobs.n);
+  const ZJob J = jb == 0 ? jobs.j[0] : (jb == 1 ? jobs.j[1] : jobs.j[2]);
   load_twiddles<N, T>(tw, p.ax[2].tw);
-  const long long r0 = (long long)blockIdx.x * (2 * TW);
+  const long long r0 = (lb / jobs.n) * (2 * TW);
   const int nr = (int)min((long long)(2 * TW), p.rows - r0);
   const float2* __restrict__ src = J.cin + r0 * H;
   const float* __restrict__ gz = p.ax[2].g;
@@ -328,14 +331,17 @@ __global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
   }
   __syncthreads();
   fft_tile<N, TW, T, +1>(s, tw);
-  float* __restrict__ dst = J.rout + r0 * N;
+  // output element (row, z) at rout[((r0 + row) * N + z) * os + oc]
+  const int os = J.ostride > 0 ? J.ostride : 1;
+  float* __restrict__ dst = J.rout + r0 * N * os + J.ocomp;
   const float sc = J.scale;
   for (int q = threadIdx.x; q < 2 * TW * N; q += T) {
     const int row = q / N, z = q % N;
     if (row < nr) {
       const float2 v = s[z * P + (row >> 1)];
       const float r = sc * ((row & 1) ? v.y : v.x);
-      dst[q] = J.accum ? dst[q] + r : r;
+      float* o = dst + (long long)q * os;
+      *o = J.accum ? *o + r : r;
     }
   }
 }
@@ -689,7 +695,7 @@ static int zr2c_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
 template <int N>
 static int zc2r_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   constexpr int TW = Cfg<N>::ZTW, T = Cfg<N>::T;
-  dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)), jobs.n);
+  dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)) * jobs.n);
   ProfScope prof_(kProfZc2r, s);
   hipLaunchKernelGGL((k_zpass_c2r<N, TW, T>), grid, dim3(T), 0, s, jobs,
                      pl.d);

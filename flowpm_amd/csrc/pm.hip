@@ -166,9 +166,12 @@ static int r2c_zy(const Plan& p, const float* const* in, float2* const* spec,
 // From a spectrum whose x pass is pending (fwd = 1) or done (fwd = 0):
 // out_d = IFFT3(spec * lap * (i g_d) * scale), d = x, y, z.
 // w0, w1, w2: spec buffers (w0 may alias spec when fwd = 1).
+// f4 (optional): write the three meshes interleaved as (x, y, z, -) float4
+// instead of to fx / fy / fz.
 static int poisson_grad(const Plan& p, const float2* spec, int fwd,
                         float2* w0, float2* w1, float2* w2, float* fx,
-                        float* fy, float* fz, float scale, hipStream_t s) {
+                        float* fy, float* fz, float scale, hipStream_t s,
+                        float* f4 = nullptr) {
   XJob xj{};
   xj.nin = 1;
   xj.nout = 2;
@@ -198,20 +201,26 @@ static int poisson_grad(const Plan& p, const float2* spec, int fwd,
   PMX_TRY(launch_ypass(p, yj, 1, s));
   ZJobs zj{};
   zj.n = 3;
-  zj.j[0] = zc2r(w1, fx, 1.0f, 0);
-  zj.j[1] = zc2r(w2, fy, 1.0f, 0);
-  zj.j[2] = zc2r(w0, fz, 1.0f, 1);
+  zj.j[0] = zc2r(w1, f4 ? f4 : fx, 1.0f, 0);
+  zj.j[1] = zc2r(w2, f4 ? f4 : fy, 1.0f, 0);
+  zj.j[2] = zc2r(w0, f4 ? f4 : fz, 1.0f, 1);
+  if (f4)
+    for (int d = 0; d < 3; ++d) {
+      zj.j[d].ostride = 4;
+      zj.j[d].ocomp = d;
+    }
   PMX_TRY(launch_zpass_c2r(p, zj, s));
   return kOk;
 }
 
 static int force_meshes(const Plan& p, const float* rho, float2* w, float* fx,
-                        float* fy, float* fz, float rho_scale, hipStream_t s) {
+                        float* fy, float* fz, float rho_scale, hipStream_t s,
+                        float* f4 = nullptr) {
   float2* w0 = w;
   float2* w1 = w + p.d.S;
   float2* w2 = w + 2 * p.d.S;
   PMX_TRY(r2c_zy(p, &rho, &w0, 1, rho_scale, s));
-  return poisson_grad(p, w0, 1, w0, w1, w2, fx, fy, fz, 1.0f, s);
+  return poisson_grad(p, w0, 1, w0, w1, w2, fx, fy, fz, 1.0f, s, f4);
 }
 
 // Adjoint of the map (g_x, g_y, g_z) <- force_meshes(rho):
@@ -437,9 +446,13 @@ static inline unsigned gridfor(long long n) {
 }
 
 // ---------------------------------------------------------------- nbody --
+static inline Mesh3 m3(const float* f4) {
+  return Mesh3{nullptr, nullptr, nullptr, reinterpret_cast<const float4*>(f4)};
+}
+
 struct NbodyWork {
   float* rho;
-  float* f[3];
+  float* f4;  // force meshes, interleaved (x, y, z, -) per cell
   float2* w;
   float* brick;  // brick-paint arena (C = 1 and C = 3 share it)
 };
@@ -466,14 +479,14 @@ static NbodyWork nbody_work(const Plan& p, float* work) {
   NbodyWork nw;
   const long long M = p.d.rows * p.d.nz;
   nw.rho = work;
-  for (int d = 0; d < 3; ++d) nw.f[d] = work + (1 + d) * M;
-  nw.w = reinterpret_cast<float2*>(work + 4 * M);
-  nw.brick = work + 4 * M + 2 * 3 * p.d.S;
+  nw.f4 = work + M;
+  nw.w = reinterpret_cast<float2*>(work + 5 * M);
+  nw.brick = work + 5 * M + 2 * 3 * p.d.S;
   return nw;
 }
 static long long nbody_work_floats(const Plan& p) {
   const long long M = p.d.rows * p.d.nz;
-  return 4 * M + 2 * 3 * p.d.S + brick_floats(p);
+  return 5 * M + 2 * 3 * p.d.S + brick_floats(p);
 }
 
 // Paint C meshes (zeroed here) with the brick paint when the particles form
@@ -511,7 +524,8 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
   // rho * (1/nbar) with nbar = N / Mb (tfpm.py:346-351), then r2c / Mb
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
-  return force_meshes(p, nw.rho, nw.w, nw.f[0], nw.f[1], nw.f[2], sc, s);
+  return force_meshes(p, nw.rho, nw.w, nullptr, nullptr, nullptr, sc, s,
+                      nw.f4);
 }
 
 // fjump / fstep (3 meshes each, may be null): where the force meshes of the
@@ -531,7 +545,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   auto at = [&](float* dst) {
     NbodyWork w = nw0;
     if (dst)
-      for (int d = 0; d < 3; ++d) w.f[d] = dst + d * M;
+      w.f4 = dst;
     return w;
   };
   KickDrift kd{};
@@ -546,7 +560,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
       kd.k1 = (float)coef[3 * b0];
       kd.d = (float)coef[3 * b0 + 1];
     }
-    PMX_TRY(launch_readout_kick_drift(X, P, F, nw.f[0], nw.f[1], nw.f[2], md,
+    PMX_TRY(launch_readout_kick_drift(X, P, F, m3(nw.f4), md,
                                       kd, s));
   } else if (b0 < b1) {
     // opening kick + drift of the first step from the stored force
@@ -568,7 +582,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     }
     // only the last force of the run has to stay in the state
     kd.skip_f = kd.do_k1d;
-    PMX_TRY(launch_readout_kick_drift(X, P, F, nw.f[0], nw.f[1], nw.f[2], md,
+    PMX_TRY(launch_readout_kick_drift(X, P, F, m3(nw.f4), md,
                                       kd, s));
   }
   return kOk;
@@ -585,9 +599,7 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
   MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
-  const float* fx = fm ? fm : nw.f[0];
-  const float* fy = fm ? fm + M : nw.f[1];
-  const float* fz = fm ? fm + 2 * M : nw.f[2];
+  const Mesh3 m = m3(fm ? fm : nw.f4);
   if (!fm) PMX_TRY(force_at(p, X, N, f, nw, s));
   float* gms[3] = {gm, gm + M, gm + 2 * M};
   const bool grid = paint_meshes(p, gms, 3, X, gF, 3, fac / f, N, f,
@@ -598,12 +610,12 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
     // mesh cotangents by the brick paint; one fused particle pass at the end
     PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nw.rho, sc,
                              s));
-    return launch_force_vjp_finish(gX, gP, gF, X, fx, fy, fz, nw.rho, md, f,
-                                   fac / f, u, s);
+    return launch_force_vjp_finish(gX, gP, gF, X, m, nw.rho, md, f, fac / f,
+                                   u, s);
   }
   for (int c = 0; c < 3; ++c) PMX_TRY(launch_fill(gms[c], M, 0.0f, s));
-  PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X, fx,
-                                     fy, fz, md, f, fac / f, 1, s));
+  PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X, m,
+                                     md, f, fac / f, 1, s));
   PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nw.rho, sc, s));
   PMX_TRY(launch_paint_adj(gX, nullptr, nw.rho, X, nullptr, md, f, 1.0f, 1,
                            s));
@@ -623,7 +635,7 @@ static long long nbody_adj_work_floats(const Plan& p, long long N, int every) {
   const long long M = p.d.rows * p.d.nz;
   const long long BN3 = 3LL * p.d.B * N;
   return nbody_work_floats(p) + 3 * M + 3 * BN3 + (long long)every * BN3 +
-         (long long)(every + 1) * 3 * M;
+         (long long)(every + 1) * 4 * M;
 }
 
 static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
@@ -637,8 +649,8 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
   float* gm = work + nbody_work_floats(p);
   float* st = gm + 3 * M;            // scratch state (3, B, N, 3)
   float* xslots = st + 3 * BN3;      // every * BN3
-  float* fslots = xslots + (long long)every * BN3;  // (every + 1) * 3M
-  float* fjump = fslots + (long long)every * 3 * M;
+  float* fslots = xslots + (long long)every * BN3;  // (every + 1) * 4M
+  float* fjump = fslots + (long long)every * 4 * M;
   float* gX = gstate;
   float* gP = gstate + BN3;
   float* gF = gstate + 2 * BN3;
@@ -659,7 +671,7 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
       const bool jump = sg == 0 && i == 0;
       PMX_TRY(nbody_run(p, st, N, f, fac, coef, i, i + 1, jump, work, s,
                         jump ? fjump : nullptr,
-                        fslots + (long long)(i - b0) * 3 * M));
+                        fslots + (long long)(i - b0) * 4 * M));
       PMX_HIP_CHECK(hipMemcpyAsync(xslots + (long long)(i - b0) * BN3, st,
                                    sizeof(float) * BN3,
                                    hipMemcpyDeviceToDevice, s));
@@ -671,7 +683,7 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
       u.k2prev = i > 0 ? (float)coef[3 * (i - 1) + 2] : 0.f;
       u.mode = i > 0 ? 2 : 1;
       PMX_TRY(force_vjp(p, xslots + (long long)(i - b0) * BN3, gF, gX, gP, N,
-                        f, fac, nw, gm, fslots + (long long)(i - b0) * 3 * M,
+                        f, fac, nw, gm, fslots + (long long)(i - b0) * 4 * M,
                         u, s));
     }
   }
@@ -1090,7 +1102,8 @@ int pmx_readout_kick_drift(float* X, float* P, float* F, const float* fx,
   c.d = d;
   c.do_k2 = flags & 1;
   c.do_k1d = (flags >> 1) & 1;
-  return launch_readout_kick_drift(X, P, F, fx, fy, fz, md, c, S(stream));
+  return launch_readout_kick_drift(X, P, F, Mesh3{fx, fy, fz, nullptr}, md, c,
+                                   S(stream));
 }
 
 long long pmx_nbody_work_floats(const pmx_plan* plan) {

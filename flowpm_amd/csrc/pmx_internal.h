@@ -149,6 +149,8 @@ struct ZJob {
   float scale;
   int ez;             // c2r: exponent of (i g_z) applied before the inverse
   int accum;          // c2r: rout += result instead of rout = result
+  int ostride;        // c2r: output element stride (0 or 1: contiguous)
+  int ocomp;          // c2r: offset inside the stride (interleaved output)
 };
 struct ZJobs {
   ZJob j[kMaxJobs];
@@ -218,10 +220,16 @@ struct KickDrift {
   int do_k2, do_k1d;
   int skip_f;     // F is consumed in registers by the next kick: not stored
 };
-int launch_readout_kick_drift(float* X, float* P, float* F,
-                              const float* fx, const float* fy,
-                              const float* fz, const MeshDims& d,
-                              const KickDrift& c, hipStream_t s);
+// The three force meshes: separate (x, y, z) or interleaved float4 (f4).
+struct Mesh3 {
+  const float* x;
+  const float* y;
+  const float* z;
+  const float4* f4;
+};
+int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
+                              const MeshDims& d, const KickDrift& c,
+                              hipStream_t s);
 
 int launch_kick_drift_only(float* X, float* P, const float* F,
                            const MeshDims& d, const KickDrift& c,
@@ -251,15 +259,14 @@ struct AdjUpdate {
   int mode;  // 0 none, 1 update, 2 update + k2prev, 3 zero gF
 };
 int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
-                            const float* fx, const float* fy, const float* fz,
-                            const float* rbar, const MeshDims& d, float f,
-                            float cf, const AdjUpdate& u, hipStream_t s);
+                            const Mesh3& m, const float* rbar,
+                            const MeshDims& d, float f, float cf,
+                            const AdjUpdate& u, hipStream_t s);
 
 // scatter = 0 skips the mesh-cotangent atomics (done by the brick paint)
 int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz,
                                float* dX, const float* gF, const float* X,
-                               const float* fx, const float* fy,
-                               const float* fz, const MeshDims& d, float f,
+                               const Mesh3& m, const MeshDims& d, float f,
                                float c, int scatter, hipStream_t s);
 
 }  // namespace pmx
