@@ -217,9 +217,11 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   const int nf = 3 * np;
   const bool kick = c.do_k2 || c.do_k1d;
   const int t = threadIdx.x;
+  // particle state is streamed with non-temporal loads / stores so it does
+  // not evict the force mesh, which neighbouring blocks re-read, from L2
   for (int i = t; i < nf; i += 256) {
-    sx[i] = X[3 * p0 + i];
-    if (kick) sp[i] = P[3 * p0 + i];
+    sx[i] = __builtin_nontemporal_load(X + 3 * p0 + i);
+    if (kick) sp[i] = __builtin_nontemporal_load(P + 3 * p0 + i);
   }
   __syncthreads();
   if (t < np) {
@@ -268,9 +270,9 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   }
   __syncthreads();
   for (int i = t; i < nf; i += 256) {
-    if (!c.skip_f) F[3 * p0 + i] = sf[i];
-    if (kick) P[3 * p0 + i] = sp[i];
-    if (c.do_k1d) X[3 * p0 + i] = sx[i];
+    if (!c.skip_f) __builtin_nontemporal_store(sf[i], F + 3 * p0 + i);
+    if (kick) __builtin_nontemporal_store(sp[i], P + 3 * p0 + i);
+    if (c.do_k1d) __builtin_nontemporal_store(sx[i], X + 3 * p0 + i);
   }
 }
 
