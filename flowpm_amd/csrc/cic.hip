@@ -219,9 +219,33 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   const int t = threadIdx.x;
   // particle state is streamed with non-temporal loads / stores so it does
   // not evict the force mesh, which neighbouring blocks re-read, from L2
-  for (int i = t; i < nf; i += 256) {
-    sx[i] = __builtin_nontemporal_load(X + 3 * p0 + i);
-    if (kick) sp[i] = __builtin_nontemporal_load(P + 3 * p0 + i);
+  if (np == 256) {
+    // full block: all loads in flight before the first LDS write
+    float vx[3], vp[3];
+    const float* xs = X + 3 * p0 + t;
+    const float* ps = P + 3 * p0 + t;
+    if (kick) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        vx[k] = __builtin_nontemporal_load(xs + 256 * k);
+        vp[k] = __builtin_nontemporal_load(ps + 256 * k);
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        sx[t + 256 * k] = vx[k];
+        sp[t + 256 * k] = vp[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) vx[k] = __builtin_nontemporal_load(xs + 256 * k);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) sx[t + 256 * k] = vx[k];
+    }
+  } else {
+    for (int i = t; i < nf; i += 256) {
+      sx[i] = __builtin_nontemporal_load(X + 3 * p0 + i);
+      if (kick) sp[i] = __builtin_nontemporal_load(P + 3 * p0 + i);
+    }
   }
   __syncthreads();
   if (t < np) {

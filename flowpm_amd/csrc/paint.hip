@@ -476,16 +476,24 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
             a.arena + ((long long)b * g.nbricks + lst[j][0]) *
                           (long long)kBoxFloats;
         const int a0 = lst[j][1];
+        // branch-free: every load is issued (out-of-box ones read box
+        // element 0) before the first is consumed, so the 16 x C loads of
+        // one brick are in flight together instead of one round trip each
+        float v[C][PER];
+        bool in[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
           int ox = t0x + k - a0;
           if (ox < 0) ox += d.nx;
-          if (k < Tx && ox < e0) {
-            const int bi = (ox * e1 + oy) * e2 + oz;
+          in[k] = k < Tx && ox < e0;
+          const int bi = in[k] ? (ox * e1 + oy) * e2 + oz : 0;
 #pragma unroll
-            for (int c = 0; c < C; ++c) acc[c][k] += src[c * vol + bi];
-          }
+          for (int c = 0; c < C; ++c) v[c][k] = src[c * vol + bi];
         }
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+#pragma unroll
+          for (int c = 0; c < C; ++c) acc[c][k] += in[k] ? v[c][k] : 0.0f;
       }
     }
     __syncthreads();  // lst is rewritten by the next chunk
