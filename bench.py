@@ -50,7 +50,8 @@ def algorithmic_bytes(kind, n, N):
       "fft_x_fused": S8 + 2 * S8,
       "fft_y_inv": 2 * S8 + 3 * S8,
       "fft_z_c2r": 3 * S8 + 3 * 4 * M,
-      "readout_kick_drift": 3 * 4 * M + 24 * N + 36 * N,
+      # X, P read and written; F stays in registers inside fused steps
+      "readout_kick_drift": 3 * 4 * M + 24 * N + 24 * N,
   }
   return table.get(kind)
 
@@ -135,19 +136,7 @@ def main():
   # live per-kernel timing on the launch stream (separate forward run)
   st = state0.clone()
   torch.cuda.synchronize()
-  _lib.call("pmx_profile_start")
-  run(st, 0, K, 1)
-  nk = _lib.lib().pmx_profile_kinds()
-  ms = (_lib._c.c_double * nk)()
-  cnt = (_lib._ll * nk)()
-  _lib.call("pmx_profile_stop", ms, cnt, nk)
-  kernels = {}
-  for k in range(nk):
-    if cnt[k]:
-      name = _lib.lib().pmx_profile_kind_name(k).decode()
-      kernels[name] = {"launches": int(cnt[k]),
-                       "avg_us": 1e3 * ms[k] / cnt[k],
-                       "total_ms": ms[k]}
+  kernels = live_profile(_lib, lambda: run(st, 0, K, 1))
   dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
   alg = algorithmic_bytes(dom, n, N)
   avg_s = kernels[dom]["avg_us"] * 1e-6
@@ -160,6 +149,7 @@ def main():
 
   # forward + adjoint (checkpoint every k, recompute in backward)
   fa_ms = None
+  adj_kernels = None
   if not args.no_adjoint:
     x0 = state0.clone().requires_grad_(True)
     gseed = torch.randn(state0.shape, generator=torch.Generator().manual_seed(
@@ -175,6 +165,11 @@ def main():
     torch.cuda.synchronize()
     fa_ms = e0.elapsed_time(e1) / K
     assert torch.isfinite(gx).all()
+
+    def fwd_adj():
+      fin = fa.nbody(cosmo, x0, stages, n, checkpoint_every=args.every)
+      torch.autograd.grad(fin, x0, grad_outputs=gseed)
+    adj_kernels = live_profile(_lib, fwd_adj)
 
   batched = None
   if not args.no_batched:
@@ -221,10 +216,30 @@ def main():
           (algorithmic_bytes(k, n, N) or 0) * v["launches"] / (K + 1)
           for k, v in kernels.items()),
       "kernels": kernels,
+      "fwd_adjoint_kernels": adj_kernels,
       "batched": batched,
       "cpu_baseline": cpu,
   }
   print(json.dumps(out))
+
+
+def live_profile(_lib, fn):
+  """Per-kind launch counts and average durations (HIP events recorded by
+  the library on its launch stream) over one call of fn."""
+  _lib.call("pmx_profile_start")
+  fn()
+  nk = _lib.lib().pmx_profile_kinds()
+  ms = (_lib._c.c_double * nk)()
+  cnt = (_lib._ll * nk)()
+  _lib.call("pmx_profile_stop", ms, cnt, nk)
+  out = {}
+  for k in range(nk):
+    if cnt[k]:
+      name = _lib.lib().pmx_profile_kind_name(k).decode()
+      out[name] = {"launches": int(cnt[k]),
+                   "avg_us": 1e3 * ms[k] / cnt[k],
+                   "total_ms": ms[k]}
+  return out
 
 
 def _ic_states(fa, cosmo, n, L, B, seed, dev):

@@ -563,7 +563,7 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
   a.d = d;
   a.g = g;
   {
-    ProfScope prof_(kProfPaint, s);
+    ProfScope prof_(C == 1 ? kProfPaint : kProfPaintW, s);
     dim3 grid(g.nbricks, d.B);
     if (C == 1)
       hipLaunchKernelGGL(k_brick_deposit<1>, grid, dim3(256), 0, s, a);
@@ -575,7 +575,7 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
                      (long long)d.B * g.nbricks, a.ctr);
   PMX_LAUNCH_CHECK();
   {
-    ProfScope prof_(kProfPaintGather, s);
+    ProfScope prof_(C == 1 ? kProfPaintGather : kProfPaintWGather, s);
     const int nt = ((d.nx + kTile - 1) / kTile) * ((d.ny + kTile - 1) / kTile) *
                    ((d.nz + kTile - 1) / kTile);
     dim3 grid(nt, d.B);

@@ -42,7 +42,7 @@ void prof_record(int kind, hipStream_t s, hipEvent_t* ev, bool end) {
 static const char* kProfNames[kProfKinds] = {
     "fill", "paint", "paint_gather", "fft_z_r2c", "fft_y_fwd", "fft_x_fused", "fft_y_inv",
     "fft_z_c2r", "readout_kick_drift", "kick_drift", "readout", "cic_adjoint",
-    "other"};
+    "paint_w3", "paint_w3_gather", "adjoint_update", "other"};
 
 // ------------------------------------------------------------------ plan --
 static int plan_create(Plan** out, int B, int nx, int ny, int nz) {
@@ -621,6 +621,7 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
                            s));
   if (u.mode == 3) return launch_fill(gF, BN3, 0.0f, s);
   if (u.mode == 1 || u.mode == 2) {
+    ProfScope prof_(kProfAdjUpdate, s);
     hipLaunchKernelGGL(k_adj_update, dim3(gridfor(BN3)), dim3(256), 0, s, gX,
                        gP, gF, BN3, u.d, u.k1, u.k2prev, u.mode);
     PMX_LAUNCH_CHECK();
@@ -663,15 +664,19 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
   }
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int b0 = sg * every, b1 = std::min(nsteps, b0 + every);
-    PMX_HIP_CHECK(hipMemcpyAsync(st, ckpt + (long long)sg * 3 * BN3,
-                                 sizeof(float) * 3 * BN3,
-                                 hipMemcpyDeviceToDevice, s));
+    {
+      ProfScope prof_(kProfAdjUpdate, s);
+      PMX_HIP_CHECK(hipMemcpyAsync(st, ckpt + (long long)sg * 3 * BN3,
+                                   sizeof(float) * 3 * BN3,
+                                   hipMemcpyDeviceToDevice, s));
+    }
     // recompute the segment, keeping X and the force meshes of every step
     for (int i = b0; i < b1; ++i) {
       const bool jump = sg == 0 && i == 0;
       PMX_TRY(nbody_run(p, st, N, f, fac, coef, i, i + 1, jump, work, s,
                         jump ? fjump : nullptr,
                         fslots + (long long)(i - b0) * 4 * M));
+      ProfScope prof_(kProfAdjUpdate, s);
       PMX_HIP_CHECK(hipMemcpyAsync(xslots + (long long)(i - b0) * BN3, st,
                                    sizeof(float) * BN3,
                                    hipMemcpyDeviceToDevice, s));

@@ -79,6 +79,9 @@ enum ProfKind : int {
   kProfKickDrift,
   kProfReadout,
   kProfAdjCic,
+  kProfPaintW,        // weighted three-channel brick deposit (force VJP)
+  kProfPaintWGather,  // its gather
+  kProfAdjUpdate,     // KDK cotangent updates and checkpoint copies
   kProfOther,
   kProfKinds
 };
