@@ -10,8 +10,11 @@ inverse FFTs, fused readout + kick + drift).
 Reports (one JSON line):
   value / ms_per_step      forward ms per PM step (K timed steps, including
                            the jump-start force of nbody)
-  fwd_adjoint_ms_per_step  forward with checkpoints every 8 steps + the
-                           hand-written adjoint with recomputation, per step
+  fwd_adjoint_ms_per_step  forward + hand-written adjoint per step, with
+                           the automatic trajectory policy (at 256^3 the
+                           whole trajectory is taped: no recomputation);
+                           fwd_adjoint_recompute8_ms_per_step: checkpoints
+                           every 8 steps, segments recomputed
   roofline                 dominant kernel: algorithmic bytes per launch /
                            average launch time from HIP events recorded on
                            the launch stream during a forward run
@@ -63,7 +66,9 @@ def main():
   ap.add_argument("--warmup", type=int, default=2)
   ap.add_argument("--n", type=int, default=256)
   ap.add_argument("--box", type=float, default=512.0)
-  ap.add_argument("--every", type=int, default=8)
+  ap.add_argument("--every", type=int, default=None,
+                  help="adjoint checkpoint interval (0: taped, no "
+                  "recomputation; default: automatic)")
   ap.add_argument("--no-adjoint", action="store_true")
   ap.add_argument("--no-cpu-baseline", action="store_true")
   ap.add_argument("--no-batched", action="store_true")
@@ -148,7 +153,7 @@ def main():
       v["GBps"] = b / (v["avg_us"] * 1e-6) / 1e9
 
   # forward + adjoint (checkpoint every k, recompute in backward)
-  fa_ms = None
+  fa_ms = fa8_ms = every_used = None
   adj_kernels = None
   if not args.no_adjoint:
     x0 = state0.clone().requires_grad_(True)
@@ -165,6 +170,16 @@ def main():
     torch.cuda.synchronize()
     fa_ms = e0.elapsed_time(e1) / K
     assert torch.isfinite(gx).all()
+    every_used = fa.tfpm._resolve_every(plan, N, K, args.every, dev)
+
+    # the recomputing adjoint (segments of 8 steps) for comparison
+    e0.record()
+    fin = fa.nbody(cosmo, x0, stages, n, checkpoint_every=8)
+    (gx8,) = torch.autograd.grad(fin, x0, grad_outputs=gseed)
+    e1.record()
+    torch.cuda.synchronize()
+    fa8_ms = e0.elapsed_time(e1) / K
+    del gx8, fin
 
     def fwd_adj():
       fin = fa.nbody(cosmo, x0, stages, n, checkpoint_every=args.every)
@@ -197,12 +212,13 @@ def main():
                       "Mpc/h, Planck15 EH P(k), 2LPT a0=0.1, %d KDK steps "
                       "to a=1" % (n, n, L, K),
           "mesh": n, "particles": N, "box_mpc_h": L, "lpt_order": 2,
-          "a0": 0.1, "checkpoint_every": args.every,
+          "a0": 0.1, "checkpoint_every": every_used,
           "fft_tiles": {"y": tiles[0], "x": tiles[1]},
       },
       "autotune_force_mesh_ms": {"%dx%d" % k: v for k, v in tile_ms.items()},
       "fwd_adjoint_ms_per_step": fa_ms,
       "adjoint_over_forward": (fa_ms / ms_step) if fa_ms else None,
+      "fwd_adjoint_recompute8_ms_per_step": fa8_ms,
       "roofline": {
           "bound": "hbm",
           "kernel": dom,

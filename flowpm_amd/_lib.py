@@ -59,6 +59,11 @@ _SIGS = {
     "pmx_nbody_adj_work_floats": (_ll, [_vp, _ll, _i]),
     "pmx_nbody_adjoint": (_i, [_vp, _vp, _i, _i, _ll, _f, _f, _dp, _vp, _vp,
                                _vp]),
+    "pmx_nbody_tape_floats": (_ll, [_vp, _ll, _i]),
+    "pmx_nbody_run_taped": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _vp, _vp,
+                                 _vp]),
+    "pmx_nbody_adjoint_taped": (_i, [_vp, _vp, _i, _ll, _f, _f, _dp, _vp,
+                                     _vp, _vp]),
 }
 
 _LIB = None

@@ -528,15 +528,32 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
                       nw.f4);
 }
 
-// fjump / fstep (3 meshes each, may be null): where the force meshes of the
-// jump-start force and of the first step's force are left (the adjoint keeps
-// them instead of recomputing them).
+// Where a run leaves what the reverse sweep needs of each force evaluation
+// (any pointer may be null): the positions (B*N*3 floats) and the
+// interleaved force meshes (4M floats) of the jump-start force, and of the
+// force of step i at xs / fs + (i - first) slots.
+struct Tape {
+  float* xj = nullptr;
+  float* fj = nullptr;
+  float* xs = nullptr;
+  float* fs = nullptr;
+  int first = 0;
+};
+
 static int nbody_run(const Plan& p, float* state, long long N, float f,
                      float fac, const double* coef, int b0, int b1,
                      int jump, float* work, hipStream_t s,
-                     float* fjump = nullptr, float* fstep = nullptr) {
+                     const Tape* tp = nullptr) {
   const long long BN3 = (long long)p.d.B * N * 3;
   const long long M = p.d.rows * p.d.nz;
+  const Tape tape = tp ? *tp : Tape{};
+  auto keep_x = [&](float* dst, const float* X) -> int {
+    if (!dst) return kOk;
+    ProfScope prof_(kProfAdjUpdate, s);
+    PMX_HIP_CHECK(hipMemcpyAsync(dst, X, sizeof(float) * BN3,
+                                 hipMemcpyDeviceToDevice, s));
+    return kOk;
+  };
   float* X = state;
   float* P = state + BN3;
   float* F = state + 2 * BN3;
@@ -552,7 +569,8 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   kd.fac = fac;
   kd.f = f;
   if (jump) {
-    const NbodyWork nw = at(fjump);
+    PMX_TRY(keep_x(tape.xj, X));
+    const NbodyWork nw = at(tape.fj);
     PMX_TRY(force_at(p, X, N, f, nw, s));
     kd.do_k2 = 0;
     kd.do_k1d = (b0 < b1);
@@ -571,7 +589,9 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     PMX_TRY(launch_kick_drift_only(X, P, F, md, c, s));
   }
   for (int i = b0; i < b1; ++i) {
-    const NbodyWork nw = at(i == b0 ? fstep : nullptr);
+    const long long slot = i - tape.first;
+    PMX_TRY(keep_x(tape.xs ? tape.xs + slot * BN3 : nullptr, X));
+    const NbodyWork nw = at(tape.fs ? tape.fs + slot * 4 * M : nullptr);
     PMX_TRY(force_at(p, X, N, f, nw, s));
     kd.do_k2 = 1;
     kd.k2 = (float)coef[3 * i + 2];
@@ -629,14 +649,72 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
   return kOk;
 }
 
-// Adjoint workspace: nbody work, 3 cotangent meshes, a scratch state, X of
-// every step of a segment and the force meshes of every step of a segment
-// (plus the jump-start force).
+// Adjoint workspace: nbody work and 3 cotangent meshes; with recomputation
+// (every >= 1) also a scratch state, X of every step of a segment and the
+// force meshes of every step of a segment plus the jump-start force.
+// every = 0: the taped reverse sweep (no recomputation).
 static long long nbody_adj_work_floats(const Plan& p, long long N, int every) {
   const long long M = p.d.rows * p.d.nz;
   const long long BN3 = 3LL * p.d.B * N;
-  return nbody_work_floats(p) + 3 * M + 3 * BN3 + (long long)every * BN3 +
+  const long long base = nbody_work_floats(p) + 3 * M;
+  if (every == 0) return base;
+  return base + 3 * BN3 + (long long)every * BN3 +
          (long long)(every + 1) * 4 * M;
+}
+
+// Tape of a whole run: X and force meshes of the jump-start force and of
+// every step, laid out X slots (nsteps + 1) then force slots (nsteps + 1).
+static long long tape_floats(const Plan& p, long long N, int nsteps) {
+  const long long M = p.d.rows * p.d.nz;
+  const long long BN3 = 3LL * p.d.B * N;
+  return (long long)(nsteps + 1) * (BN3 + 4 * M);
+}
+
+static Tape tape_of(const Plan& p, float* tape, long long N, int nsteps) {
+  const long long BN3 = 3LL * p.d.B * N;
+  Tape t;
+  t.xj = tape;
+  t.xs = tape + BN3;
+  t.fj = tape + (long long)(nsteps + 1) * BN3;
+  t.fs = t.fj + 4 * p.d.rows * p.d.nz;
+  t.first = 0;
+  return t;
+}
+
+// Reverse sweep over steps [b0, b1) whose X and force meshes are in tp.
+static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
+                         long long N, float f, float fac, const double* coef,
+                         float* gstate, const NbodyWork& nw, float* gm,
+                         hipStream_t s) {
+  const long long M = p.d.rows * p.d.nz;
+  const long long BN3 = (long long)p.d.B * N * 3;
+  float* gX = gstate;
+  float* gP = gstate + BN3;
+  float* gF = gstate + 2 * BN3;
+  for (int i = b1 - 1; i >= b0; --i) {
+    AdjUpdate u;
+    u.d = (float)coef[3 * i + 1];
+    u.k1 = (float)coef[3 * i];
+    u.k2prev = i > 0 ? (float)coef[3 * (i - 1) + 2] : 0.f;
+    u.mode = i > 0 ? 2 : 1;
+    const long long slot = i - tp.first;
+    PMX_TRY(force_vjp(p, tp.xs + slot * BN3, gF, gX, gP, N, f, fac, nw, gm,
+                      tp.fs + slot * 4 * M, u, s));
+  }
+  return kOk;
+}
+
+// First update of the reverse sweep: the closing kick of the last step.
+static int adjoint_open(const Plan& p, long long N, int nsteps,
+                        const double* coef, float* gstate, hipStream_t s) {
+  if (nsteps <= 0) return kOk;
+  const long long BN3 = (long long)p.d.B * N * 3;
+  ProfScope prof_(kProfAdjUpdate, s);
+  hipLaunchKernelGGL(k_adj_update, dim3(gridfor(BN3)), dim3(256), 0, s,
+                     gstate, gstate + BN3, gstate + 2 * BN3, BN3, 0.f, 0.f,
+                     (float)coef[3 * (nsteps - 1) + 2], 0);
+  PMX_LAUNCH_CHECK();
+  return kOk;
 }
 
 static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
@@ -652,16 +730,8 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
   float* xslots = st + 3 * BN3;      // every * BN3
   float* fslots = xslots + (long long)every * BN3;  // (every + 1) * 4M
   float* fjump = fslots + (long long)every * 4 * M;
-  float* gX = gstate;
-  float* gP = gstate + BN3;
-  float* gF = gstate + 2 * BN3;
-  const unsigned g = gridfor(BN3);
   const int nseg = (nsteps + every - 1) / every;
-  if (nsteps > 0) {
-    hipLaunchKernelGGL(k_adj_update, dim3(g), dim3(256), 0, s, gX, gP, gF,
-                       BN3, 0.f, 0.f, (float)coef[3 * (nsteps - 1) + 2], 0);
-    PMX_LAUNCH_CHECK();
-  }
+  PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int b0 = sg * every, b1 = std::min(nsteps, b0 + every);
     {
@@ -671,32 +741,35 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
                                    hipMemcpyDeviceToDevice, s));
     }
     // recompute the segment, keeping X and the force meshes of every step
-    for (int i = b0; i < b1; ++i) {
-      const bool jump = sg == 0 && i == 0;
-      PMX_TRY(nbody_run(p, st, N, f, fac, coef, i, i + 1, jump, work, s,
-                        jump ? fjump : nullptr,
-                        fslots + (long long)(i - b0) * 4 * M));
-      ProfScope prof_(kProfAdjUpdate, s);
-      PMX_HIP_CHECK(hipMemcpyAsync(xslots + (long long)(i - b0) * BN3, st,
-                                   sizeof(float) * BN3,
-                                   hipMemcpyDeviceToDevice, s));
-    }
-    for (int i = b1 - 1; i >= b0; --i) {
-      AdjUpdate u;
-      u.d = (float)coef[3 * i + 1];
-      u.k1 = (float)coef[3 * i];
-      u.k2prev = i > 0 ? (float)coef[3 * (i - 1) + 2] : 0.f;
-      u.mode = i > 0 ? 2 : 1;
-      PMX_TRY(force_vjp(p, xslots + (long long)(i - b0) * BN3, gF, gX, gP, N,
-                        f, fac, nw, gm, fslots + (long long)(i - b0) * 4 * M,
-                        u, s));
-    }
+    Tape tp;
+    tp.fj = fjump;
+    tp.xs = xslots;
+    tp.fs = fslots;
+    tp.first = b0;
+    PMX_TRY(nbody_run(p, st, N, f, fac, coef, b0, b1, sg == 0, work, s, &tp));
+    PMX_TRY(reverse_steps(p, tp, b0, b1, N, f, fac, coef, gstate, nw, gm, s));
   }
   // jump-start force at the initial positions; the initial F is discarded
   AdjUpdate u0{0.f, 0.f, 0.f, 3};
-  PMX_TRY(force_vjp(p, ckpt, gF, gX, gP, N, f, fac, nw, gm,
-                    nsteps > 0 ? fjump : nullptr, u0, s));
-  return kOk;
+  return force_vjp(p, ckpt, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
+                   fac, nw, gm, nsteps > 0 ? fjump : nullptr, u0, s);
+}
+
+// Reverse sweep over a tape recorded by the forward run (no recomputation).
+static int nbody_adjoint_taped(const Plan& p, float* tape, int nsteps,
+                               long long N, float f, float fac,
+                               const double* coef, float* gstate,
+                               float* work, hipStream_t s) {
+  const long long BN3 = (long long)p.d.B * N * 3;
+  NbodyWork nw = nbody_work(p, work);
+  float* gm = work + nbody_work_floats(p);
+  const Tape tp = tape_of(p, tape, N, nsteps);
+  PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
+  PMX_TRY(reverse_steps(p, tp, 0, nsteps, N, f, fac, coef, gstate, nw, gm,
+                        s));
+  AdjUpdate u0{0.f, 0.f, 0.f, 3};
+  return force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
+                   fac, nw, gm, tp.fj, u0, s);
 }
 
 }  // namespace pmx
@@ -1128,7 +1201,38 @@ int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
 
 long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N,
                                     int every) {
-  return (plan && every >= 1) ? nbody_adj_work_floats(P(plan), N, every) : -1;
+  return (plan && every >= 0) ? nbody_adj_work_floats(P(plan), N, every) : -1;
+}
+
+long long pmx_nbody_tape_floats(const pmx_plan* plan, long long N,
+                                int nsteps) {
+  return (plan && N >= 1 && nsteps >= 0) ? tape_floats(P(plan), N, nsteps)
+                                         : -1;
+}
+
+int pmx_nbody_run_taped(const pmx_plan* plan, float* state, long long N,
+                        float f, float fac, const double* coef, int nsteps,
+                        float* tape, float* work, void* stream) {
+  PMX_REQUIRE(plan && state && tape && work,
+              "pmx_nbody_run_taped: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_nbody_run_taped: need at least one particle");
+  PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
+              "pmx_nbody_run_taped: bad steps");
+  const Tape tp = tape_of(P(plan), tape, N, nsteps);
+  return nbody_run(P(plan), state, N, f, fac, coef, 0, nsteps, 1, work,
+                   S(stream), &tp);
+}
+
+int pmx_nbody_adjoint_taped(const pmx_plan* plan, float* tape, int nsteps,
+                            long long N, float f, float fac,
+                            const double* coef, float* gstate, float* work,
+                            void* stream) {
+  PMX_REQUIRE(plan && tape && gstate && work,
+              "pmx_nbody_adjoint_taped: null pointer");
+  PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
+              "pmx_nbody_adjoint_taped: bad steps");
+  return nbody_adjoint_taped(P(plan), tape, nsteps, N, f, fac, coef, gstate,
+                             work, S(stream));
 }
 
 int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,

@@ -33,14 +33,31 @@ __all__ = ["linear_field", "whitenoise_to_linear", "lpt1", "lpt2_source",
            "lpt_init", "apply_longrange", "kick", "drift", "force", "nbody",
            "step_coefficients", "set_checkpoint_every"]
 
-_CKPT_EVERY = [8]
+_CKPT_EVERY = [None]
+# automatic policy: record the whole trajectory when the tape fits in this
+# fraction of the free device memory, else recompute segments of this size
+_TAPE_FRACTION = 0.5
+_AUTO_SEGMENT = 8
 
 
 def set_checkpoint_every(k):
-  """Trajectory checkpoint interval used by differentiable nbody calls."""
-  if int(k) < 1:
-    raise ValueError("checkpoint interval must be >= 1")
-  _CKPT_EVERY[0] = int(k)
+  """How differentiable nbody calls keep the trajectory for the adjoint.
+
+  k >= 1: keep the state every k steps and recompute each segment in the
+  backward pass; k = 0: record the positions and force meshes of every step
+  in the forward pass (no recomputation); None: automatic (k = 0 when that
+  tape fits in half of the free device memory, else k = 8)."""
+  if k is not None and int(k) < 0:
+    raise ValueError("checkpoint interval must be >= 0 (or None)")
+  _CKPT_EVERY[0] = None if k is None else int(k)
+
+
+def _resolve_every(plan, N, nsteps, every, device):
+  if every is not None:
+    return int(every)
+  tape = _lib.lib().pmx_nbody_tape_floats(plan.handle, N, nsteps) * 4
+  free = torch.cuda.mem_get_info(device)[0]
+  return 0 if tape <= _TAPE_FRACTION * free else _AUTO_SEGMENT
 
 
 def _nc3(nc):
@@ -341,6 +358,15 @@ class _Nbody(torch.autograd.Function):
       _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
                 0, nsteps, 1, _lib.ptr(work), stream)
       return st
+    if every == 0:
+      tape = torch.empty(_lib.lib().pmx_nbody_tape_floats(plan.handle, N,
+                                                          nsteps),
+                         dtype=torch.float32, device=st.device)
+      _lib.call("pmx_nbody_run_taped", plan.handle, _lib.ptr(st), N, f, fac,
+                cptr, nsteps, _lib.ptr(tape), _lib.ptr(work), stream)
+      ctx.tape = tape
+      ctx.args = (plan, N, f, fac, coef, every)
+      return st
     nseg = max(1, (nsteps + every - 1) // every)
     ckpt = torch.empty((nseg, 3, B, N, 3), dtype=torch.float32,
                        device=st.device)
@@ -355,10 +381,20 @@ class _Nbody(torch.autograd.Function):
 
   @staticmethod
   def backward(ctx, g):
-    (ckpt,) = ctx.saved_tensors
     plan, N, f, fac, coef, every = ctx.args
     nsteps = len(coef) // 3
     gs = _f32(g).clone()
+    if every == 0:
+      tape = ctx.tape
+      work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
+                                                              0),
+                         dtype=torch.float32, device=gs.device)
+      _lib.call("pmx_nbody_adjoint_taped", plan.handle, _lib.ptr(tape),
+                nsteps, N, f, fac, _lib.dbl(coef), _lib.ptr(gs),
+                _lib.ptr(work), _lib.stream_of(gs))
+      del ctx.tape
+      return gs, None, None, None, None, None, None, None
+    (ckpt,) = ctx.saved_tensors
     k = min(every, max(nsteps, 1))
     work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
                                                             k),
@@ -373,9 +409,12 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
           return_intermediate_states=False, pgdparams=None, name="NBody",
           checkpoint_every=None):
   """Integrates the state across ``stages`` with the FastPM KDK scheme
-  (tfpm.py:425-503).  Differentiable w.r.t. ``state``: the backward pass
-  recomputes each segment of ``checkpoint_every`` steps from a stored
-  checkpoint and runs the hand-written adjoint kernels in reverse."""
+  (tfpm.py:425-503).  Differentiable w.r.t. ``state`` through hand-written
+  adjoint kernels run in reverse: with ``checkpoint_every`` = k >= 1 the
+  backward pass recomputes each segment of k steps from a stored
+  checkpoint; with 0 the forward pass records the positions and force
+  meshes of every step and nothing is recomputed; None uses the setting of
+  ``set_checkpoint_every`` (automatic by default)."""
   if pgdparams is not None:
     raise NotImplementedError("PGD correction (tfpm.py:369-422) is not built")
   _lib.require_cuda(state)
@@ -394,6 +433,8 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
                 _lib.stream_of(st))
       out.append((stages[i + 1], st.clone()))
     return out
-  every = int(checkpoint_every or _CKPT_EVERY[0])
   need_grad = torch.is_grad_enabled() and state.requires_grad
+  every = checkpoint_every if checkpoint_every is not None else _CKPT_EVERY[0]
+  every = _resolve_every(plan, N, len(coef) // 3, every,
+                         state.device) if need_grad else 1
   return _Nbody.apply(state, plan, N, f, fac, coef, every, need_grad)

@@ -197,6 +197,23 @@ int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
                       const double* coef, float* gstate, float* work,
                       void* stream);
 
+/* Taped forward + reverse sweep: the same integration as pmx_nbody_run over
+ * steps [0, nsteps) with the jump-start force, recording the positions and
+ * the force meshes of every force evaluation in `tape`
+ * (pmx_nbody_tape_floats(plan, N, nsteps) floats), so that
+ * pmx_nbody_adjoint_taped needs no recomputation.  This replaces TF's
+ * stored forward intermediates of tfpm.py:425-503 under autodiff; the
+ * adjoint work is pmx_nbody_adj_work_floats(plan, N, 0) floats. */
+long long pmx_nbody_tape_floats(const pmx_plan* plan, long long N,
+                                int nsteps);
+int pmx_nbody_run_taped(const pmx_plan* plan, float* state, long long N,
+                        float f, float fac, const double* coef, int nsteps,
+                        float* tape, float* work, void* stream);
+int pmx_nbody_adjoint_taped(const pmx_plan* plan, float* tape, int nsteps,
+                            long long N, float f, float fac,
+                            const double* coef, float* gstate, float* work,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif
