@@ -369,48 +369,63 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
   const long long p =
       xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
   if (p >= tot) return;
-  const int b = (int)(p / d.N);
-  const float x = X[3 * p] * f, y = X[3 * p + 1] * f, z = X[3 * p + 2] * f;
-  const float gfx = gF[3 * p] * cf, gfy = gF[3 * p + 1] * cf,
-              gfz = gF[3 * p + 2] * cf;
+  const bool upd = u.mode == 1 || u.mode == 2;
+  // every particle stream is loaded up front (non-temporal: read once, and
+  // kept out of L2 where the gathered meshes are re-read by neighbours)
+  float xv[3], gfv[3], gxv[3], gpv[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    xv[c] = __builtin_nontemporal_load(X + 3 * p + c);
+    gfv[c] = __builtin_nontemporal_load(gF + 3 * p + c);
+    gxv[c] = __builtin_nontemporal_load(gX + 3 * p + c);
+    gpv[c] = upd ? __builtin_nontemporal_load(gP + 3 * p + c) : 0.0f;
+  }
+  const int b = d.B == 1 ? 0 : (int)(p / d.N);
+  const float x = xv[0] * f, y = xv[1] * f, z = xv[2] * f;
+  const float gfx = gfv[0] * cf, gfy = gfv[1] * cf, gfz = gfv[2] * cf;
   float sx = 0.f, sy = 0.f, sz = 0.f, rx = 0.f, ry = 0.f, rz = 0.f;
   if (pos_ok(x, y, z)) {
     const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+    // the 16 gathers are all issued before the first is used
+    float3 fm[8];
+    float rb[8];
     PMX_CORNERS({
-      const float3 fm = load3<I4>(m, cell);
-      const float v = gfx * fm.x + gfy * fm.y + gfz * fm.z;
-      const float r = rbar[cell];
+      fm[cc] = load3<I4>(m, cell);
+      rb[cc] = rbar[cell];
+    })
+    PMX_CORNERS({
+      const float v = gfx * fm[cc].x + gfy * fm[cc].y + gfz * fm[cc].z;
       const float kx = (dx * wy) * wz;
       const float ky = (wx * dy) * wz;
       const float kz = (wx * wy) * dz;
       sx += v * kx;
       sy += v * ky;
       sz += v * kz;
-      rx += r * kx;
-      ry += r * ky;
-      rz += r * kz;
+      rx += rb[cc] * kx;
+      ry += rb[cc] * ky;
+      rz += rb[cc] * kz;
     })
   }
-  float g[3] = {gX[3 * p], gX[3 * p + 1], gX[3 * p + 2]};
+  float g[3];
   const float s3[3] = {sx, sy, sz}, r3[3] = {rx, ry, rz};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    g[c] = g[c] + f * s3[c];
+    g[c] = gxv[c] + f * s3[c];
     g[c] = g[c] + f * r3[c];
-    gX[3 * p + c] = g[c];
+    __builtin_nontemporal_store(g[c], gX + 3 * p + c);
   }
-  if (u.mode == 1 || u.mode == 2) {
+  if (upd) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float gp = gP[3 * p + c] + u.d * g[c];
-      gP[3 * p + c] = gp;
+      const float gp = gpv[c] + u.d * g[c];
+      __builtin_nontemporal_store(gp, gP + 3 * p + c);
       float gf = u.k1 * gp;
       if (u.mode == 2) gf = gf + u.k2prev * gp;
-      gF[3 * p + c] = gf;
+      __builtin_nontemporal_store(gf, gF + 3 * p + c);
     }
   } else if (u.mode == 3) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) gF[3 * p + c] = 0.0f;
+    for (int c = 0; c < 3; ++c) __builtin_nontemporal_store(0.0f, gF + 3 * p + c);
   }
 }
 

@@ -6,13 +6,14 @@
 // phase 1 (k_brick_deposit): one workgroup per Lagrangian brick of
 //   particles (e.g. 8x16x16).  Particles that start close stay close, so
 //   the brick's deposits fit a small box.  The workgroup finds the box,
-//   accumulates all 8*C deposits per particle in LDS as 32-bit fixed point
+//   accumulates the 8 deposits per particle in LDS as 32-bit fixed point
 //   (integer LDS atomics run several times faster than float ones on
 //   gfx950, and integer sums do not depend on their order), and stores the
-//   dense box, converted back to float, to its own arena slot.  The fixed
-//   point scale is a power of two chosen per brick from the sum of the
-//   brick's |weights| so no cell can overflow: one deposit is rounded to
-//   at most 2^-31 of that sum.
+//   dense box, converted back to float, to its own arena slot; the C
+//   channels are deposited one after the other through the same box.  The
+//   fixed-point scale is a power of two chosen per brick and channel from
+//   the sum of the channel's |weights| so no cell can overflow: one deposit
+//   is rounded to at most 2^-31 of that sum.
 //   A brick whose box does not fit the LDS budget or the arena falls back
 //   to device-scope float atomics straight into the mesh.
 // phase 2 (k_brick_gather): one workgroup per 16^3 mesh tile evaluates the
@@ -81,9 +82,9 @@ template <int C>
 __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
   __shared__ int box[kBoxFloats];  // fixed-point accumulators
   __shared__ int red[4][6];
-  __shared__ float redw[4];
+  __shared__ float redw[4][C];
   __shared__ int sh[10];
-  __shared__ float shs[2];
+  __shared__ float shs[C][2];
   const BrickGrid& g = a.g;
   const int bid = blockIdx.x, b = blockIdx.y;
   const int tid = threadIdx.x;
@@ -120,7 +121,9 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     }
   }
   int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
-  float sw = 0.0f;  // sum over particles of max_c |weight|
+  float sw[C];  // per channel: sum over particles of |weight|
+#pragma unroll
+  for (int c = 0; c < C; ++c) sw[c] = 0.0f;
 #pragma unroll
   for (int u = 0; u < kPPT; ++u) {
     px[u] *= ps;
@@ -135,10 +138,8 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
         mn[d] = min(mn[d], f[d]);
         mx[d] = max(mx[d], f[d]);
       }
-      float am = 0.0f;
 #pragma unroll
-      for (int c = 0; c < C; ++c) am = fmaxf(am, fabsf(wc[u][c]));
-      sw += am;
+      for (int c = 0; c < C; ++c) sw[c] += fabsf(wc[u][c]);
     }
   }
 #pragma unroll
@@ -147,7 +148,9 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     mx[d] = wave_max(mx[d]);
   }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) sw += __shfl_xor(sw, o);
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sw[c] += __shfl_xor(sw[c], o);
   const int wv = tid >> 6;
   if ((tid & 63) == 0) {
 #pragma unroll
@@ -155,7 +158,8 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
       red[wv][d] = mn[d];
       red[wv][3 + d] = mx[d];
     }
-    redw[wv] = sw;
+#pragma unroll
+    for (int c = 0; c < C; ++c) redw[wv][c] = sw[c];
   }
   __syncthreads();
   if (tid == 0) {
@@ -182,24 +186,27 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     }
     int mode = 0;  // 0: nothing, 1: LDS box, 2: device atomics
     const long long off =
-        ((long long)b * g.nbricks + bid) * (long long)kBoxFloats;
+        ((long long)b * g.nbricks + bid) * (long long)(C * kBoxFloats);
     int R = 0;
-    // fixed-point scale 2^e with (sum of |weights|) * 2^e <= 2^30: no cell
+    // fixed-point scale 2^e with (sum of |weights|) * 2^e < 2^30: no cell
     // total (nor a partial sum) can leave the int32 range
-    const float swt = (redw[0] + redw[1]) + (redw[2] + redw[3]);
-    int e = 0;
-    const bool wfin = swt <= 3.0e38f;  // false for inf / nan weights
-    if (wfin && swt > 0.0f) {
-      int ex;
-      frexpf(swt, &ex);  // swt = m 2^ex, m in [0.5, 1)
-      e = min(max(30 - ex, -120), 120);
+    bool wfin = true;
+    for (int c = 0; c < C; ++c) {
+      const float swt = (redw[0][c] + redw[1][c]) + (redw[2][c] + redw[3][c]);
+      int e = 0;
+      wfin = wfin && swt <= 3.0e38f;  // false for inf / nan weights
+      if (swt <= 3.0e38f && swt > 0.0f) {
+        int ex;
+        frexpf(swt, &ex);  // swt = m 2^ex, m in [0.5, 1)
+        e = min(max(30 - ex, -120), 120);
+      }
+      shs[c][0] = ldexpf(1.0f, e);
+      shs[c][1] = ldexpf(1.0f, -e);
     }
-    shs[0] = ldexpf(1.0f, e);
-    shs[1] = ldexpf(1.0f, -e);
     if (any) {
       // every brick owns a fixed arena slot: no shared allocator, and no
       // shared counter either (k_brick_reduce takes the radius maximum)
-      mode = (small && vol * C <= kBoxFloats && wfin) ? 1 : 2;
+      mode = (small && vol <= kBoxFloats && wfin) ? 1 : 2;
       if (mode == 1) {
         // box radius around the brick's own mesh interval, for the gather
         const int q0[3] = {q0x, q0y, q0z}, cc[3] = {cx, cy, cz};
@@ -235,37 +242,20 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
   if (mode == 0) return;
   const int lo0 = sh[1], lo1 = sh[2], lo2 = sh[3];
   const int e1 = sh[5], e2 = sh[6], vol = sh[7];
-  const float fxs = shs[0], fxinv = shs[1];
-  if (mode == 1) {
-    for (int i = tid; i < vol * C; i += 256) box[i] = 0;
-    __syncthreads();
-  }
   constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};
   constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};
   constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+  if (mode == 2) {
+    const MeshDims& d = a.d;
 #pragma unroll
-  for (int u = 0; u < kPPT; ++u) {
-    if (!ok[u]) continue;
-    const float x = px[u], y = py[u], z = pz[u];
-    const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
-    const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
-    const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
-    const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
-    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-    if (mode == 1) {
-      const int bx = ix - lo0, by = iy - lo1, bz = iz - lo2;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
-                         (OZ[k] ? w1z : w0z);
-        const int li = ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-          atomicAdd(&box[c * vol + li],
-                    __float2int_rn((wc[u][c] * kw) * fxs));
-      }
-    } else {
-      const MeshDims& d = a.d;
+    for (int u = 0; u < kPPT; ++u) {
+      if (!ok[u]) continue;
+      const float x = px[u], y = py[u], z = pz[u];
+      const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+      const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
+      const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
+      const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
+      const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
@@ -278,12 +268,36 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
           atomicAdd(a.mesh[c] + cell, wc[u][c] * kw);
       }
     }
+    return;
   }
-  if (mode == 1) {
+  const long long off = (long long)sh[8] | ((long long)sh[9] << 31);
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const float fxs = shs[c][0], fxinv = shs[c][1];
+    for (int i = tid; i < vol; i += 256) box[i] = 0;
     __syncthreads();
-    const long long off = (long long)sh[8] | ((long long)sh[9] << 31);
-    float* dst = a.arena + off;
-    for (int i = tid; i < vol * C; i += 256) dst[i] = (float)box[i] * fxinv;
+#pragma unroll
+    for (int u = 0; u < kPPT; ++u) {
+      if (!ok[u]) continue;
+      const float x = px[u], y = py[u], z = pz[u];
+      const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+      const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
+      const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
+      const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
+      const int bx = (int)flx - lo0, by = (int)fly - lo1, bz = (int)flz - lo2;
+      const float wq = wc[u][c] * fxs;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
+                         (OZ[k] ? w1z : w0z);
+        const int li = ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
+        atomicAdd(&box[li], __float2int_rn(wq * kw));
+      }
+    }
+    __syncthreads();
+    float* dst = a.arena + off + (long long)c * vol;
+    for (int i = tid; i < vol; i += 256) dst[i] = (float)box[i] * fxinv;
+    if (c + 1 < C) __syncthreads();  // the box is cleared for the next one
   }
 }
 
@@ -474,7 +488,7 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
         const int vol = e0 * e1 * e2;
         const float* src =
             a.arena + ((long long)b * g.nbricks + lst[j][0]) *
-                          (long long)kBoxFloats;
+                          (long long)(C * kBoxFloats);
         const int a0 = lst[j][1];
         // branch-free: every load is issued (out-of-box ones read box
         // element 0) before the first is consumed, so the 16 x C loads of
@@ -518,8 +532,9 @@ BrickGrid make_brick_grid(int gx, int gy, int gz, int f, int C) {
   g.gy = gy;
   g.gz = gz;
   g.f = f;
-  // mesh extent of a brick ~ 8 x 16 x 16 cells (C = 1) or 8 x 8 x 16 (C = 3)
-  const int bx = 8, by = C == 1 ? 16 : 8, bz = 16;
+  // mesh extent of a brick ~ 8 x 16 x 16 cells
+  (void)C;
+  const int bx = 8, by = 16, bz = 16;
   g.ex = std::max(1, std::min(gx, bx / f));
   g.ey = std::max(1, std::min(gy, by / f));
   g.ez = std::max(1, std::min(gz, bz / f));
@@ -538,8 +553,8 @@ bool brick_paint_ok(const MeshDims& d, const BrickGrid& g) {
 }
 
 long long brick_work_floats(const MeshDims& d, const BrickGrid& g, int C) {
-  (void)C;
-  const long long arena = (long long)d.B * g.nbricks * kBoxFloats;
+  // one arena slot of C boxes per brick
+  const long long arena = (long long)d.B * g.nbricks * C * kBoxFloats;
   const long long meta = 8LL * d.B * g.nbricks;
   return arena + meta + 16;
 }
@@ -552,7 +567,7 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
   BrickArgs a{};
   for (int c = 0; c < C; ++c) a.mesh[c] = mesh[c];
   a.arena = work;
-  a.arena_cap = (long long)d.B * g.nbricks * kBoxFloats;
+  a.arena_cap = (long long)d.B * g.nbricks * C * kBoxFloats;
   a.meta = reinterpret_cast<int*>(work + a.arena_cap);
   a.ctr = a.meta + 8LL * d.B * g.nbricks;
   a.X = X;
