@@ -360,25 +360,55 @@ __global__ __launch_bounds__(256) void k_force_vjp_particles(
 //   then, by mode: 1: gP += d gX ; gF = k1 gP
 //                  2: as 1, plus gF += k2prev gP
 //                  3: gF = 0 (the jump-start force replaced F)
+// With the interleaved mesh (I4) rbar is its fourth component and the rbar
+// argument is unused.
 template <bool I4>
 __global__ __launch_bounds__(256) void k_force_vjp_finish(
     float* __restrict__ gX, float* __restrict__ gP, float* __restrict__ gF,
     const float* __restrict__ X, Mesh3 m, const float* __restrict__ rbar,
     MeshDims d, float f, float cf, AdjUpdate u) {
+  __shared__ float sh[4][768];
   const long long tot = (long long)d.B * d.N;
-  const long long p =
-      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
-  if (p >= tot) return;
+  const long long p0 = xcd_block(blockIdx.x, gridDim.x) * 256;
+  const int t = threadIdx.x;
+  const long long p = p0 + t;
+  const bool full = p0 + 256 <= tot;
   const bool upd = u.mode == 1 || u.mode == 2;
-  // every particle stream is loaded up front (non-temporal: read once, and
-  // kept out of L2 where the gathered meshes are re-read by neighbours)
+  // particle streams: non-temporal (read once, kept out of L2 where the
+  // gathered mesh is re-read by neighbouring blocks); a full block stages
+  // them through LDS so every load and store is a contiguous run
   float xv[3], gfv[3], gxv[3], gpv[3];
+  if (full) {
+    float v[4][3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    xv[c] = __builtin_nontemporal_load(X + 3 * p + c);
-    gfv[c] = __builtin_nontemporal_load(gF + 3 * p + c);
-    gxv[c] = __builtin_nontemporal_load(gX + 3 * p + c);
-    gpv[c] = upd ? __builtin_nontemporal_load(gP + 3 * p + c) : 0.0f;
+    for (int k = 0; k < 3; ++k) {
+      const long long o = 3 * p0 + t + 256 * k;
+      v[0][k] = __builtin_nontemporal_load(X + o);
+      v[1][k] = __builtin_nontemporal_load(gF + o);
+      v[2][k] = __builtin_nontemporal_load(gX + o);
+      v[3][k] = upd ? __builtin_nontemporal_load(gP + o) : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) sh[q][t + 256 * k] = v[q][k];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xv[c] = sh[0][3 * t + c];
+      gfv[c] = sh[1][3 * t + c];
+      gxv[c] = sh[2][3 * t + c];
+      gpv[c] = sh[3][3 * t + c];
+    }
+  } else {
+    if (p >= tot) return;  // no barrier follows on this path
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xv[c] = X[3 * p + c];
+      gfv[c] = gF[3 * p + c];
+      gxv[c] = gX[3 * p + c];
+      gpv[c] = upd ? gP[3 * p + c] : 0.0f;
+    }
   }
   const int b = d.B == 1 ? 0 : (int)(p / d.N);
   const float x = xv[0] * f, y = xv[1] * f, z = xv[2] * f;
@@ -386,12 +416,19 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
   float sx = 0.f, sy = 0.f, sz = 0.f, rx = 0.f, ry = 0.f, rz = 0.f;
   if (pos_ok(x, y, z)) {
     const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
-    // the 16 gathers are all issued before the first is used
+    // the 8 (or 16) gathers are all issued before the first is used
     float3 fm[8];
     float rb[8];
     PMX_CORNERS({
-      fm[cc] = load3<I4>(m, cell);
-      rb[cc] = rbar[cell];
+      if (I4) {
+        // interleaved: (force x, y, z, density cotangent)
+        const float4 v = m.f4[cell];
+        fm[cc] = make_float3(v.x, v.y, v.z);
+        rb[cc] = v.w;
+      } else {
+        fm[cc] = load3<I4>(m, cell);
+        rb[cc] = rbar[cell];
+      }
     })
     PMX_CORNERS({
       const float v = gfx * fm[cc].x + gfy * fm[cc].y + gfz * fm[cc].z;
@@ -406,26 +443,41 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
       rz += rb[cc] * kz;
     })
   }
-  float g[3];
+  float g[3], gp[3], gf[3];
   const float s3[3] = {sx, sy, sz}, r3[3] = {rx, ry, rz};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     g[c] = gxv[c] + f * s3[c];
     g[c] = g[c] + f * r3[c];
-    __builtin_nontemporal_store(g[c], gX + 3 * p + c);
+    gp[c] = gpv[c] + u.d * g[c];
+    gf[c] = u.k1 * gp[c];
+    if (u.mode == 2) gf[c] = gf[c] + u.k2prev * gp[c];
+    if (u.mode == 3) gf[c] = 0.0f;
   }
-  if (upd) {
+  const bool wgf = upd || u.mode == 3;
+  if (full) {
+    __syncthreads();  // every thread has read its inputs from sh
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float gp = gpv[c] + u.d * g[c];
-      __builtin_nontemporal_store(gp, gP + 3 * p + c);
-      float gf = u.k1 * gp;
-      if (u.mode == 2) gf = gf + u.k2prev * gp;
-      __builtin_nontemporal_store(gf, gF + 3 * p + c);
+      sh[0][3 * t + c] = g[c];
+      sh[1][3 * t + c] = gp[c];
+      sh[2][3 * t + c] = gf[c];
     }
-  } else if (u.mode == 3) {
+    __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 3; ++c) __builtin_nontemporal_store(0.0f, gF + 3 * p + c);
+    for (int k = 0; k < 3; ++k) {
+      const long long o = 3 * p0 + t + 256 * k;
+      __builtin_nontemporal_store(sh[0][t + 256 * k], gX + o);
+      if (upd) __builtin_nontemporal_store(sh[1][t + 256 * k], gP + o);
+      if (wgf) __builtin_nontemporal_store(sh[2][t + 256 * k], gF + o);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gX[3 * p + c] = g[c];
+      if (upd) gP[3 * p + c] = gp[c];
+      if (wgf) gF[3 * p + c] = gf[c];
+    }
   }
 }
 

@@ -225,9 +225,12 @@ static int force_meshes(const Plan& p, const float* rho, float2* w, float* fx,
 
 // Adjoint of the map (g_x, g_y, g_z) <- force_meshes(rho):
 // grho = IFFT3(sum_d conj(lap i g_d) FFT3(g_d) * rho_scale).
+// f4w (optional): write grho into component 3 of this interleaved
+// (x, y, z, -) mesh instead of to grho.
 static int force_meshes_adj(const Plan& p, const float* gx, const float* gy,
                             const float* gz, float2* w, float* grho,
-                            float rho_scale, hipStream_t s) {
+                            float rho_scale, hipStream_t s,
+                            float* f4w = nullptr) {
   float2* ws[3] = {w, w + p.d.S, w + 2 * p.d.S};
   const float* in[3] = {gx, gy, gz};
   PMX_TRY(r2c_zy(p, in, ws, 3, rho_scale, s));
@@ -251,7 +254,11 @@ static int force_meshes_adj(const Plan& p, const float* gx, const float* gy,
   PMX_TRY(launch_ypass(p, yj, 1, s));
   ZJobs zj{};
   zj.n = 1;
-  zj.j[0] = zc2r(ws[0], grho, 1.0f, 0);
+  zj.j[0] = zc2r(ws[0], f4w ? f4w : grho, 1.0f, 0);
+  if (f4w) {
+    zj.j[0].ostride = 4;
+    zj.j[0].ocomp = 3;
+  }
   PMX_TRY(launch_zpass_c2r(p, zj, s));
   return kOk;
 }
@@ -613,13 +620,14 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
 // pass (recomputed when null).
 static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
                      float* gP, long long N, float f, float fac,
-                     const NbodyWork& nw, float* gm, const float* fm,
+                     const NbodyWork& nw, float* gm, float* fm,
                      const AdjUpdate& u, hipStream_t s) {
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
   MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
-  const Mesh3 m = m3(fm ? fm : nw.f4);
+  float* f4 = fm ? fm : nw.f4;
+  const Mesh3 m = m3(f4);
   if (!fm) PMX_TRY(force_at(p, X, N, f, nw, s));
   float* gms[3] = {gm, gm + M, gm + 2 * M};
   const bool grid = paint_meshes(p, gms, 3, X, gF, 3, fac / f, N, f,
@@ -628,9 +636,11 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
   const float sc = (float)(inv_nbar / (double)Mb);
   if (grid) {
     // mesh cotangents by the brick paint; one fused particle pass at the end
-    PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nw.rho, sc,
-                             s));
-    return launch_force_vjp_finish(gX, gP, gF, X, m, nw.rho, md, f, fac / f,
+    // the density cotangent goes to the unused fourth component of the
+    // force mesh, so the finish kernel gathers one float4 per corner
+    PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nullptr, sc,
+                             s, f4));
+    return launch_force_vjp_finish(gX, gP, gF, X, m, nullptr, md, f, fac / f,
                                    u, s);
   }
   for (int c = 0; c < 3; ++c) PMX_TRY(launch_fill(gms[c], M, 0.0f, s));

@@ -261,6 +261,7 @@ struct AdjUpdate {
   float d, k1, k2prev;
   int mode;  // 0 none, 1 update, 2 update + k2prev, 3 zero gF
 };
+// m.f4 set: the density cotangent is m.f4[cell].w and rbar is unused
 int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
                             const Mesh3& m, const float* rbar,
                             const MeshDims& d, float f, float cf,
