@@ -226,7 +226,7 @@ def main():
           "peak": HBM_PEAK_GBS,
           "unit": "GB/s",
           "frac": achieved / HBM_PEAK_GBS,
-          "traffic": None,
+          "traffic": hbm_traffic(dom, n),
       },
       "step_alg_bytes": sum(
           (algorithmic_bytes(k, n, N) or 0) * v["launches"] / (K + 1)
@@ -237,6 +237,30 @@ def main():
       "cpu_baseline": cpu,
   }
   print(json.dumps(out))
+
+
+# rocprofv3 counter kernel names of the profiler kinds
+_KIND_KERNEL = {
+    "readout_kick_drift": "k_readout_kick_drift", "paint": "k_brick_deposit<1>",
+    "paint_gather": "k_brick_gather<1>", "fft_z_r2c": "k_zpass_r2c",
+    "fft_y_fwd": "k_ypass<256, 16, 256, -1", "fft_x_fused": "k_xpass<256, 16",
+    "fft_y_inv": "k_ypass<256, 16, 256, 1, true", "fft_z_c2r": "k_zpass_c2r",
+    "fill": "k_fill",
+}
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "pm_step_256_hbm_traffic.md")
+
+
+def hbm_traffic(kind, n):
+  """HBM bytes per launch of this kernel kind, from the committed rocprofv3
+  FETCH_SIZE / WRITE_SIZE summary of the 256^3 workload (each counter scaled
+  by what it reads for a plain 1 GiB copy); None when not measured."""
+  if n != 256 or kind not in _KIND_KERNEL or not os.path.exists(TRAFFIC_TABLE):
+    return None
+  for line in open(TRAFFIC_TABLE):
+    if line.startswith("| `") and _KIND_KERNEL[kind] in line:
+      cols = [c.strip() for c in line.strip().strip("|").split("|")]
+      return float(cols[-1]) * 1e6
+  return None
 
 
 def live_profile(_lib, fn):
