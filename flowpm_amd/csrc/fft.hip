@@ -272,10 +272,60 @@ __global__ __launch_bounds__(T) void k_zpass_r2c(ZJobs jobs, PlanDev p) {
 // A thread that loads (X1[k], X2[k]) writes both Z[k] = X1 + i X2 and
 // Z[N-k] = conj(X1) + i conj(X2), so the half rows never need their own LDS
 // staging buffer.
+// Stages job J's rows [r0, r0 + nr) of the half spectrum into the LDS tile
+// as packed complex lines: line l = rows 2l and 2l + 1.
+template <int N, int TW, int T>
+__device__ __forceinline__ void c2r_load(float2* s, const ZJob& J,
+                                         const PlanDev& p, long long r0,
+                                         int nr) {
+  constexpr int P = Pitch<TW>::P;
+  constexpr int H = N / 2;
+  const float2* __restrict__ src = J.cin + r0 * H;
+  const float* __restrict__ gz = p.ax[2].g;
+  const int ez = J.ez;
+  // element q = (line l, k), k = 0..H (k = H is the Nyquist plane)
+  constexpr int NE = TW * (H + 1);
+  constexpr int E = (NE + T - 1) / T;
+  float2 x1[E], x2[E];
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int q = threadIdx.x + u * T;
+    const int l = q / (H + 1), k = q % (H + 1);
+    const int ra = 2 * l, rb = 2 * l + 1;
+    const bool in = NE % T == 0 || q < NE;
+    x1[u] = make_float2(0.f, 0.f);
+    x2[u] = make_float2(0.f, 0.f);
+    if (in && ra < nr)
+      x1[u] = k < H ? src[ra * H + k] : J.cin[p.nyq0 + r0 + ra];
+    if (in && rb < nr)
+      x2[u] = k < H ? src[rb * H + k] : J.cin[p.nyq0 + r0 + rb];
+  }
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int q = threadIdx.x + u * T;
+    if (NE % T == 0 || q < NE) {
+      const int l = q / (H + 1), k = q % (H + 1);
+      float2 a = x1[u], b = x2[u];
+      if (ez) {
+        const float g = gz[k];
+        const float c = ez == 2 ? g * g : g;
+        a = rot_i(cscale(a, c), ez);
+        b = rot_i(cscale(b, c), ez);
+      }
+      if (k == 0 || k == H) {
+        a.y = 0.0f;
+        b.y = 0.0f;
+      }
+      s[k * P + l] = make_float2(a.x - b.y, a.y + b.x);
+      if (k > 0 && k < H)  // conj(X1) + i conj(X2)
+        s[(N - k) * P + l] = make_float2(a.x + b.y, -a.y + b.x);
+    }
+  }
+}
+
 template <int N, int TW, int T>
 __global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
   constexpr int P = Pitch<TW>::P;
-  constexpr int H = N / 2;
   __shared__ float2 s[N * P];
   __shared__ float2 tw[N];
   // 1-D grid: the jobs of one row chunk are consecutive logical blocks on
@@ -286,49 +336,7 @@ __global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
   load_twiddles<N, T>(tw, p.ax[2].tw);
   const long long r0 = (lb / jobs.n) * (2 * TW);
   const int nr = (int)min((long long)(2 * TW), p.rows - r0);
-  const float2* __restrict__ src = J.cin + r0 * H;
-  const float* __restrict__ gz = p.ax[2].g;
-  const int ez = J.ez;
-  {
-    // element q = (line l, k), k = 0..H (k = H is the Nyquist plane)
-    constexpr int NE = TW * (H + 1);
-    constexpr int E = (NE + T - 1) / T;
-    float2 x1[E], x2[E];
-#pragma unroll
-    for (int u = 0; u < E; ++u) {
-      const int q = threadIdx.x + u * T;
-      const int l = q / (H + 1), k = q % (H + 1);
-      const int ra = 2 * l, rb = 2 * l + 1;
-      const bool in = NE % T == 0 || q < NE;
-      x1[u] = make_float2(0.f, 0.f);
-      x2[u] = make_float2(0.f, 0.f);
-      if (in && ra < nr)
-        x1[u] = k < H ? src[ra * H + k] : J.cin[p.nyq0 + r0 + ra];
-      if (in && rb < nr)
-        x2[u] = k < H ? src[rb * H + k] : J.cin[p.nyq0 + r0 + rb];
-    }
-#pragma unroll
-    for (int u = 0; u < E; ++u) {
-      const int q = threadIdx.x + u * T;
-      if (NE % T == 0 || q < NE) {
-        const int l = q / (H + 1), k = q % (H + 1);
-        float2 a = x1[u], b = x2[u];
-        if (ez) {
-          const float g = gz[k];
-          const float c = ez == 2 ? g * g : g;
-          a = rot_i(cscale(a, c), ez);
-          b = rot_i(cscale(b, c), ez);
-        }
-        if (k == 0 || k == H) {
-          a.y = 0.0f;
-          b.y = 0.0f;
-        }
-        s[k * P + l] = make_float2(a.x - b.y, a.y + b.x);
-        if (k > 0 && k < H)  // conj(X1) + i conj(X2)
-          s[(N - k) * P + l] = make_float2(a.x + b.y, -a.y + b.x);
-      }
-    }
-  }
+  c2r_load<N, TW, T>(s, J, p, r0, nr);
   __syncthreads();
   fft_tile<N, TW, T, +1>(s, tw);
   // output element (row, z) at rout[((r0 + row) * N + z) * os + oc]
@@ -343,6 +351,51 @@ __global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
       float* o = dst + (long long)q * os;
       *o = J.accum ? *o + r : r;
     }
+  }
+}
+
+// The three force-mesh rows of one chunk in one workgroup: jobs 0, 1, 2 are
+// transformed one after the other through the same LDS tile, their outputs
+// kept in registers, and each cell is written once as the float4
+// (job 0, job 1, job 2, 0) -- whole-line stores instead of three strided
+// partial ones.
+template <int N, int TW, int T>
+__global__ __launch_bounds__(T) void k_zpass_c2r_f4(ZJobs jobs, PlanDev p) {
+  constexpr int P = Pitch<TW>::P;
+  constexpr int NO = 2 * TW * N;  // real outputs per job
+  constexpr int EO = (NO + T - 1) / T;
+  __shared__ float2 s[N * P];
+  __shared__ float2 tw[N];
+  const long long lb = xcd_block(blockIdx.x, gridDim.x);
+  load_twiddles<N, T>(tw, p.ax[2].tw);
+  const long long r0 = lb * (2 * TW);
+  const int nr = (int)min((long long)(2 * TW), p.rows - r0);
+  float res[3][EO];
+#pragma unroll
+  for (int jb = 0; jb < 3; ++jb) {
+    const ZJob& J = jobs.j[jb];
+    if (jb > 0) __syncthreads();  // every read of the previous tile is done
+    c2r_load<N, TW, T>(s, J, p, r0, nr);
+    __syncthreads();
+    fft_tile<N, TW, T, +1>(s, tw);
+#pragma unroll
+    for (int u = 0; u < EO; ++u) {
+      const int q = threadIdx.x + u * T;
+      const int row = q / N, z = q % N;
+      float r = 0.0f;
+      if (NO % T == 0 || q < NO) {
+        const float2 v = s[z * P + (row >> 1)];
+        r = J.scale * ((row & 1) ? v.y : v.x);
+      }
+      res[jb][u] = r;
+    }
+  }
+  float4* __restrict__ dst = reinterpret_cast<float4*>(jobs.j[0].rout) + r0 * N;
+#pragma unroll
+  for (int u = 0; u < EO; ++u) {
+    const int q = threadIdx.x + u * T;
+    if ((NO % T == 0 || q < NO) && q / N < nr)
+      dst[q] = make_float4(res[0][u], res[1][u], res[2][u], 0.0f);
   }
 }
 
@@ -692,13 +745,30 @@ static int zr2c_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   PMX_LAUNCH_CHECK();
   return kOk;
 }
+// jobs 0..2 writing components 0..2 of one float4 mesh, plain stores
+static bool is_f4_triplet(const ZJobs& jobs) {
+  if (jobs.n != 3) return false;
+  for (int d = 0; d < 3; ++d)
+    if (jobs.j[d].rout != jobs.j[0].rout || jobs.j[d].ostride != 4 ||
+        jobs.j[d].ocomp != d || jobs.j[d].accum)
+      return false;
+  return (reinterpret_cast<uintptr_t>(jobs.j[0].rout) & 15) == 0;
+}
+
 template <int N>
 static int zc2r_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   constexpr int TW = Cfg<N>::ZTW, T = Cfg<N>::T;
-  dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)) * jobs.n);
   ProfScope prof_(kProfZc2r, s);
-  hipLaunchKernelGGL((k_zpass_c2r<N, TW, T>), grid, dim3(T), 0, s, jobs,
-                     pl.d);
+  if (is_f4_triplet(jobs)) {
+    constexpr int TW4 = TW >= 16 ? 8 : TW;
+    dim3 grid((unsigned)((pl.d.rows + 2 * TW4 - 1) / (2 * TW4)));
+    hipLaunchKernelGGL((k_zpass_c2r_f4<N, TW4, T>), grid, dim3(T), 0, s, jobs,
+                       pl.d);
+  } else {
+    dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)) * jobs.n);
+    hipLaunchKernelGGL((k_zpass_c2r<N, TW, T>), grid, dim3(T), 0, s, jobs,
+                       pl.d);
+  }
   PMX_LAUNCH_CHECK();
   return kOk;
 }
