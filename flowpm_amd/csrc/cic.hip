@@ -206,29 +206,65 @@ __global__ __launch_bounds__(256) void k_readout_adj(
 // The (B,N,3) state rows are moved through LDS so that every global load
 // and store of X, P and F is a contiguous, coalesced dword stream instead
 // of three 12-byte-strided accesses per particle.
+// Particle tiles of the fused readout: with a particle grid, a workgroup
+// takes 4 (y) x 64 (z) particles at one x and consecutive workgroups step x
+// first, so the mesh rows that neighbouring tiles gather are shared in L1 /
+// L2 (late-time positions are far from their grid cells: one particle row
+// per workgroup made each mesh line be fetched from L2 ~4 times); the
+// 64-particle runs keep the particle streams in 768-byte contiguous pieces.
+constexpr int kTileY = 4, kTileZ = 64;
+
+__device__ __forceinline__ bool rkd_tiled(const KickDrift& c) {
+  return c.gx > 0 && c.gy % kTileY == 0 && c.gz % kTileZ == 0;
+}
+
+// first particle of logical workgroup lb and the stride between its 4 runs
+__device__ __forceinline__ long long rkd_tile(long long lb, const KickDrift& c,
+                                              long long N, long long* rs) {
+  const long long per_b = (long long)c.gx * (c.gy / kTileY) * (c.gz / kTileZ);
+  const long long b = lb / per_b, r = lb % per_b;
+  const int x = (int)(r % c.gx);
+  const long long tile = r / c.gx;
+  const int ntz = c.gz / kTileZ;
+  const int ty = (int)(tile / ntz), tz = (int)(tile % ntz);
+  *rs = c.gz;
+  return b * N + ((long long)x * c.gy + ty * kTileY) * c.gz + tz * kTileZ;
+}
+
 template <bool I4>
 __global__ __launch_bounds__(256) void k_readout_kick_drift(
     float* __restrict__ X, float* __restrict__ P, float* __restrict__ F,
     Mesh3 m, MeshDims d, KickDrift c) {
   __shared__ float sx[768], sp[768], sf[768];
   const long long tot = (long long)d.B * d.N;
-  const long long p0 = xcd_block(blockIdx.x, gridDim.x) * 256;
-  const int np = (int)min((long long)256, tot - p0);
+  const long long lb = xcd_block(blockIdx.x, gridDim.x);
+  const bool tiled = rkd_tiled(c);
+  long long p0, rs = 0;
+  if (tiled)
+    p0 = rkd_tile(lb, c, d.N, &rs);
+  else
+    p0 = lb * 256;
+  const int np = tiled ? 256 : (int)min((long long)256, tot - p0);
   const int nf = 3 * np;
   const bool kick = c.do_k2 || c.do_k1d;
   const int t = threadIdx.x;
+  // float offset of staged element i (tiled: 4 runs of 192 floats)
+  auto goff = [&](int i) -> long long {
+    if (!tiled) return 3 * p0 + i;
+    const int r = i / (3 * kTileZ), o = i % (3 * kTileZ);
+    return 3 * (p0 + r * rs) + o;
+  };
   // particle state is streamed with non-temporal loads / stores so it does
   // not evict the force mesh, which neighbouring blocks re-read, from L2
   if (np == 256) {
     // full block: all loads in flight before the first LDS write
     float vx[3], vp[3];
-    const float* xs = X + 3 * p0 + t;
-    const float* ps = P + 3 * p0 + t;
     if (kick) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        vx[k] = __builtin_nontemporal_load(xs + 256 * k);
-        vp[k] = __builtin_nontemporal_load(ps + 256 * k);
+        const long long o = goff(t + 256 * k);
+        vx[k] = __builtin_nontemporal_load(X + o);
+        vp[k] = __builtin_nontemporal_load(P + o);
       }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -237,7 +273,8 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) vx[k] = __builtin_nontemporal_load(xs + 256 * k);
+      for (int k = 0; k < 3; ++k)
+        vx[k] = __builtin_nontemporal_load(X + goff(t + 256 * k));
 #pragma unroll
       for (int k = 0; k < 3; ++k) sx[t + 256 * k] = vx[k];
     }
@@ -249,19 +286,20 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   }
   __syncthreads();
   if (t < np) {
-    const long long p = p0 + t;
-    const int b = (int)(p / d.N);
+    // a tile never spans batch members; a linear block may
+    const int b = d.B == 1 ? 0 : (int)((tiled ? p0 : p0 + t) / d.N);
     float px = sx[3 * t], py = sx[3 * t + 1], pz = sx[3 * t + 2];
     const float x = px * c.f, y = py * c.f, z = pz * c.f;
     float ax = 0.f, ay = 0.f, az = 0.f;
     if (pos_ok(x, y, z)) {
       const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+      float3 v[8];
+      PMX_CORNERS({ v[cc] = load3<I4>(m, cell); })
       PMX_CORNERS({
         const float k = (wx * wy) * wz;
-        const float3 v = load3<I4>(m, cell);
-        ax += v.x * k;
-        ay += v.y * k;
-        az += v.z * k;
+        ax += v[cc].x * k;
+        ay += v[cc].y * k;
+        az += v[cc].z * k;
       })
     }
     const float Fx = (ax * c.fac) / c.f, Fy = (ay * c.fac) / c.f,
@@ -293,10 +331,21 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
     }
   }
   __syncthreads();
-  for (int i = t; i < nf; i += 256) {
-    if (!c.skip_f) __builtin_nontemporal_store(sf[i], F + 3 * p0 + i);
-    if (kick) __builtin_nontemporal_store(sp[i], P + 3 * p0 + i);
-    if (c.do_k1d) __builtin_nontemporal_store(sx[i], X + 3 * p0 + i);
+  if (np == 256) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int i = t + 256 * k;
+      const long long o = goff(i);
+      if (!c.skip_f) __builtin_nontemporal_store(sf[i], F + o);
+      if (kick) __builtin_nontemporal_store(sp[i], P + o);
+      if (c.do_k1d) __builtin_nontemporal_store(sx[i], X + o);
+    }
+  } else {
+    for (int i = t; i < nf; i += 256) {
+      if (!c.skip_f) __builtin_nontemporal_store(sf[i], F + 3 * p0 + i);
+      if (kick) __builtin_nontemporal_store(sp[i], P + 3 * p0 + i);
+      if (c.do_k1d) __builtin_nontemporal_store(sx[i], X + 3 * p0 + i);
+    }
   }
 }
 
@@ -548,12 +597,16 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   ProfScope prof_(kProfReadoutKD, s);
+  KickDrift cc = c;
+  const bool tiled = c.gx > 0 && c.gy % kTileY == 0 && c.gz % kTileZ == 0 &&
+                     (long long)c.gx * c.gy * c.gz == d.N;
+  if (!tiled) cc.gx = cc.gy = cc.gz = 0;
   if (m.f4)
     hipLaunchKernelGGL(k_readout_kick_drift<true>, dim3(nblk(tot, 256)),
-                       dim3(256), 0, s, X, P, F, m, d, c);
+                       dim3(256), 0, s, X, P, F, m, d, cc);
   else
     hipLaunchKernelGGL(k_readout_kick_drift<false>, dim3(nblk(tot, 256)),
-                       dim3(256), 0, s, X, P, F, m, d, c);
+                       dim3(256), 0, s, X, P, F, m, d, cc);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

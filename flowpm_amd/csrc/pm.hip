@@ -575,6 +575,17 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   KickDrift kd{};
   kd.fac = fac;
   kd.f = f;
+  {
+    // particles on a grid of (n / f)^3 (the nbody state): tiled readout
+    const int fi = (int)f;
+    if ((float)fi == f && fi >= 1 && p.d.nx % fi == 0 && p.d.ny % fi == 0 &&
+        p.d.nz % fi == 0 &&
+        (long long)(p.d.nx / fi) * (p.d.ny / fi) * (p.d.nz / fi) == N) {
+      kd.gx = p.d.nx / fi;
+      kd.gy = p.d.ny / fi;
+      kd.gz = p.d.nz / fi;
+    }
+  }
   if (jump) {
     PMX_TRY(keep_x(tape.xj, X));
     const NbodyWork nw = at(tape.fj);

@@ -222,6 +222,9 @@ struct KickDrift {
   float d;        // drift of the next step
   int do_k2, do_k1d;
   int skip_f;     // F is consumed in registers by the next kick: not stored
+  // particle grid (N = gx*gy*gz per batch member, C order) or 0: lets the
+  // fused readout visit 4 x 64 (y, z) particle tiles, x fastest
+  int gx, gy, gz;
 };
 // The three force meshes: separate (x, y, z) or interleaved float4 (f4).
 struct Mesh3 {

@@ -228,6 +228,27 @@ def test_nbody_readme_workflow_32(gpu):
   assert ex < 1e-4 and ep < 1e-4 and er < 1e-4
 
 
+def test_nbody_64_batch2_against_oracle(gpu):
+  """nbody at 64^3 with two batch members: the fused readout visits 4 x 64
+  particle tiles here (grid sides divisible by 64), the brick paint and the
+  batched FFT all run on B = 2."""
+  n, L = 64, 200.0
+  white, pk = _ics(n, L, 300, B=2)
+  g = obg.GrowthTable(obg.planck15())
+  stages = np.linspace(0.1, 1.0, 6)
+  lin = opm.whitenoise_to_linear(white, L, pk)
+  st_o = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=2), stages, [n] * 3)
+  c = fa.cosmology.Planck15()
+  st_g = fa.nbody(c, fa.lpt_init(c, lin.float().to(gpu), 0.1, order=2),
+                  stages, n)
+  q = opm.lagrangian_grid([n] * 3, 2, torch.float64)
+  ex = rel_rms(st_g[0] - q.float().to(gpu), st_o[0] - q)
+  ep = rel_rms(st_g[1], st_o[1])
+  ef = rel_rms(st_g[2], st_o[2])
+  print("64^3 B=2 nbody rel RMS: dX %.3g P %.3g F %.3g" % (ex, ep, ef))
+  assert ex < 1e-4 and ep < 1e-4 and ef < 1e-4
+
+
 def test_nbody_gradient_against_oracle(gpu):
   n, L = 16, 64.0
   white, pk = _ics(n, L, 11)
