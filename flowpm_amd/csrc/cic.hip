@@ -214,21 +214,31 @@ __global__ __launch_bounds__(256) void k_readout_adj(
 // 64-particle runs keep the particle streams in 768-byte contiguous pieces.
 constexpr int kTileY = 4, kTileZ = 64;
 
-__device__ __forceinline__ bool rkd_tiled(const KickDrift& c) {
-  return c.gx > 0 && c.gy % kTileY == 0 && c.gz % kTileZ == 0;
+__host__ __device__ __forceinline__ bool grid_tiled(int gx, int gy, int gz,
+                                                    long long N) {
+  return gx > 0 && gy % kTileY == 0 && gz % kTileZ == 0 &&
+         (long long)gx * gy * gz == N;
 }
 
-// first particle of logical workgroup lb and the stride between its 4 runs
-__device__ __forceinline__ long long rkd_tile(long long lb, const KickDrift& c,
-                                              long long N, long long* rs) {
-  const long long per_b = (long long)c.gx * (c.gy / kTileY) * (c.gz / kTileZ);
+// first particle of logical workgroup lb (its 4 runs are gz apart)
+__device__ __forceinline__ long long grid_tile(long long lb, int gx, int gy,
+                                               int gz, long long N) {
+  const long long per_b = (long long)gx * (gy / kTileY) * (gz / kTileZ);
   const long long b = lb / per_b, r = lb % per_b;
-  const int x = (int)(r % c.gx);
-  const long long tile = r / c.gx;
-  const int ntz = c.gz / kTileZ;
+  const int x = (int)(r % gx);
+  const long long tile = r / gx;
+  const int ntz = gz / kTileZ;
   const int ty = (int)(tile / ntz), tz = (int)(tile % ntz);
-  *rs = c.gz;
-  return b * N + ((long long)x * c.gy + ty * kTileY) * c.gz + tz * kTileZ;
+  return b * N + ((long long)x * gy + ty * kTileY) * gz + tz * kTileZ;
+}
+
+// float offset of staged element i (0..767) of a 256-particle workgroup:
+// tiled, 4 runs of 192 floats gz particles apart; else contiguous
+__device__ __forceinline__ long long stage_off(bool tiled, long long p0,
+                                               int gz, int i) {
+  if (!tiled) return 3 * p0 + i;
+  const int r = i / (3 * kTileZ), o = i % (3 * kTileZ);
+  return 3 * (p0 + (long long)r * gz) + o;
 }
 
 template <bool I4>
@@ -238,22 +248,14 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   __shared__ float sx[768], sp[768], sf[768];
   const long long tot = (long long)d.B * d.N;
   const long long lb = xcd_block(blockIdx.x, gridDim.x);
-  const bool tiled = rkd_tiled(c);
-  long long p0, rs = 0;
-  if (tiled)
-    p0 = rkd_tile(lb, c, d.N, &rs);
-  else
-    p0 = lb * 256;
+  const bool tiled = c.gx > 0;  // the launcher checked the grid
+  const long long p0 =
+      tiled ? grid_tile(lb, c.gx, c.gy, c.gz, d.N) : lb * 256;
   const int np = tiled ? 256 : (int)min((long long)256, tot - p0);
   const int nf = 3 * np;
   const bool kick = c.do_k2 || c.do_k1d;
   const int t = threadIdx.x;
-  // float offset of staged element i (tiled: 4 runs of 192 floats)
-  auto goff = [&](int i) -> long long {
-    if (!tiled) return 3 * p0 + i;
-    const int r = i / (3 * kTileZ), o = i % (3 * kTileZ);
-    return 3 * (p0 + r * rs) + o;
-  };
+  auto goff = [&](int i) { return stage_off(tiled, p0, c.gz, i); };
   // particle state is streamed with non-temporal loads / stores so it does
   // not evict the force mesh, which neighbouring blocks re-read, from L2
   if (np == 256) {
@@ -418,10 +420,13 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
     MeshDims d, float f, float cf, AdjUpdate u) {
   __shared__ float sh[4][768];
   const long long tot = (long long)d.B * d.N;
-  const long long p0 = xcd_block(blockIdx.x, gridDim.x) * 256;
+  const long long lb = xcd_block(blockIdx.x, gridDim.x);
+  const bool tiled = u.gx > 0;  // the launcher checked the grid
+  const long long p0 =
+      tiled ? grid_tile(lb, u.gx, u.gy, u.gz, d.N) : lb * 256;
   const int t = threadIdx.x;
-  const long long p = p0 + t;
-  const bool full = p0 + 256 <= tot;
+  const long long p = p0 + t;  // (linear order only)
+  const bool full = tiled || p0 + 256 <= tot;
   const bool upd = u.mode == 1 || u.mode == 2;
   // particle streams: non-temporal (read once, kept out of L2 where the
   // gathered mesh is re-read by neighbouring blocks); a full block stages
@@ -431,7 +436,7 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
     float v[4][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const long long o = 3 * p0 + t + 256 * k;
+      const long long o = stage_off(tiled, p0, u.gz, t + 256 * k);
       v[0][k] = __builtin_nontemporal_load(X + o);
       v[1][k] = __builtin_nontemporal_load(gF + o);
       v[2][k] = __builtin_nontemporal_load(gX + o);
@@ -459,7 +464,8 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
       gpv[c] = upd ? gP[3 * p + c] : 0.0f;
     }
   }
-  const int b = d.B == 1 ? 0 : (int)(p / d.N);
+  // a tile never spans batch members; a linear block may
+  const int b = d.B == 1 ? 0 : (int)((tiled ? p0 : p) / d.N);
   const float x = xv[0] * f, y = xv[1] * f, z = xv[2] * f;
   const float gfx = gfv[0] * cf, gfy = gfv[1] * cf, gfz = gfv[2] * cf;
   float sx = 0.f, sy = 0.f, sz = 0.f, rx = 0.f, ry = 0.f, rz = 0.f;
@@ -515,7 +521,7 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const long long o = 3 * p0 + t + 256 * k;
+      const long long o = stage_off(tiled, p0, u.gz, t + 256 * k);
       __builtin_nontemporal_store(sh[0][t + 256 * k], gX + o);
       if (upd) __builtin_nontemporal_store(sh[1][t + 256 * k], gP + o);
       if (wgf) __builtin_nontemporal_store(sh[2][t + 256 * k], gF + o);
@@ -598,9 +604,7 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
   if (tot == 0) return kOk;
   ProfScope prof_(kProfReadoutKD, s);
   KickDrift cc = c;
-  const bool tiled = c.gx > 0 && c.gy % kTileY == 0 && c.gz % kTileZ == 0 &&
-                     (long long)c.gx * c.gy * c.gz == d.N;
-  if (!tiled) cc.gx = cc.gy = cc.gz = 0;
+  if (!grid_tiled(c.gx, c.gy, c.gz, d.N)) cc.gx = cc.gy = cc.gz = 0;
   if (m.f4)
     hipLaunchKernelGGL(k_readout_kick_drift<true>, dim3(nblk(tot, 256)),
                        dim3(256), 0, s, X, P, F, m, d, cc);
@@ -630,12 +634,14 @@ int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   ProfScope prof_(kProfAdjCic, s);
+  AdjUpdate uu = u;
+  if (!grid_tiled(u.gx, u.gy, u.gz, d.N)) uu.gx = uu.gy = uu.gz = 0;
   if (m.f4)
     hipLaunchKernelGGL(k_force_vjp_finish<true>, dim3(nblk(tot, 256)),
-                       dim3(256), 0, s, gX, gP, gF, X, m, rbar, d, f, cf, u);
+                       dim3(256), 0, s, gX, gP, gF, X, m, rbar, d, f, cf, uu);
   else
     hipLaunchKernelGGL(k_force_vjp_finish<false>, dim3(nblk(tot, 256)),
-                       dim3(256), 0, s, gX, gP, gF, X, m, rbar, d, f, cf, u);
+                       dim3(256), 0, s, gX, gP, gF, X, m, rbar, d, f, cf, uu);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

@@ -453,6 +453,21 @@ static inline unsigned gridfor(long long n) {
 }
 
 // ---------------------------------------------------------------- nbody --
+// Particles on the Lagrangian grid of (n / f)^3 (the nbody state) let the
+// particle passes visit tiles; 0 otherwise.
+static void particle_grid(const Plan& p, long long N, float f, int* gx,
+                          int* gy, int* gz) {
+  *gx = *gy = *gz = 0;
+  const int fi = (int)f;
+  if ((float)fi == f && fi >= 1 && p.d.nx % fi == 0 && p.d.ny % fi == 0 &&
+      p.d.nz % fi == 0 &&
+      (long long)(p.d.nx / fi) * (p.d.ny / fi) * (p.d.nz / fi) == N) {
+    *gx = p.d.nx / fi;
+    *gy = p.d.ny / fi;
+    *gz = p.d.nz / fi;
+  }
+}
+
 static inline Mesh3 m3(const float* f4) {
   return Mesh3{nullptr, nullptr, nullptr, reinterpret_cast<const float4*>(f4)};
 }
@@ -575,17 +590,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   KickDrift kd{};
   kd.fac = fac;
   kd.f = f;
-  {
-    // particles on a grid of (n / f)^3 (the nbody state): tiled readout
-    const int fi = (int)f;
-    if ((float)fi == f && fi >= 1 && p.d.nx % fi == 0 && p.d.ny % fi == 0 &&
-        p.d.nz % fi == 0 &&
-        (long long)(p.d.nx / fi) * (p.d.ny / fi) * (p.d.nz / fi) == N) {
-      kd.gx = p.d.nx / fi;
-      kd.gy = p.d.ny / fi;
-      kd.gz = p.d.nz / fi;
-    }
-  }
+  particle_grid(p, N, f, &kd.gx, &kd.gy, &kd.gz);
   if (jump) {
     PMX_TRY(keep_x(tape.xj, X));
     const NbodyWork nw = at(tape.fj);
@@ -713,11 +718,12 @@ static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
   float* gP = gstate + BN3;
   float* gF = gstate + 2 * BN3;
   for (int i = b1 - 1; i >= b0; --i) {
-    AdjUpdate u;
+    AdjUpdate u{};
     u.d = (float)coef[3 * i + 1];
     u.k1 = (float)coef[3 * i];
     u.k2prev = i > 0 ? (float)coef[3 * (i - 1) + 2] : 0.f;
     u.mode = i > 0 ? 2 : 1;
+    particle_grid(p, N, f, &u.gx, &u.gy, &u.gz);
     const long long slot = i - tp.first;
     PMX_TRY(force_vjp(p, tp.xs + slot * BN3, gF, gX, gP, N, f, fac, nw, gm,
                       tp.fs + slot * 4 * M, u, s));
@@ -771,7 +777,8 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
     PMX_TRY(reverse_steps(p, tp, b0, b1, N, f, fac, coef, gstate, nw, gm, s));
   }
   // jump-start force at the initial positions; the initial F is discarded
-  AdjUpdate u0{0.f, 0.f, 0.f, 3};
+  AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
+  particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
   return force_vjp(p, ckpt, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
                    fac, nw, gm, nsteps > 0 ? fjump : nullptr, u0, s);
 }
@@ -788,7 +795,8 @@ static int nbody_adjoint_taped(const Plan& p, float* tape, int nsteps,
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
   PMX_TRY(reverse_steps(p, tp, 0, nsteps, N, f, fac, coef, gstate, nw, gm,
                         s));
-  AdjUpdate u0{0.f, 0.f, 0.f, 3};
+  AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
+  particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
   return force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
                    fac, nw, gm, tp.fj, u0, s);
 }

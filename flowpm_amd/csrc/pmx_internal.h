@@ -263,6 +263,7 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
 struct AdjUpdate {
   float d, k1, k2prev;
   int mode;  // 0 none, 1 update, 2 update + k2prev, 3 zero gF
+  int gx, gy, gz;  // particle grid (tiled particle order) or 0
 };
 // m.f4 set: the density cotangent is m.f4[cell].w and rbar is unused
 int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
