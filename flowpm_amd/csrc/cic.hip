@@ -341,12 +341,15 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
       if (!c.skip_f) __builtin_nontemporal_store(sf[i], F + o);
       if (kick) __builtin_nontemporal_store(sp[i], P + o);
       if (c.do_k1d) __builtin_nontemporal_store(sx[i], X + o);
+      if (c.do_k1d && c.xcopy) __builtin_nontemporal_store(sx[i], c.xcopy + o);
     }
   } else {
     for (int i = t; i < nf; i += 256) {
       if (!c.skip_f) __builtin_nontemporal_store(sf[i], F + 3 * p0 + i);
       if (kick) __builtin_nontemporal_store(sp[i], P + 3 * p0 + i);
       if (c.do_k1d) __builtin_nontemporal_store(sx[i], X + 3 * p0 + i);
+      if (c.do_k1d && c.xcopy)
+        __builtin_nontemporal_store(sx[i], c.xcopy + 3 * p0 + i);
     }
   }
 }

@@ -591,6 +591,11 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   kd.fac = fac;
   kd.f = f;
   particle_grid(p, N, f, &kd.gx, &kd.gy, &kd.gz);
+  // X slot of step i in the tape (null when not kept)
+  auto xslot = [&](int i) -> float* {
+    return tape.xs ? tape.xs + (long long)(i - tape.first) * BN3 : nullptr;
+  };
+  bool x_kept = false;  // X of the coming force is already in its slot
   if (jump) {
     PMX_TRY(keep_x(tape.xj, X));
     const NbodyWork nw = at(tape.fj);
@@ -600,6 +605,8 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     if (kd.do_k1d) {
       kd.k1 = (float)coef[3 * b0];
       kd.d = (float)coef[3 * b0 + 1];
+      kd.xcopy = xslot(b0);  // the drifted X goes to the tape as well
+      x_kept = kd.xcopy != nullptr;
     }
     PMX_TRY(launch_readout_kick_drift(X, P, F, m3(nw.f4), md,
                                       kd, s));
@@ -613,15 +620,19 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   }
   for (int i = b0; i < b1; ++i) {
     const long long slot = i - tape.first;
-    PMX_TRY(keep_x(tape.xs ? tape.xs + slot * BN3 : nullptr, X));
+    if (!x_kept) PMX_TRY(keep_x(xslot(i), X));
     const NbodyWork nw = at(tape.fs ? tape.fs + slot * 4 * M : nullptr);
     PMX_TRY(force_at(p, X, N, f, nw, s));
     kd.do_k2 = 1;
     kd.k2 = (float)coef[3 * i + 2];
     kd.do_k1d = (i + 1 < b1);
+    kd.xcopy = nullptr;
+    x_kept = false;
     if (kd.do_k1d) {
       kd.k1 = (float)coef[3 * (i + 1)];
       kd.d = (float)coef[3 * (i + 1) + 1];
+      kd.xcopy = xslot(i + 1);
+      x_kept = kd.xcopy != nullptr;
     }
     // only the last force of the run has to stay in the state
     kd.skip_f = kd.do_k1d;
