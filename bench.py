@@ -307,10 +307,10 @@ def _time(fn, reps):
 
 
 def batched_runs(fa, cosmo, dev):
-  """Forward nbody throughput of many simulations (ICs excluded):
-  64^3 sims (box 200, 1LPT a0=0.1, 10 steps) batched in one plan under a
-  captured graph vs one at a time; and the uneven mix of 8 each of
-  32/64/96/128^3 (boxes 100/200/300/400) under one graph."""
+  """Nbody throughput of many simulations (ICs excluded): 64^3 sims (box
+  200, 1LPT a0=0.1, 10 steps) batched in one plan under a captured graph vs
+  one at a time; the uneven mix of 8 each of 32/64/96/128^3 (boxes
+  100/200/300/400) forward under one graph, and forward + gradient."""
   from flowpm_amd.batched import BatchedNbody
   stages = np.linspace(0.1, 1.0, 11)
   B = 64
@@ -328,6 +328,14 @@ def batched_runs(fa, cosmo, dev):
                        stages=stages, nc=n))
   mix = BatchedNbody(groups)
   t_mix = _time(mix.run, 3)
+
+  def mix_grad():
+    # forward and gradient of every group (taped adjoint per size class)
+    for g in groups:
+      x0 = g["state"].detach().requires_grad_(True)
+      out = fa.nbody(cosmo, x0, stages, g["nc"])
+      torch.autograd.grad(out, x0, grad_outputs=torch.ones_like(out))
+  t_mix_grad = _time(mix_grad, 2)
   return {
       "sims64_batch": B,
       "sims64_per_s_batched": B / (t_b * 1e-3),
@@ -335,7 +343,10 @@ def batched_runs(fa, cosmo, dev):
       "speedup_vs_sequential": (B / t_b) / (1.0 / t_1),
       "uneven_mix_ms": t_mix,
       "uneven_mix_sims_per_s": 32 / (t_mix * 1e-3),
-      "note": "10-step forward nbody per sim, ICs excluded; graph replay",
+      "uneven_mix_fwd_grad_ms": t_mix_grad,
+      "uneven_mix_fwd_grad_sims_per_s": 32 / (t_mix_grad * 1e-3),
+      "note": "10-step nbody per sim, ICs excluded; forward: graph replay; "
+              "fwd_grad: forward + taped adjoint per size class",
   }
 
 

@@ -28,6 +28,8 @@ _SIGS = {
     "pmx_profile_kind_name": (_c.c_char_p, [_i]),
     "pmx_plan_create": (_i, [_c.POINTER(_vp), _i, _i, _i, _i]),
     "pmx_plan_destroy": (_i, [_vp]),
+    "pmx_fft_plan_create": (_i, [_c.POINTER(_vp), _i, _i, _i, _i]),
+    "pmx_fft_plan_destroy": (_i, [_vp]),
     "pmx_plan_spec_elems": (_ll, [_vp]),
     "pmx_plan_set_tiles": (_i, [_vp, _i, _i]),
     "pmx_cic_paint": (_i, [_vp, _vp, _vp, _i, _ll, _i, _i, _i, _f, _vp]),
@@ -59,6 +61,11 @@ _SIGS = {
     "pmx_nbody_adj_work_floats": (_ll, [_vp, _ll, _i]),
     "pmx_nbody_adjoint": (_i, [_vp, _vp, _i, _i, _ll, _f, _f, _dp, _vp, _vp,
                                _vp]),
+    "pmx_nbody_batched_step": (_i, [_i, _c.POINTER(_vp), _c.POINTER(_vp),
+                                    _c.POINTER(_ll), _c.POINTER(_f),
+                                    _c.POINTER(_f), _c.POINTER(_dp),
+                                    _c.POINTER(_i), _c.POINTER(_i),
+                                    _c.POINTER(_i), _c.POINTER(_vp), _vp]),
     "pmx_nbody_tape_floats": (_ll, [_vp, _ll, _i]),
     "pmx_nbody_run_taped": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _vp, _vp,
                                  _vp]),

@@ -69,6 +69,28 @@ class BatchedNbody:
     for st in self.streams:
       main.wait_stream(st)
 
+  def run_one_call(self):
+    """Integrates every group with one pmx_nbody_batched_step call (all
+    groups enqueued on the current stream, no graph); returns the final
+    states."""
+    self.reset()
+    G = len(self.groups)
+    c = _lib._c
+    arr = lambda typ, vals: (typ * G)(*vals)
+    _lib.call("pmx_nbody_batched_step", G,
+              arr(_lib._vp, [g.plan.handle for g in self.groups]),
+              arr(_lib._vp, [g.state.data_ptr() for g in self.groups]),
+              arr(_lib._ll, [g.N for g in self.groups]),
+              arr(_lib._f, [g.f for g in self.groups]),
+              arr(_lib._f, [g.fac for g in self.groups]),
+              arr(_lib._dp, [c.cast(g.cptr, _lib._dp) for g in self.groups]),
+              arr(_lib._i, [0] * G),
+              arr(_lib._i, [g.nsteps for g in self.groups]),
+              arr(_lib._i, [1] * G),
+              arr(_lib._vp, [g.work.data_ptr() for g in self.groups]),
+              _lib.stream_of(self.groups[0].state))
+    return [g.state for g in self.groups]
+
   def reset(self, states=None):
     for k, g in enumerate(self.groups):
       src = self.init[k] if states is None else tfpm._f32(states[k])

@@ -888,6 +888,12 @@ int pmx_plan_destroy(pmx_plan* plan) {
   return 0;
 }
 
+int pmx_fft_plan_create(pmx_plan** plan, int B, int nx, int ny, int nz) {
+  return pmx_plan_create(plan, B, nx, ny, nz);
+}
+
+int pmx_fft_plan_destroy(pmx_plan* plan) { return pmx_plan_destroy(plan); }
+
 long long pmx_plan_spec_elems(const pmx_plan* plan) {
   return plan ? plan->p.d.S : -1;
 }
@@ -1242,6 +1248,33 @@ int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
 long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N,
                                     int every) {
   return (plan && every >= 0) ? nbody_adj_work_floats(P(plan), N, every) : -1;
+}
+
+int pmx_nbody_batched_step(int ngroups, const pmx_plan* const* plans,
+                           float* const* states, const long long* N,
+                           const float* f, const float* fac,
+                           const double* const* coefs, const int* step_begin,
+                           const int* step_end, const int* jump_start,
+                           float* const* work, void* stream) {
+  PMX_REQUIRE(ngroups >= 0, "pmx_nbody_batched_step: ngroups < 0");
+  if (ngroups == 0) return 0;
+  PMX_REQUIRE(plans && states && N && f && fac && coefs && step_begin &&
+                  step_end && jump_start && work,
+              "pmx_nbody_batched_step: null array");
+  // validate every group before anything is enqueued
+  for (int g = 0; g < ngroups; ++g) {
+    PMX_REQUIRE(plans[g] && states[g] && work[g],
+                "pmx_nbody_batched_step: null pointer in a group");
+    PMX_REQUIRE(N[g] >= 1, "pmx_nbody_batched_step: group without particles");
+    PMX_REQUIRE(step_begin[g] >= 0 && step_end[g] >= step_begin[g] &&
+                    (coefs[g] || step_end[g] == step_begin[g]),
+                "pmx_nbody_batched_step: bad step range");
+  }
+  for (int g = 0; g < ngroups; ++g)
+    PMX_TRY(nbody_run(P(plans[g]), states[g], N[g], f[g], fac[g], coefs[g],
+                      step_begin[g], step_end[g], jump_start[g], work[g],
+                      S(stream)));
+  return 0;
 }
 
 long long pmx_nbody_tape_floats(const pmx_plan* plan, long long N,

@@ -50,6 +50,11 @@ const char* pmx_profile_kind_name(int kind);
  * (flowpm/kernels.py:10-25, :28-78) computed in float64. */
 int pmx_plan_create(pmx_plan** plan, int B, int nx, int ny, int nz);
 int pmx_plan_destroy(pmx_plan* plan);
+/* The same plan under the names of the FFT plan pair (TF builds its FFT3D
+ * plans inside the op, utils.py:213-272; one plan here serves the FFT, the
+ * Fourier multipliers and the nbody loop of one batch shape). */
+int pmx_fft_plan_create(pmx_plan** plan, int B, int nx, int ny, int nz);
+int pmx_fft_plan_destroy(pmx_plan* plan);
 long long pmx_plan_spec_elems(const pmx_plan* plan);
 /* Column-tile widths (float2 columns) of the y and x line passes; the
  * autotuner sets them.  Allowed: 8 or 16 (16 only for lines <= 512). */
@@ -196,6 +201,20 @@ int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
                       int nsteps, long long N, float f, float fac,
                       const double* coef, float* gstate, float* work,
                       void* stream);
+
+/* Several independent groups of simulations of uneven sizes (one plan per
+ * size class; the reference batches only equal sizes, tfpm.py:15-63):
+ * group g advances its state over steps [step_begin[g], step_end[g]) of
+ * its own coefficient table, exactly as pmx_nbody_run would, all enqueued
+ * on `stream` in group order (for concurrent groups, capture one stream
+ * per group into a graph, as flowpm_amd.batched does).  Arrays have
+ * ngroups entries; work[g] holds pmx_nbody_work_floats(plans[g]) floats. */
+int pmx_nbody_batched_step(int ngroups, const pmx_plan* const* plans,
+                           float* const* states, const long long* N,
+                           const float* f, const float* fac,
+                           const double* const* coefs, const int* step_begin,
+                           const int* step_end, const int* jump_start,
+                           float* const* work, void* stream);
 
 /* Taped forward + reverse sweep: the same integration as pmx_nbody_run over
  * steps [0, nsteps) with the jump-start force, recording the positions and

@@ -61,3 +61,19 @@ def test_same_size_batch_matches_singles(gpu):
   for b in range(3):
     one = fa.nbody(cosmo, st[:, b:b + 1].contiguous(), stages, 32)
     assert _rel(out[:, b:b + 1], one) < 1e-5
+
+
+def test_batched_step_entry_matches_nbody(gpu):
+  """pmx_nbody_batched_step over uneven groups = one nbody per group."""
+  cosmo = fa.cosmology.Planck15()
+  stages = np.linspace(0.1, 1.0, 5)
+  spec = [(16, 1), (24, 2), (32, 1)]
+  groups, ref = [], []
+  for i, (n, B) in enumerate(spec):
+    st = _state(cosmo, n, B, 2000 + i, gpu)
+    groups.append(dict(cosmo=cosmo, state=st, stages=stages, nc=n))
+    ref.append(fa.nbody(cosmo, st, stages, n))
+  out = BatchedNbody(groups, graph=False).run_one_call()
+  torch.cuda.synchronize()
+  for o, r in zip(out, ref):
+    assert _rel(o, r) < 1e-6
