@@ -66,6 +66,7 @@ def main():
   ap.add_argument("--warmup", type=int, default=2)
   ap.add_argument("--n", type=int, default=256)
   ap.add_argument("--box", type=float, default=512.0)
+  ap.add_argument("--lpt-order", type=int, default=2, choices=(1, 2))
   ap.add_argument("--every", type=int, default=None,
                   help="adjoint checkpoint interval (0: taped, no "
                   "recomputation; default: automatic)")
@@ -105,7 +106,7 @@ def main():
   white = (torch.randn(1, n, n, n, generator=gen, dtype=torch.float64) *
            n**1.5).float().to(dev)
   lin = fa.whitenoise_to_linear(n, L, white, pk)
-  state0 = fa.lpt_init(cosmo, lin, 0.1, order=2)
+  state0 = fa.lpt_init(cosmo, lin, 0.1, order=args.lpt_order)
   stages = np.linspace(0.1, 1.0, K + 1)
   coef = fa.tfpm.step_coefficients(cosmo, stages)
   plan = _plans.get(dev, 1, n, n, n)
@@ -209,9 +210,10 @@ def main():
       "data": "synthetic",
       "config": {
           "workload": "single sim, %d^3 particles, %d^3 force mesh, box %g "
-                      "Mpc/h, Planck15 EH P(k), 2LPT a0=0.1, %d KDK steps "
-                      "to a=1" % (n, n, L, K),
-          "mesh": n, "particles": N, "box_mpc_h": L, "lpt_order": 2,
+                      "Mpc/h, Planck15 EH P(k), %dLPT a0=0.1, %d KDK steps "
+                      "to a=1" % (n, n, L, args.lpt_order, K),
+          "mesh": n, "particles": N, "box_mpc_h": L,
+          "lpt_order": args.lpt_order,
           "a0": 0.1, "checkpoint_every": every_used,
           "fft_tiles": {"y": tiles[0], "x": tiles[1]},
       },
