@@ -190,9 +190,23 @@ __device__ __forceinline__ float2 apply_term(float2 v, const Term& t, int kx,
                                              int ky, int kz,
                                              const PlanDev& p) {
   float c = t.scale;
-  if (t.lap) {
-    double kk = p.ax[0].k2[kx] + p.ax[1].k2[ky] + p.ax[2].k2[kz];
-    c *= (kk == 0.0) ? 1.0f : (float)(1.0 / kk);
+  if (t.lap || t.pgd) {
+    const double kk = p.ax[0].k2[kx] + p.ax[1].k2[ky] + p.ax[2].k2[kz];
+    // the reference sets the zero mode's kk to 1 before its mask is formed,
+    // so the mask keeps everything (kernels.py:43-47, :128-133)
+    const double kk1 = kk == 0.0 ? 1.0 : kk;
+    if (t.lap) c *= (float)(1.0 / kk1);
+    if (t.pgd) {
+      // float32 as TF evaluates it: exp(-kl^2 / kk) exp(-kk^2 / ks^4)
+      const float e1 = expf(-t.pgd_kl2 / (float)kk1);
+      const float e2 = expf(-(float)(kk1 * kk1) / t.pgd_ks4);
+      c *= e1 * e2;
+    }
+  }
+  if (t.cic) {
+    // (sinc_x sinc_y sinc_z)^-2 in float64, rounded once (kernels.py:137-152)
+    const double sp = p.ax[0].sinc[kx] * p.ax[1].sinc[ky] * p.ax[2].sinc[kz];
+    c *= (float)(1.0 / (sp * sp));
   }
   const int kidx[3] = {kx, ky, kz};
   int E = 0;

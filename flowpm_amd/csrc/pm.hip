@@ -64,9 +64,11 @@ static int plan_create(Plan** out, int B, int nx, int ny, int nz) {
   d.S = d.rows * (d.H + 1);
   const int n[3] = {nx, ny, nz};
   size_t bytes = 0;
-  size_t off_tw[3], off_k2[3], off_g[3];
+  size_t off_tw[3], off_k2[3], off_g[3], off_sinc[3];
   for (int a = 0; a < 3; ++a) {
     off_k2[a] = bytes;
+    bytes += sizeof(double) * n[a];
+    off_sinc[a] = bytes;
     bytes += sizeof(double) * n[a];
     off_tw[a] = bytes;
     bytes += sizeof(float2) * n[a];
@@ -79,12 +81,17 @@ static int plan_create(Plan** out, int B, int nx, int ny, int nz) {
     double* k2 = reinterpret_cast<double*>(host.data() + off_k2[a]);
     float2* tw = reinterpret_cast<float2*>(host.data() + off_tw[a]);
     float* g = reinterpret_cast<float*>(host.data() + off_g[a]);
+    double* sn = reinterpret_cast<double*>(host.data() + off_sinc[a]);
     for (int i = 0; i < n[a]; ++i) {
       // numpy.fft.fftfreq: i/n for i < (n+1)/2 (n even: i < n/2), else
       // (i - n)/n; times 2 pi (kernels.py:16-17)
       const int fi = (i < (n[a] + 1) / 2) ? i : i - n[a];
       const double w = (double)fi / (double)n[a] * 2.0 * M_PI;
       k2[i] = w * w;
+      // numpy.sinc(w / 2 pi): sin(pi x) / (pi x), x = 0 -> 1e-20
+      const double x = w / (2.0 * M_PI);
+      const double y = M_PI * (x == 0.0 ? 1.0e-20 : x);
+      sn[i] = std::sin(y) / y;
       g[i] = (float)((8.0 * std::sin(w) - std::sin(2.0 * w)) / 6.0);
       const double ang = -2.0 * M_PI * (double)i / (double)n[a];
       tw[i] = make_float2((float)std::cos(ang), (float)std::sin(ang));
@@ -108,13 +115,14 @@ static int plan_create(Plan** out, int B, int nx, int ny, int nz) {
     d.ax[a].k2 = reinterpret_cast<const double*>(base + off_k2[a]);
     d.ax[a].tw = reinterpret_cast<const float2*>(base + off_tw[a]);
     d.ax[a].g = reinterpret_cast<const float*>(base + off_g[a]);
+    d.ax[a].sinc = reinterpret_cast<const double*>(base + off_sinc[a]);
   }
   *out = p;
   return kOk;
 }
 
 static inline Term term(float scale, int lap, int ex, int ey, int ez) {
-  Term t;
+  Term t{};
   t.scale = scale;
   t.lap = lap;
   t.e[0] = ex;
@@ -1177,17 +1185,29 @@ int pmx_fourier_apply(const pmx_plan* plan, int nin, const float* const* in,
   PMX_REQUIRE(plan && in && e && scales && work && out,
               "pmx_fourier_apply: null pointer");
   PMX_REQUIRE(nin >= 1 && nin <= 3, "pmx_fourier_apply: 1 <= nin <= 3");
-  Term t[3];
+  PMX_REQUIRE((lap & ~3) == 0, "pmx_fourier_apply: unknown kernel flag");
+  Term t[3] = {};
   for (int q = 0; q < nin; ++q) {
     PMX_REQUIRE(in[q] != nullptr, "pmx_fourier_apply: null input");
     for (int d = 0; d < 3; ++d)
       PMX_REQUIRE(e[3 * q + d] >= 0 && e[3 * q + d] <= 2,
                   "pmx_fourier_apply: exponents must be 0, 1 or 2");
-    t[q] = term(scales[q], lap ? 1 : 0, e[3 * q], e[3 * q + 1], e[3 * q + 2]);
+    t[q] = term(scales[q], lap & 1, e[3 * q], e[3 * q + 1], e[3 * q + 2]);
+    t[q].cic = (lap >> 1) & 1;
   }
   return fourier_apply(P(plan), nin, in, t, in_scale,
                        reinterpret_cast<float2*>(work), out, accumulate,
                        S(stream));
+}
+
+int pmx_power_spectrum_bins(const pmx_plan* plan, const float* spec,
+                            const double* kx, const double* ky,
+                            const double* kz, const double* edges, int nedges,
+                            double* psum, void* stream) {
+  PMX_REQUIRE(plan && spec && kx && ky && kz && edges && psum,
+              "pmx_power_spectrum_bins: null pointer");
+  return launch_power_bins(P(plan), reinterpret_cast<const float2*>(spec), kx,
+                           ky, kz, edges, nedges, psum, S(stream));
 }
 
 int pmx_spec_mul_real(const pmx_plan* plan, float* spec, const float* amp,

@@ -112,9 +112,10 @@ __device__ __forceinline__ long long xcd_block(long long i, long long nb) {
 // ------------------------------------------------------ plan (device) ------
 // Per-axis tables, computed on the host in float64 and uploaded once.
 struct AxisTab {
-  const float2* tw;  // [n]   exp(-2 pi i t / n)
-  const double* k2;  // [n]   (2 pi fftfreq(n))^2          kernels.py:10-25
-  const float* g;    // [n]   (8 sin w - sin 2w) / 6       kernels.py:51-78
+  const float2* tw;    // [n]   exp(-2 pi i t / n)
+  const double* k2;    // [n]   (2 pi fftfreq(n))^2          kernels.py:10-25
+  const float* g;      // [n]   (8 sin w - sin 2w) / 6       kernels.py:51-78
+  const double* sinc;  // [n]   numpy.sinc(fftfreq(n))       kernels.py:137-152
 };
 
 struct PlanDev {
@@ -139,6 +140,9 @@ struct Term {
   float scale;
   int lap;
   int e[3];
+  int cic;                 // x (sinc_x sinc_y sinc_z)^-2 (cic_compensation)
+  int pgd;                 // x PGD_kernel(kl, ks) (kernels.py:105-134)
+  float pgd_kl2, pgd_ks4;  // kl^2 and ks^4
 };
 
 constexpr int kMaxJobs = 3;
@@ -241,6 +245,12 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
 int launch_kick_drift_only(float* X, float* P, const float* F,
                            const MeshDims& d, const KickDrift& c,
                            hipStream_t s);
+
+// Binned power spectrum of a half spectrum (spectrum.hip).
+constexpr int kMaxPkBins = 8192;
+int launch_power_bins(const Plan& p, const float2* spec, const double* kx,
+                      const double* ky, const double* kz, const double* edges,
+                      int nedges, double* psum, hipStream_t s);
 
 // Lagrangian-brick paint (paint.hip).
 struct BrickGrid {

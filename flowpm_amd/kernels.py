@@ -7,7 +7,8 @@ gradient weight), so no M-sized kernel array is read from HBM.
 """
 import numpy as np
 
-__all__ = ["fftk", "laplace_kernel", "gradient_kernel", "longrange_kernel"]
+__all__ = ["fftk", "laplace_kernel", "gradient_kernel", "longrange_kernel",
+           "PGD_kernel", "cic_compensation"]
 
 
 def fftk(shape, symmetric=True, finite=False, dtype=np.float64):
@@ -47,3 +48,22 @@ def longrange_kernel(kvec, r_split):
     kk = sum(ki**2 for ki in kvec)
     return np.exp(-kk * r_split**2)
   return 1.0
+
+
+def PGD_kernel(kvec, kl, ks):
+  """exp(-kl^2 / k^2) exp(-k^4 / ks^4) (kernels.py:105-134); as there, the
+  zero mode uses k^2 = 1 and is kept (its mask is formed after the
+  replacement).  The reference evaluates it in float32 (TF converts k^2 to
+  the dtype of kl); so does this host version."""
+  kk = sum(ki**2 for ki in kvec)
+  kk = np.where(kk == 0, 1.0, kk)
+  kl2 = np.float32(kl)**2
+  ks4 = np.float32(ks)**4
+  return (np.exp(-kl2 / kk.astype(np.float32)) *
+          np.exp(-(kk**2).astype(np.float32) / ks4)).astype(np.float32)
+
+
+def cic_compensation(kvec):
+  """(sinc(kx/2pi) sinc(ky/2pi) sinc(kz/2pi))^-2 (kernels.py:137-152)."""
+  kwts = [np.sinc(kvec[i] / (2 * np.pi)) for i in range(3)]
+  return (kwts[0] * kwts[1] * kwts[2])**(-2)

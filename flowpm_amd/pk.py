@@ -1,0 +1,75 @@
+"""Power spectrum (flowpm/pk.py:7-103) on the GPU.
+
+``power_spectrum(field, kmin, dk, boxsize)`` keeps the reference's
+signature and its arithmetic: the host builds the bin edges and the
+weighted mode counts exactly as ``_initialize_pk`` does (NumPy float64), the
+HIP library transforms the field (unnormalised half spectrum) and sums
+W |F|^2 per bin (``pmx_power_spectrum_bins``, the reference's full-spectrum
+weights accounted on the half spectrum), and the normalisation follows the
+reference's float32 steps.  Not differentiable (the reference's bincount
+path is used as a diagnostic / loss on fixed bins; this one returns a
+detached tensor).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from . import _plans
+from .utils import _f32, _mesh_dims, fwd_real
+
+__all__ = ["power_spectrum"]
+
+
+def _initialize_pk(shape, boxsize, kmin, dk):
+  """Bin edges, per-axis wavenumbers and weighted mode counts
+  (pk.py:7-37), NumPy float64."""
+  shape = [int(s) for s in shape]
+  boxsize = np.asarray(boxsize, dtype=np.float64)
+  W = np.empty(shape, dtype="f4")
+  W[...] = 2.0
+  W[..., 0] = 1.0
+  W[..., -1] = 1.0
+  kmax = np.pi * np.min(shape) / np.max(boxsize) + dk / 2
+  kedges = np.arange(kmin, kmax, dk)
+  k = [np.fft.fftfreq(N, 1. / (N * 2 * np.pi / L)) for N, L in
+       zip(shape, boxsize)]
+  kmag = (k[0].reshape(-1, 1, 1)**2 + k[1].reshape(1, -1, 1)**2 +
+          k[2].reshape(1, 1, -1)**2)**0.5
+  dig = np.digitize(kmag.flat, kedges)
+  Nsum = np.bincount(dig, weights=W.flat, minlength=len(kedges) + 1)
+  return k, kedges, Nsum
+
+
+def power_spectrum(field, kmin=5, dk=0.5, boxsize=False):
+  """Binned power spectrum of a real (B, nx, ny, nz) field.
+
+  Returns (kbins, power): kbins the NumPy bin centres, power a float32
+  (B, len(kbins)) tensor on the field's device (pk.py:40-103)."""
+  _lib.require_cuda(field)
+  B, nx, ny, nz = _mesh_dims(field)
+  if boxsize is False or boxsize is None:
+    raise ValueError("power_spectrum needs the box size per dimension "
+                     "(boxsize.prod() normalises the power, pk.py:90)")
+  boxsize = np.broadcast_to(np.asarray(boxsize, dtype=np.float64), (3,))
+  k, kedges, Nsum = _initialize_pk((nx, ny, nz), boxsize, kmin, dk)
+  if len(kedges) < 2:
+    raise ValueError("no complete k bin between kmin and the Nyquist "
+                     "wavenumber")
+  dev = field.device
+  plan = _plans.get(dev, B, nx, ny, nz)
+  spec = fwd_real(plan, _f32(field), 1.0)  # unnormalised, as fft3d
+  tab = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64),
+                                  device=dev)
+  kx, ky, kz, edges = tab(k[0]), tab(k[1]), tab(k[2]), tab(kedges)
+  psum = torch.empty((B, len(kedges) + 1), dtype=torch.float64, device=dev)
+  _lib.call("pmx_power_spectrum_bins", plan.handle, _lib.ptr(spec),
+            _lib.ptr(kx), _lib.ptr(ky), _lib.ptr(kz), _lib.ptr(edges),
+            len(kedges), _lib.ptr(psum), _lib.stream_of(field))
+  # float32 steps of pk.py:82-103: Psum (complex64) / Nsum (complex64),
+  # [1:-1], * boxsize.prod(), / float32(M)**2
+  nsum = torch.as_tensor(Nsum.astype(np.float32), device=dev)
+  power = psum.to(torch.float32) / nsum
+  power = power[:, 1:-1] * np.float32(boxsize.prod())
+  norm = np.float32(nx * ny * nz)**2
+  kbins = kedges[:-1] + (kedges[1:] - kedges[:-1]) / 2
+  return kbins, power / norm

@@ -20,7 +20,8 @@ import torch
 from . import _lib
 from . import _plans
 
-__all__ = ["cic_paint", "cic_readout", "r2c3d", "c2r3d", "white_noise"]
+__all__ = ["cic_paint", "cic_readout", "r2c3d", "c2r3d", "white_noise",
+           "compensate_cic"]
 
 
 def _f32(t):
@@ -245,3 +246,39 @@ def white_noise(nc, batch_size=1, seed=None, type="complex",
   if type == "complex":
     return r2c3d(white, norm=M)
   raise ValueError("type must be 'real' or 'complex'")
+
+
+class _CompensateCic(torch.autograd.Function):
+  """c2r3d(cic_compensation * r2c3d(field, norm=M), norm=M): a real,
+  symmetric Fourier filter, so its adjoint is itself."""
+
+  @staticmethod
+  def apply_op(x):
+    B, nx, ny, nz = _mesh_dims(x)
+    plan = _plans.get(x.device, B, nx, ny, nz)
+    xin = _f32(x)
+    out = torch.empty_like(xin)
+    work = plan.spec(3)
+    ins = (_lib._vp * 1)(_lib._vp(xin.data_ptr()))
+    e = (_lib._i * 3)(0, 0, 0)
+    sc = (_lib._f * 1)(1.0)
+    _lib.call("pmx_fourier_apply", plan.handle, 1, ins, e, sc, 2,
+              1.0 / (nx * ny * nz), _lib.ptr(work), _lib.ptr(out), 0,
+              _lib.stream_of(x))
+    return out
+
+  @staticmethod
+  def forward(ctx, x):
+    return _CompensateCic.apply_op(x)
+
+  @staticmethod
+  def backward(ctx, g):
+    return _CompensateCic.apply_op(g)
+
+
+def compensate_cic(field, name="CompensateCiC"):
+  """Deconvolves the CIC window (utils.py:11-29): multiplies the spectrum by
+  (sinc(kx/2pi) sinc(ky/2pi) sinc(kz/2pi))^-2 (kernels.py:137-152, Jing
+  2005 with p = 2), evaluated per element from float64 per-axis tables."""
+  _lib.require_cuda(field)
+  return _CompensateCic.apply(field)

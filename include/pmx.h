@@ -150,15 +150,31 @@ int pmx_lpt_assemble_adj(const float* gstate, const double* coef6, float* g1,
 int pmx_lpt2_source_adj(const pmx_plan* plan, const float* work,
                         const float* gsrc, float* gphi, void* stream);
 
-/* Generic Fourier filter used by the adjoints of lpt_init and force:
- * out (+)= c2r3d( sum_q scales[q] [laplace_kernel] prod_d
- *          gradient_kernel(d)^e[3q+d] r2c3d(in[q]) * in_scale ),
- * nin <= 3 real meshes, exponents 0..2 (kernels.py:28-78).
+/* Generic Fourier filter used by the adjoints of lpt_init and force, and
+ * by compensate_cic (utils.py:11-29):
+ * out (+)= c2r3d( sum_q scales[q] [laplace_kernel] [cic_compensation]
+ *          prod_d gradient_kernel(d)^e[3q+d] r2c3d(in[q]) * in_scale ),
+ * nin <= 3 real meshes, exponents 0..2 (kernels.py:28-78, :137-152).
+ * `lap` holds kernel flags: bit 0 laplace_kernel, bit 1 cic_compensation.
  * work: 3 spec buffers. */
 int pmx_fourier_apply(const pmx_plan* plan, int nin, const float* const* in,
                       const int* e, const float* scales, int lap,
                       float in_scale, float* work, float* out, int accumulate,
                       void* stream);
+
+/* Power-spectrum bins (flowpm/pk.py:7-103): psum[b][i] = sum over the
+ * modes of batch member b with numpy.digitize(|k|, edges) == i of
+ * W |F|^2, where F is the unnormalised transform held in spec (the half
+ * spectrum of pmx_fft_r2c with scale 1) and W the reference's weights
+ * (1 for the first and last full-spectrum kz index, 2 otherwise; pk.py:16-19),
+ * accounted over the full spectrum.  kx, ky, kz: device float64 tables of
+ * the full per-axis wavenumbers (nx, ny, nz values, pk.py:24-27); edges:
+ * nedges increasing float64 bin edges (nedges + 1 <= 8192); psum: device
+ * float64 [B][nedges + 1], overwritten. */
+int pmx_power_spectrum_bins(const pmx_plan* plan, const float* spec,
+                            const double* kx, const double* ky,
+                            const double* kz, const double* edges, int nedges,
+                            double* psum, void* stream);
 
 /* spec *= amp, amp a real half-spectrum array of one mesh (nx*ny*nz/2 main
  * values then nx*ny Nyquist-plane values), applied to every batch member:
