@@ -52,10 +52,10 @@ def power_spectrum(field, kmin=5, dk=0.5, boxsize=False):
                      "(boxsize.prod() normalises the power, pk.py:90)")
   boxsize = np.broadcast_to(np.asarray(boxsize, dtype=np.float64), (3,))
   k, kedges, Nsum = _initialize_pk((nx, ny, nz), boxsize, kmin, dk)
-  if len(kedges) < 2:
-    raise ValueError("no complete k bin between kmin and the Nyquist "
-                     "wavenumber")
   dev = field.device
+  if len(kedges) < 2:  # no complete bin: empty, as the reference returns
+    return kedges[:0].astype(np.float64), torch.zeros(
+        (B, 0), dtype=torch.float32, device=dev)
   plan = _plans.get(dev, B, nx, ny, nz)
   spec = fwd_real(plan, _f32(field), 1.0)  # unnormalised, as fft3d
   tab = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64),
