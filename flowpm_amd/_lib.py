@@ -54,6 +54,8 @@ _SIGS = {
     "pmx_fourier_apply": (_i, [_vp, _i, _c.POINTER(_vp), _c.POINTER(_i),
                                _c.POINTER(_f), _i, _f, _vp, _vp, _i, _vp]),
     "pmx_spec_mul_real": (_i, [_vp, _vp, _vp, _vp]),
+    "pmx_white_noise": (_i, [_vp, _ll, _c.c_ulonglong, _c.c_ulonglong,
+                             _c.c_ulonglong, _c.c_double, _vp]),
     "pmx_power_spectrum_bins": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp,
                                      _vp]),
     "pmx_readout_kick_drift": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _ll, _i,

@@ -1200,6 +1200,15 @@ int pmx_fourier_apply(const pmx_plan* plan, int nin, const float* const* in,
                        S(stream));
 }
 
+int pmx_white_noise(float* out, long long n, unsigned long long key_lo,
+                    unsigned long long key_hi, unsigned long long counter,
+                    double scale, void* stream) {
+  PMX_REQUIRE(out || n == 0, "pmx_white_noise: null output");
+  PMX_REQUIRE(n >= 0, "pmx_white_noise: n < 0");
+  return launch_white_noise(out, n, key_lo, key_hi, counter, scale,
+                            S(stream));
+}
+
 int pmx_power_spectrum_bins(const pmx_plan* plan, const float* spec,
                             const double* kx, const double* ky,
                             const double* kz, const double* edges, int nedges,

@@ -246,6 +246,11 @@ int launch_kick_drift_only(float* X, float* P, const float* F,
                            const MeshDims& d, const KickDrift& c,
                            hipStream_t s);
 
+// Philox4x64-10 Gaussian white noise (noise.hip).
+int launch_white_noise(float* out, long long n, unsigned long long k0,
+                       unsigned long long k1, unsigned long long ctr0,
+                       double scale, hipStream_t s);
+
 // Binned power spectrum of a half spectrum (spectrum.hip).
 constexpr int kMaxPkBins = 8192;
 int launch_power_bins(const Plan& p, const float2* spec, const double* kx,

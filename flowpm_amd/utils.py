@@ -10,9 +10,10 @@ hand-written HIP adjoint:
 * cic_readout (utils.py:163-210) backward: weighted paint + position gradient
 * r2c3d       (utils.py:213-241) backward: the conjugate transform (c2r)
 * c2r3d       (utils.py:244-272) backward: the conjugate transform (r2c)
-* white_noise (utils.py:337-355) N(0, sqrt(M)) drawn from a seeded
-                                 torch.Generator on the host (TF's RNG
+* white_noise (utils.py:337-355) N(0, sqrt(M)) drawn in HBM from the
+                                 seeded Philox4x64-10 stream (TF's RNG
                                  stream cannot be reproduced)
+* compensate_cic (utils.py:11-29) the CIC window deconvolution
 """
 import numpy as np
 import torch
@@ -223,24 +224,42 @@ def c2r3d(cfield, norm=None, dtype=torch.float32, name="C2R3D"):
 
 
 def white_noise(nc, batch_size=1, seed=None, type="complex",
-                name="WhiteNoise", device=None):
+                name="WhiteNoise", device=None, rng="philox"):
   """N(0, sqrt(M)) real white noise (flowpm/utils.py:337-355).
 
-  Drawn in float64 from ``torch.Generator().manual_seed(seed)`` on the host
-  and rounded to float32, so a seed gives the same field on every machine.
-  ``type='complex'`` returns r2c3d of it."""
+  TF's RNG stream cannot be reproduced, so the stream is this package's:
+  ``rng='philox'`` (default) draws it in HBM from the counter-based
+  Philox4x64-10 generator keyed by ``seed`` -- the stream of
+  ``numpy.random.Philox(key=seed)``, so the host can regenerate any
+  element (``pmx_white_noise``); ``rng='torch'`` draws float64 normals from
+  ``torch.Generator().manual_seed(seed)`` on the host.  ``type='complex'``
+  returns r2c3d of the field (norm M)."""
   if isinstance(nc, int):
     nc = [nc, nc, nc]
   M = nc[0] * nc[1] * nc[2]
-  gen = torch.Generator()
-  if seed is not None:
-    gen.manual_seed(int(seed))
-  else:
-    gen.seed()
-  white = (torch.randn([batch_size] + list(nc), generator=gen,
-                       dtype=torch.float64) * M**0.5).to(torch.float32)
   device = device if device is not None else torch.device("cuda")
-  white = white.to(device)
+  shape = [batch_size] + list(nc)
+  if rng == "philox":
+    if seed is None:
+      seed = int.from_bytes(np.random.bytes(16), "little")
+    seed = int(seed)
+    if seed < 0 or seed >= 1 << 128:
+      raise ValueError("philox seed must be in [0, 2^128)")
+    white = torch.empty(shape, dtype=torch.float32, device=device)
+    _lib.require_cuda(white)
+    _lib.call("pmx_white_noise", _lib.ptr(white), white.numel(),
+              seed & ((1 << 64) - 1), seed >> 64, 0, float(M)**0.5,
+              _lib.stream_of(white))
+  elif rng == "torch":
+    gen = torch.Generator()
+    if seed is not None:
+      gen.manual_seed(int(seed))
+    else:
+      gen.seed()
+    white = (torch.randn(shape, generator=gen, dtype=torch.float64) *
+             M**0.5).to(torch.float32).to(device)
+  else:
+    raise ValueError("rng must be 'philox' or 'torch'")
   if type == "real":
     return white
   if type == "complex":

@@ -162,6 +162,16 @@ int pmx_fourier_apply(const pmx_plan* plan, int nin, const float* const* in,
                       float in_scale, float* work, float* out, int accumulate,
                       void* stream);
 
+/* Seeded white noise in HBM (replaces TF's RNG in utils.py:337-355):
+ * out[i] = float32(scale * z_i), z_i the i-th standard normal of the
+ * Philox4x64-10 stream with key (key_lo, key_hi) -- the generator of
+ * numpy.random.Philox: normals 4q..4q+3 come from block q, i.e. the words
+ * philox4x64_10(counter + q + 1, key), by two Box-Muller pairs
+ * (u1 = ((w >> 11) + 1) 2^-53, u2 = (w' >> 11) 2^-53, float64). */
+int pmx_white_noise(float* out, long long n, unsigned long long key_lo,
+                    unsigned long long key_hi, unsigned long long counter,
+                    double scale, void* stream);
+
 /* Power-spectrum bins (flowpm/pk.py:7-103): psum[b][i] = sum over the
  * modes of batch member b with numpy.digitize(|k|, edges) == i of
  * W |F|^2, where F is the unnormalised transform held in spec (the half

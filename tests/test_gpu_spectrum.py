@@ -74,3 +74,23 @@ def test_compensate_cic_adjoint(gpu):
   ref = fa.compensate_cic(y)
   e = float((g - ref).norm() / ref.norm())
   assert e < 1e-6, e
+
+
+@pytest.mark.parametrize("shape,seed", [((1, 16, 16, 16), 0),
+                                        ((2, 32, 24, 16), 12345),
+                                        ((1, 8, 8, 10), (1 << 70) + 11)])
+def test_white_noise_philox_against_oracle(gpu, shape, seed):
+  """The on-device stream equals numpy.random.Philox's words through the
+  same Box-Muller transform (float64 transcendentals may differ in the last
+  bit between the GPU and libm: compare to 2 float32 ulps)."""
+  from oracle import noise
+  M = shape[1] * shape[2] * shape[3]
+  w = fa.white_noise(shape[1:], batch_size=shape[0], seed=seed, type="real")
+  ref = noise.white_noise(shape, seed, float(M)**0.5)
+  w = w.cpu().numpy()
+  ulp = np.spacing(np.abs(ref).astype(np.float32))
+  assert (np.abs(w - ref) <= 2 * ulp).all()
+  # a different seed is a different field
+  w2 = fa.white_noise(shape[1:], batch_size=shape[0], seed=seed + 1,
+                      type="real").cpu().numpy()
+  assert np.abs(w2 - w).max() > 1.0
