@@ -354,6 +354,37 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   }
 }
 
+// PGD displacement (tfpm.py:404-421): X += c * readout(m, X * f), with
+// c = alpha / pm_nc_factor and m the three PGD meshes.
+template <bool I4>
+__global__ __launch_bounds__(256) void k_pgd_shift(float* __restrict__ X,
+                                                   Mesh3 m, MeshDims d,
+                                                   float f, float c) {
+  const long long tot = (long long)d.B * d.N;
+  const long long p =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
+  if (p >= tot) return;
+  const int b = d.B == 1 ? 0 : (int)(p / d.N);
+  float xv[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) xv[k] = X[3 * p + k];
+  const float x = xv[0] * f, y = xv[1] * f, z = xv[2] * f;
+  if (!pos_ok(x, y, z)) return;
+  const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+  float3 v[8];
+  PMX_CORNERS({ v[cc] = load3<I4>(m, cell); })
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  PMX_CORNERS({
+    const float k = (wx * wy) * wz;
+    ax += v[cc].x * k;
+    ay += v[cc].y * k;
+    az += v[cc].z * k;
+  })
+  X[3 * p] = xv[0] + c * ax;
+  X[3 * p + 1] = xv[1] + c * ay;
+  X[3 * p + 2] = xv[2] + c * az;
+}
+
 // Kick / drift from the stored force (no readout).
 __global__ __launch_bounds__(256) void k_kick_drift_only(
     float* __restrict__ X, float* __restrict__ P, const float* __restrict__ F,
@@ -614,6 +645,21 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
   else
     hipLaunchKernelGGL(k_readout_kick_drift<false>, dim3(nblk(tot, 256)),
                        dim3(256), 0, s, X, P, F, m, d, cc);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_pgd_shift(float* X, const Mesh3& m, const MeshDims& d, float f,
+                     float c, hipStream_t s) {
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  ProfScope prof_(kProfReadout, s);
+  if (m.f4)
+    hipLaunchKernelGGL(k_pgd_shift<true>, dim3(nblk(tot, 256)), dim3(256), 0,
+                       s, X, m, d, f, c);
+  else
+    hipLaunchKernelGGL(k_pgd_shift<false>, dim3(nblk(tot, 256)), dim3(256), 0,
+                       s, X, m, d, f, c);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

@@ -176,18 +176,32 @@ static int r2c_zy(const Plan& p, const float* const* in, float2* const* spec,
 // w0, w1, w2: spec buffers (w0 may alias spec when fwd = 1).
 // f4 (optional): write the three meshes interleaved as (x, y, z, -) float4
 // instead of to fx / fy / fz.
+// pgd (optional): the PGD kernel exp(-kl^2/k^2) exp(-k^4/ks^4) multiplies
+// the Laplacian (apply_pgd, tfpm.py:222-270) instead of the long-range 1.
+struct PgdTerm {
+  float kl2, ks4;
+};
+static inline Term with_pgd(Term t, const PgdTerm* pgd) {
+  if (pgd) {
+    t.pgd = 1;
+    t.pgd_kl2 = pgd->kl2;
+    t.pgd_ks4 = pgd->ks4;
+  }
+  return t;
+}
+
 static int poisson_grad(const Plan& p, const float2* spec, int fwd,
                         float2* w0, float2* w1, float2* w2, float* fx,
                         float* fy, float* fz, float scale, hipStream_t s,
-                        float* f4 = nullptr) {
+                        float* f4 = nullptr, const PgdTerm* pgd = nullptr) {
   XJob xj{};
   xj.nin = 1;
   xj.nout = 2;
   xj.in[0] = spec;
   xj.out[0] = w0;  // lap
   xj.out[1] = w1;  // lap * i gx
-  xj.term[0][0] = term(scale, 1, 0, 0, 0);
-  xj.term[0][1] = term(scale, 1, 1, 0, 0);
+  xj.term[0][0] = with_pgd(term(scale, 1, 0, 0, 0), pgd);
+  xj.term[0][1] = with_pgd(term(scale, 1, 1, 0, 0), pgd);
   xj.do_fwd = fwd;
   xj.do_inv = 1;
   PMX_TRY(launch_xpass(p, xj, s));
@@ -223,12 +237,12 @@ static int poisson_grad(const Plan& p, const float2* spec, int fwd,
 
 static int force_meshes(const Plan& p, const float* rho, float2* w, float* fx,
                         float* fy, float* fz, float rho_scale, hipStream_t s,
-                        float* f4 = nullptr) {
+                        float* f4 = nullptr, const PgdTerm* pgd = nullptr) {
   float2* w0 = w;
   float2* w1 = w + p.d.S;
   float2* w2 = w + 2 * p.d.S;
   PMX_TRY(r2c_zy(p, &rho, &w0, 1, rho_scale, s));
-  return poisson_grad(p, w0, 1, w0, w1, w2, fx, fy, fz, 1.0f, s, f4);
+  return poisson_grad(p, w0, 1, w0, w1, w2, fx, fy, fz, 1.0f, s, f4, pgd);
 }
 
 // Adjoint of the map (g_x, g_y, g_z) <- force_meshes(rho):
@@ -541,9 +555,11 @@ static int paint_meshes(const Plan& p, float* const* mesh, int C,
   return kUnsupported;  // C = 3 only arises on the grid path
 }
 
-// force meshes at X (positions * f), density scaled by 1/nbar and 1/M.
+// force meshes at X (positions * f), density scaled by 1/nbar and 1/M
+// (with pgd: the PGD meshes of the same density).
 static int force_at(const Plan& p, const float* X, long long N, float f,
-                    const NbodyWork& nw, hipStream_t s) {
+                    const NbodyWork& nw, hipStream_t s,
+                    const PgdTerm* pgd = nullptr) {
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   (void)M;
@@ -555,7 +571,7 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
   return force_meshes(p, nw.rho, nw.w, nullptr, nullptr, nullptr, sc, s,
-                      nw.f4);
+                      nw.f4, pgd);
 }
 
 // Where a run leaves what the reverse sweep needs of each force evaluation
@@ -570,10 +586,13 @@ struct Tape {
   int first = 0;
 };
 
+// pgd (optional): (alpha, kl, ks) of every step; after the drift of step
+// i the positions move by the PGD displacement (tfpm.py:487-488).
 static int nbody_run(const Plan& p, float* state, long long N, float f,
                      float fac, const double* coef, int b0, int b1,
                      int jump, float* work, hipStream_t s,
-                     const Tape* tp = nullptr) {
+                     const Tape* tp = nullptr, const float* pgd = nullptr) {
+  PMX_REQUIRE(!(pgd && tp), "nbody: the PGD correction has no adjoint yet");
   const long long BN3 = (long long)p.d.B * N * 3;
   const long long M = p.d.rows * p.d.nz;
   const Tape tape = tp ? *tp : Tape{};
@@ -628,6 +647,13 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   }
   for (int i = b0; i < b1; ++i) {
     const long long slot = i - tape.first;
+    if (pgd) {
+      const float alpha = pgd[3 * i], kl = pgd[3 * i + 1],
+                  ks = pgd[3 * i + 2];
+      const PgdTerm pt{kl * kl, (float)std::pow((double)ks, 4.0)};
+      PMX_TRY(force_at(p, X, N, f, nw0, s, &pt));
+      PMX_TRY(launch_pgd_shift(X, m3(nw0.f4), md, f, alpha / f, s));
+    }
     if (!x_kept) PMX_TRY(keep_x(xslot(i), X));
     const NbodyWork nw = at(tape.fs ? tape.fs + slot * 4 * M : nullptr);
     PMX_TRY(force_at(p, X, N, f, nw, s));
@@ -1272,6 +1298,22 @@ int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
               "pmx_nbody_run: bad step range");
   return nbody_run(P(plan), state, N, f, fac, coef, b0, b1, jump, work,
                    S(stream));
+}
+
+int pmx_nbody_run_pgd(const pmx_plan* plan, float* state, long long N,
+                      float f, float fac, const double* coef,
+                      const float* pgd, int b0, int b1, int jump, float* work,
+                      void* stream) {
+  PMX_REQUIRE(plan && state && work && pgd,
+              "pmx_nbody_run_pgd: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_nbody_run_pgd: need at least one particle");
+  PMX_REQUIRE(b0 >= 0 && b1 >= b0 && (coef || b1 == b0),
+              "pmx_nbody_run_pgd: bad step range");
+  for (int i = b0; i < b1; ++i)
+    PMX_REQUIRE(pgd[3 * i + 1] != 0.0f && pgd[3 * i + 2] != 0.0f,
+                "pmx_nbody_run_pgd: kl and ks must be nonzero");
+  return nbody_run(P(plan), state, N, f, fac, coef, b0, b1, jump, work,
+                   S(stream), nullptr, pgd);
 }
 
 long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N,

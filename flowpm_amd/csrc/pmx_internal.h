@@ -242,6 +242,10 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
                               const MeshDims& d, const KickDrift& c,
                               hipStream_t s);
 
+// X += c * readout(m, X * f) (the PGD displacement)
+int launch_pgd_shift(float* X, const Mesh3& m, const MeshDims& d, float f,
+                     float c, hipStream_t s);
+
 int launch_kick_drift_only(float* X, float* P, const float* F,
                            const MeshDims& d, const KickDrift& c,
                            hipStream_t s);

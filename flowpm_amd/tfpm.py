@@ -405,6 +405,47 @@ class _Nbody(torch.autograd.Function):
     return gs, None, None, None, None, None, None, None
 
 
+def _pgd_table(pgdparams, nsteps):
+  """(alpha, kl, ks) per step as a float32 C array (pgdparams: tfpm.py:451,
+  one row per step)."""
+  if isinstance(pgdparams, torch.Tensor):
+    pgdparams = pgdparams.detach().cpu().numpy()
+  arr = np.asarray(pgdparams, dtype=np.float32).reshape(-1, 3)
+  if len(arr) != nsteps:
+    raise ValueError("pgdparams needs one (alpha, kl, ks) row per step: "
+                     "%d rows for %d steps" % (len(arr), nsteps))
+  if (arr[:, 1] == 0).any() or (arr[:, 2] == 0).any():
+    raise ValueError("PGD scales kl and ks must be nonzero")
+  return (_lib._f * arr.size)(*arr.ravel().tolist())
+
+
+def _run(plan, st, N, f, fac, cptr, pgd, b0, b1, jump, work):
+  if pgd is None:
+    _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
+              b0, b1, jump, _lib.ptr(work), _lib.stream_of(st))
+  else:
+    _lib.call("pmx_nbody_run_pgd", plan.handle, _lib.ptr(st), N, f, fac,
+              cptr, pgd, b0, b1, jump, _lib.ptr(work), _lib.stream_of(st))
+
+
+class _NbodyPgd(torch.autograd.Function):
+  """nbody with the PGD correction, differentiable (placeholder until the
+  PGD adjoint kernels exist: the forward runs, the backward raises)."""
+
+  @staticmethod
+  def forward(ctx, state, pgdparams, plan, N, f, fac, coef):
+    st = _f32(state).clone()
+    nsteps = len(coef) // 3
+    _run(plan, st, N, f, fac, _lib.dbl(coef), _pgd_table(pgdparams, nsteps),
+         0, nsteps, 1, _work(plan))
+    return st
+
+  @staticmethod
+  def backward(ctx, g):
+    raise NotImplementedError("gradients through the PGD correction are not "
+                              "built yet")
+
+
 def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
           return_intermediate_states=False, pgdparams=None, name="NBody",
           checkpoint_every=None):
@@ -415,24 +456,32 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
   checkpoint; with 0 the forward pass records the positions and force
   meshes of every step and nothing is recomputed; None uses the setting of
   ``set_checkpoint_every`` (automatic by default)."""
-  if pgdparams is not None:
-    raise NotImplementedError("PGD correction (tfpm.py:369-422) is not built")
   _lib.require_cuda(state)
   if len(stages) == 0:
     return state
   plan, B, N, f, fac = _force_setup(cosmo, state, nc, pm_nc_factor)
   coef = step_coefficients(cosmo, stages)
+  nsteps = len(stages) - 1
+  pgd = None
+  if pgdparams is not None:
+    pgd = _pgd_table(pgdparams, nsteps)
+    if torch.is_grad_enabled() and (
+        state.requires_grad or getattr(pgdparams, "requires_grad", False)):
+      return _NbodyPgd.apply(state, pgdparams, plan, N, f, fac, coef)
   if return_intermediate_states:
     st = _f32(state).clone()
     work = _work(plan)
     cptr = _lib.dbl(coef)
     out = []
-    for i in range(len(stages) - 1):
-      _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
-                i, i + 1, 1 if i == 0 else 0, _lib.ptr(work),
-                _lib.stream_of(st))
+    for i in range(nsteps):
+      _run(plan, st, N, f, fac, cptr, pgd, i, i + 1, 1 if i == 0 else 0,
+           work)
       out.append((stages[i + 1], st.clone()))
     return out
+  if pgd is not None:
+    st = _f32(state).clone()
+    _run(plan, st, N, f, fac, _lib.dbl(coef), pgd, 0, nsteps, 1, _work(plan))
+    return st
   need_grad = torch.is_grad_enabled() and state.requires_grad
   every = checkpoint_every if checkpoint_every is not None else _CKPT_EVERY[0]
   every = _resolve_every(plan, N, len(coef) // 3, every,

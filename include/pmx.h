@@ -214,6 +214,16 @@ int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
                   float fac, const double* coef, int step_begin, int step_end,
                   int jump_start, float* work, void* stream);
 
+/* pmx_nbody_run with the PGD correction (tfpm.py:369-422, :487-488): after
+ * the drift of step i, X += alpha_i / f * readout(c2r(delta_k lap
+ * PGD_kernel(kl_i, ks_i) grad_d), f X) with delta_k the density of the
+ * drifted positions (kernels.py:105-134).  pgd[3*i + {0,1,2}] =
+ * (alpha, kl, ks) of step i (float32, as pgdparams). */
+int pmx_nbody_run_pgd(const pmx_plan* plan, float* state, long long N,
+                      float f, float fac, const double* coef,
+                      const float* pgd, int step_begin, int step_end,
+                      int jump_start, float* work, void* stream);
+
 /* Reverse sweep (hand-written adjoint with recompute).  ckpt holds the
  * states at the start of every segment of `every` steps (segment s starts
  * at step s*every; segment 0 at step 0 *before* the jump-start force);

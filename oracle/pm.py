@@ -28,6 +28,8 @@ Function map (reference file:line):
   apply_longrange / kick / drift / force / nbody
                                                 tfpm.py:183-219 / 273-292 /
                                                 295-314 / 317-366 / 425-503
+  pgd_kernel / apply_pgd / pgd                  kernels.py:105-134 /
+                                                tfpm.py:222-270 / 369-422
 Pins: tests/test_oracle_pm.py (numpy.fft, brute-force DFT, brute-force CIC
 loops, paint/readout adjoint identity, mass and momentum conservation,
 plane-wave closed forms for lpt1 and lpt2_source, linear growth of a single
@@ -251,6 +253,50 @@ def apply_longrange(x, delta_k, split=0, factor=1.):
   return torch.stack(f, dim=2) * factor
 
 
+def pgd_kernel(kvec, kl, ks, dtype):
+  """kernels.py:105-134: exp(-kl^2/kk) exp(-kk^2/ks^4) with the zero mode's
+  kk set to 1 and kept.  kl, ks may be torch scalars (differentiable)."""
+  kk = sum(ki**2 for ki in kvec)
+  kk = np.where(kk == 0, 1.0, kk)
+  kk_t = torch.as_tensor(kk, dtype=dtype)
+  kk2_t = torch.as_tensor(kk**2, dtype=dtype)
+  kl = torch.as_tensor(kl, dtype=dtype)
+  ks = torch.as_tensor(ks, dtype=dtype)
+  return torch.exp(-kl**2 / kk_t) * torch.exp(-kk2_t / ks**4)
+
+
+def apply_pgd(x, delta_k, alpha, kl, ks):
+  """tfpm.py:222-270: alpha * readout(c2r(delta_k lap PGD grad_d), x)."""
+  nc = list(delta_k.shape[1:])
+  cdt = delta_k.dtype
+  rdt = torch.float64 if cdt == torch.complex128 else torch.float32
+  kvec = fftk(nc, symmetric=False)
+  lap = _t(laplace_kernel(kvec), cdt)
+  kweight = lap * pgd_kernel(kvec, kl, ks, rdt).to(cdt)
+  pot_k = delta_k * kweight
+  f = []
+  for d in range(3):
+    forced = c2r3d(pot_k * _t(gradient_kernel(kvec, d), cdt),
+                   norm=nc[0] * nc[1] * nc[2])
+    f.append(cic_readout(forced, x))
+  return torch.stack(f, dim=2) * alpha
+
+
+def pgd(state, pgdparams, nc, pm_nc_factor=1):
+  """tfpm.py:369-422: X += apply_pgd(f X, delta_k(X), alpha, kl, ks) / f."""
+  B = state.shape[1]
+  ncf = [n * pm_nc_factor for n in nc]
+  dt = state.dtype
+  rho = torch.zeros([B] + ncf, dtype=dt)
+  nbar = nc[0] * nc[1] * nc[2] / (ncf[0] * ncf[1] * ncf[2])
+  xs = state[0] * pm_nc_factor
+  rho = cic_paint(rho, xs) * (1. / nbar)
+  delta_k = r2c3d(rho, norm=ncf[0] * ncf[1] * ncf[2])
+  alpha, kl, ks = pgdparams[0], pgdparams[1], pgdparams[2]
+  upd = apply_pgd(xs, delta_k, alpha, kl, ks) / pm_nc_factor
+  return torch.stack((state[0] + upd, state[1], state[2]))
+
+
 def kick(growth, state, ai, ac, af):
   g = growth
   fac = float(1 / (ac**2 * g.E(ac)) * (g.Gf(af) - g.Gf(ai)) / g.gf(ac))
@@ -281,7 +327,7 @@ def force(growth, state, nc, pm_nc_factor=1):
 
 
 def nbody(growth, state, stages, nc, pm_nc_factor=1,
-          return_intermediate_states=False):
+          return_intermediate_states=False, pgdparams=None):
   if isinstance(nc, int):
     nc = [nc, nc, nc]
   if len(stages) == 0:
@@ -296,6 +342,8 @@ def nbody(growth, state, stages, nc, pm_nc_factor=1,
     state = kick(growth, state, p, f, ah)
     p = ah
     state = drift(growth, state, x, p, a1)
+    if pgdparams is not None:
+      state = pgd(state, pgdparams[i], nc, pm_nc_factor)
     x = a1
     state = force(growth, state, nc, pm_nc_factor)
     f = a1

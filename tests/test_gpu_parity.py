@@ -319,3 +319,31 @@ def test_taped_adjoint_matches_recompute(gpu):
     out = fa.nbody(c, x0, stages, n, checkpoint_every=every)
     (grads[every],) = torch.autograd.grad(out, x0, grad_outputs=gseed)
   close(grads[0], grads[3], 1e-6)
+
+
+@pytest.mark.parametrize("f", [1, 2])
+def test_nbody_pgd_against_oracle(gpu, f):
+  """nbody with the PGD correction (tfpm.py:369-422, :487-488): final
+  state at 32^3 (f = 1) and 16^3 particles on a 32^3 mesh (f = 2)."""
+  n = 32 // f
+  L = 100.0
+  white, pk = _ics(n, L, 40 + f)
+  g = obg.GrowthTable(obg.planck15())
+  stages = np.linspace(0.1, 1.0, 6)
+  pgdp = np.array([[0.3 + 0.05 * i, 0.6, 3.0 + 0.2 * i] for i in range(5)],
+                  dtype=np.float32)
+  lin = opm.whitenoise_to_linear(white, L, pk)
+  st_o = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=1), stages, [n] * 3,
+                   pm_nc_factor=f, pgdparams=pgdp.astype(np.float64))
+  c = fa.cosmology.Planck15()
+  st_g = fa.nbody(c, fa.lpt_init(c, lin.float().to(gpu), 0.1, order=1),
+                  stages, n, pm_nc_factor=f, pgdparams=pgdp)
+  q = opm.lagrangian_grid([n] * 3, 1, torch.float64)
+  ex = rel_rms(st_g[0] - q.float().to(gpu), st_o[0] - q)
+  ep = rel_rms(st_g[1], st_o[1])
+  st_n = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=1), stages, [n] * 3,
+                   pm_nc_factor=f)
+  moved = rel_rms(st_o[0] - st_n[0], st_o[0] - q)
+  print("PGD f=%d: dX %.3g P %.3g (PGD moved X by %.3g)" % (f, ex, ep, moved))
+  assert moved > 1e-3  # the correction matters at these parameters
+  assert ex < 1e-4 and ep < 1e-4

@@ -211,3 +211,43 @@ def test_oracle_gradient_vs_finite_differences():
     eps = 1e-4
     fd = (loss(white + eps * v) - loss(white - eps * v)) / (2 * eps)
     np.testing.assert_allclose((gr * v).sum().item(), fd.item(), rtol=2e-5)
+
+
+def test_pgd_plane_wave_closed_form():
+  """apply_pgd on delta = A cos(k.q) read out at grid points: the
+  displacement is alpha * (-A g(k) / k^2) * PGD(k) * sin(k.q), PGD(k) =
+  exp(-kl^2/k^2) exp(-k^4/ks^4) (kernels.py:105-134, tfpm.py:222-270)."""
+  n, A, alpha, kl, ks = 16, 0.4, 0.7, 0.3, 1.7
+  m = np.array([1, -2, 3])
+  q = np.indices([n, n, n]).reshape(3, -1).T
+  kv = 2 * np.pi * m / n
+  delta = A * np.cos(q @ kv).reshape(n, n, n)
+  dk = pm.r2c3d(torch.tensor(delta[None]))
+  pos = pm.lagrangian_grid([n, n, n], 1, torch.float64)
+  s = pm.apply_pgd(pos, dk, alpha, kl, ks)[0].numpy()
+  kw = np.where(kv > np.pi, kv - 2 * np.pi, kv)
+  g = (8 * np.sin(kw) - np.sin(2 * kw)) / 6
+  kk = np.sum(kw**2)
+  w = np.exp(-kl**2 / kk) * np.exp(-kk**2 / ks**4)
+  ref = alpha * -A * g[None, :] / kk * w * np.sin(q @ kv)[:, None]
+  np.testing.assert_allclose(s, ref, atol=1e-12)
+
+
+def test_pgd_limits_and_momentum():
+  """kl -> 0, ks -> inf: the PGD kernel is 1 and apply_pgd equals
+  apply_longrange (same paint, FFT and readout); the displacement of a
+  displaced grid sums to zero over the particles (odd kernel, adjoint CIC)."""
+  n = 16
+  gen = torch.Generator().manual_seed(3)
+  q = pm.lagrangian_grid([n] * 3, 1, torch.float64)
+  x = q + 0.8 * torch.randn(q.shape, generator=gen, dtype=torch.float64)
+  rho = pm.cic_paint(torch.zeros(1, n, n, n, dtype=torch.float64), x)
+  dk = pm.r2c3d(rho, norm=n**3)
+  a = pm.apply_pgd(x, dk, 0.9, 1e-9, 1e9)
+  b = pm.apply_longrange(x, dk, factor=0.9)
+  np.testing.assert_allclose(a.numpy(), b.numpy(), rtol=1e-12, atol=1e-14)
+  st = torch.stack([x, torch.zeros_like(x), torch.zeros_like(x)])
+  out = pm.pgd(st, [0.5, 0.4, 2.0], [n] * 3)
+  shift = (out[0] - x)[0]
+  assert shift.abs().max() > 1e-3
+  assert shift.sum(0).abs().max() < 1e-10 * shift.abs().sum()
