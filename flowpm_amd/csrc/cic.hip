@@ -354,6 +354,44 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   }
 }
 
+// Contraction of a particle cotangent with a readout: sum_p sum_c
+// G[p][c] readout(m_c, X_p f), reduced in float64 (the PGD parameter
+// gradients).
+template <bool I4>
+__global__ __launch_bounds__(256) void k_readout_dot(
+    const float* __restrict__ X, const float* __restrict__ G, Mesh3 m,
+    MeshDims d, float f, double* __restrict__ out, double scale) {
+  __shared__ double red[4];
+  const long long tot = (long long)d.B * d.N;
+  const long long p =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (p < tot) {
+    const int b = d.B == 1 ? 0 : (int)(p / d.N);
+    const float x = X[3 * p] * f, y = X[3 * p + 1] * f, z = X[3 * p + 2] * f;
+    if (pos_ok(x, y, z)) {
+      const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+      float3 v[8];
+      PMX_CORNERS({ v[cc] = load3<I4>(m, cell); })
+      float ax = 0.f, ay = 0.f, az = 0.f;
+      PMX_CORNERS({
+        const float k = (wx * wy) * wz;
+        ax += v[cc].x * k;
+        ay += v[cc].y * k;
+        az += v[cc].z * k;
+      })
+      acc = (double)G[3 * p] * ax + (double)G[3 * p + 1] * ay +
+            (double)G[3 * p + 2] * az;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicAdd(out, scale * ((red[0] + red[1]) + (red[2] + red[3])));
+}
+
 // PGD displacement (tfpm.py:404-421): X += c * readout(m, X * f), with
 // c = alpha / pm_nc_factor and m the three PGD meshes.
 template <bool I4>
@@ -447,9 +485,12 @@ __global__ __launch_bounds__(256) void k_force_vjp_particles(
 //                  3: gF = 0 (the jump-start force replaced F)
 // With the interleaved mesh (I4) rbar is its fourth component and the rbar
 // argument is unused.
+// gin is the readout's cotangent (gF for the force, gX itself for the PGD
+// displacement: gX and gin may alias, so neither is __restrict__; every
+// thread loads its inputs before it stores).
 template <bool I4>
 __global__ __launch_bounds__(256) void k_force_vjp_finish(
-    float* __restrict__ gX, float* __restrict__ gP, float* __restrict__ gF,
+    float* gX, float* __restrict__ gP, float* gF, const float* gin,
     const float* __restrict__ X, Mesh3 m, const float* __restrict__ rbar,
     MeshDims d, float f, float cf, AdjUpdate u) {
   __shared__ float sh[4][768];
@@ -472,7 +513,7 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
     for (int k = 0; k < 3; ++k) {
       const long long o = stage_off(tiled, p0, u.gz, t + 256 * k);
       v[0][k] = __builtin_nontemporal_load(X + o);
-      v[1][k] = __builtin_nontemporal_load(gF + o);
+      v[1][k] = __builtin_nontemporal_load(gin + o);
       v[2][k] = __builtin_nontemporal_load(gX + o);
       v[3][k] = upd ? __builtin_nontemporal_load(gP + o) : 0.0f;
     }
@@ -493,7 +534,7 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       xv[c] = X[3 * p + c];
-      gfv[c] = gF[3 * p + c];
+      gfv[c] = gin[3 * p + c];
       gxv[c] = gX[3 * p + c];
       gpv[c] = upd ? gP[3 * p + c] : 0.0f;
     }
@@ -649,6 +690,22 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
   return kOk;
 }
 
+int launch_readout_dot(const float* X, const float* G, const Mesh3& m,
+                       const MeshDims& d, float f, double* out, double scale,
+                       hipStream_t s) {
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  ProfScope prof_(kProfAdjCic, s);
+  if (m.f4)
+    hipLaunchKernelGGL(k_readout_dot<true>, dim3(nblk(tot, 256)), dim3(256),
+                       0, s, X, G, m, d, f, out, scale);
+  else
+    hipLaunchKernelGGL(k_readout_dot<false>, dim3(nblk(tot, 256)), dim3(256),
+                       0, s, X, G, m, d, f, out, scale);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
 int launch_pgd_shift(float* X, const Mesh3& m, const MeshDims& d, float f,
                      float c, hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
@@ -676,8 +733,8 @@ int launch_kick_drift_only(float* X, float* P, const float* F,
   return kOk;
 }
 
-int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
-                            const Mesh3& m, const float* rbar,
+int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* gin,
+                            const float* X, const Mesh3& m, const float* rbar,
                             const MeshDims& d, float f, float cf,
                             const AdjUpdate& u, hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
@@ -687,10 +744,12 @@ int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
   if (!grid_tiled(u.gx, u.gy, u.gz, d.N)) uu.gx = uu.gy = uu.gz = 0;
   if (m.f4)
     hipLaunchKernelGGL(k_force_vjp_finish<true>, dim3(nblk(tot, 256)),
-                       dim3(256), 0, s, gX, gP, gF, X, m, rbar, d, f, cf, uu);
+                       dim3(256), 0, s, gX, gP, gF, gin ? gin : gF, X, m,
+                       rbar, d, f, cf, uu);
   else
     hipLaunchKernelGGL(k_force_vjp_finish<false>, dim3(nblk(tot, 256)),
-                       dim3(256), 0, s, gX, gP, gF, X, m, rbar, d, f, cf, uu);
+                       dim3(256), 0, s, gX, gP, gF, gin ? gin : gF, X, m,
+                       rbar, d, f, cf, uu);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

@@ -198,9 +198,13 @@ __device__ __forceinline__ float2 apply_term(float2 v, const Term& t, int kx,
     if (t.lap) c *= (float)(1.0 / kk1);
     if (t.pgd) {
       // float32 as TF evaluates it: exp(-kl^2 / kk) exp(-kk^2 / ks^4)
-      const float e1 = expf(-t.pgd_kl2 / (float)kk1);
-      const float e2 = expf(-(float)(kk1 * kk1) / t.pgd_ks4);
-      c *= e1 * e2;
+      const float kkf = (float)kk1, kk2f = (float)(kk1 * kk1);
+      const float e1 = expf(-t.pgd_kl2 / kkf);
+      const float e2 = expf(-kk2f / t.pgd_ks4);
+      float w = e1 * e2;
+      if (t.pgd == 2) w *= -2.0f * t.pgd_kl / kkf;            // d/dkl
+      if (t.pgd == 3) w *= 4.0f * kk2f / (t.pgd_ks4 * t.pgd_ks);  // d/dks
+      c *= w;
     }
   }
   if (t.cic) {

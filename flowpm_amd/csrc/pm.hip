@@ -178,14 +178,18 @@ static int r2c_zy(const Plan& p, const float* const* in, float2* const* spec,
 // instead of to fx / fy / fz.
 // pgd (optional): the PGD kernel exp(-kl^2/k^2) exp(-k^4/ks^4) multiplies
 // the Laplacian (apply_pgd, tfpm.py:222-270) instead of the long-range 1.
+// mode 1: the kernel; 2 / 3: its derivative w.r.t. kl / ks.
 struct PgdTerm {
-  float kl2, ks4;
+  float kl, ks;
+  int mode;
 };
 static inline Term with_pgd(Term t, const PgdTerm* pgd) {
   if (pgd) {
-    t.pgd = 1;
-    t.pgd_kl2 = pgd->kl2;
-    t.pgd_ks4 = pgd->ks4;
+    t.pgd = pgd->mode;
+    t.pgd_kl = pgd->kl;
+    t.pgd_ks = pgd->ks;
+    t.pgd_kl2 = pgd->kl * pgd->kl;
+    t.pgd_ks4 = (float)std::pow((double)pgd->ks, 4.0);
   }
   return t;
 }
@@ -252,7 +256,8 @@ static int force_meshes(const Plan& p, const float* rho, float2* w, float* fx,
 static int force_meshes_adj(const Plan& p, const float* gx, const float* gy,
                             const float* gz, float2* w, float* grho,
                             float rho_scale, hipStream_t s,
-                            float* f4w = nullptr) {
+                            float* f4w = nullptr,
+                            const PgdTerm* pgd = nullptr) {
   float2* ws[3] = {w, w + p.d.S, w + 2 * p.d.S};
   const float* in[3] = {gx, gy, gz};
   PMX_TRY(r2c_zy(p, in, ws, 3, rho_scale, s));
@@ -261,7 +266,7 @@ static int force_meshes_adj(const Plan& p, const float* gx, const float* gy,
   xj.nout = 1;
   for (int d = 0; d < 3; ++d) {
     xj.in[d] = ws[d];
-    xj.term[d][0] = term(-1.0f, 1, d == 0, d == 1, d == 2);
+    xj.term[d][0] = with_pgd(term(-1.0f, 1, d == 0, d == 1, d == 2), pgd);
   }
   xj.out[0] = ws[0];
   xj.do_fwd = 1;
@@ -583,6 +588,10 @@ struct Tape {
   float* fj = nullptr;
   float* xs = nullptr;
   float* fs = nullptr;
+  // with the PGD correction: the drifted positions before the correction
+  // and the PGD meshes at them, per step (same slot numbering)
+  float* xp = nullptr;
+  float* fp = nullptr;
   int first = 0;
 };
 
@@ -592,7 +601,9 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
                      float fac, const double* coef, int b0, int b1,
                      int jump, float* work, hipStream_t s,
                      const Tape* tp = nullptr, const float* pgd = nullptr) {
-  PMX_REQUIRE(!(pgd && tp), "nbody: the PGD correction has no adjoint yet");
+  PMX_REQUIRE(!(pgd && tp && (!tp->xp || !tp->fp)),
+              "nbody: a PGD run keeps its pre-correction positions and PGD "
+              "meshes in the tape");
   const long long BN3 = (long long)p.d.B * N * 3;
   const long long M = p.d.rows * p.d.nz;
   const Tape tape = tp ? *tp : Tape{};
@@ -618,9 +629,11 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   kd.fac = fac;
   kd.f = f;
   particle_grid(p, N, f, &kd.gx, &kd.gy, &kd.gz);
-  // X slot of step i in the tape (null when not kept)
+  // slot of step i in the tape of the positions the drift produces (the
+  // pre-correction ones with PGD), null when not kept
   auto xslot = [&](int i) -> float* {
-    return tape.xs ? tape.xs + (long long)(i - tape.first) * BN3 : nullptr;
+    float* base = pgd ? tape.xp : tape.xs;
+    return base ? base + (long long)(i - tape.first) * BN3 : nullptr;
   };
   bool x_kept = false;  // X of the coming force is already in its slot
   if (jump) {
@@ -648,13 +661,18 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   for (int i = b0; i < b1; ++i) {
     const long long slot = i - tape.first;
     if (pgd) {
+      if (!x_kept) PMX_TRY(keep_x(xslot(i), X));
       const float alpha = pgd[3 * i], kl = pgd[3 * i + 1],
                   ks = pgd[3 * i + 2];
-      const PgdTerm pt{kl * kl, (float)std::pow((double)ks, 4.0)};
-      PMX_TRY(force_at(p, X, N, f, nw0, s, &pt));
-      PMX_TRY(launch_pgd_shift(X, m3(nw0.f4), md, f, alpha / f, s));
+      const PgdTerm pt{kl, ks, 1};
+      const NbodyWork nwp = at(tape.fp ? tape.fp + slot * 4 * M : nullptr);
+      PMX_TRY(force_at(p, X, N, f, nwp, s, &pt));
+      PMX_TRY(launch_pgd_shift(X, m3(nwp.f4), md, f, alpha / f, s));
+      // the corrected positions are the force's
+      PMX_TRY(keep_x(tape.xs ? tape.xs + slot * BN3 : nullptr, X));
+    } else if (!x_kept) {
+      PMX_TRY(keep_x(xslot(i), X));
     }
-    if (!x_kept) PMX_TRY(keep_x(xslot(i), X));
     const NbodyWork nw = at(tape.fs ? tape.fs + slot * 4 * M : nullptr);
     PMX_TRY(force_at(p, X, N, f, nw, s));
     kd.do_k2 = 1;
@@ -678,20 +696,25 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
 
 // X-cotangent contribution of F = force(X) for cotangent gF, followed by
 // the KDK cotangent update u.  fm: the force meshes at X from the forward
-// pass (recomputed when null).
+// pass (recomputed when null).  With pgd the same for the PGD displacement
+// S = (alpha / f) readout(PGD meshes): fac is alpha, gin (the cotangent of
+// S) is gX itself, and gF is only the target of the update u.
 static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
                      float* gP, long long N, float f, float fac,
                      const NbodyWork& nw, float* gm, float* fm,
-                     const AdjUpdate& u, hipStream_t s) {
+                     const AdjUpdate& u, hipStream_t s,
+                     const PgdTerm* pgd = nullptr,
+                     const float* gin = nullptr) {
+  const float* gc = gin ? gin : gF;
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
   MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
   float* f4 = fm ? fm : nw.f4;
   const Mesh3 m = m3(f4);
-  if (!fm) PMX_TRY(force_at(p, X, N, f, nw, s));
+  if (!fm) PMX_TRY(force_at(p, X, N, f, nw, s, pgd));
   float* gms[3] = {gm, gm + M, gm + 2 * M};
-  const bool grid = paint_meshes(p, gms, 3, X, gF, 3, fac / f, N, f,
+  const bool grid = paint_meshes(p, gms, 3, X, gc, 3, fac / f, N, f,
                                  nw.brick, s) == kOk;
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
@@ -700,10 +723,11 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
     // the density cotangent goes to the unused fourth component of the
     // force mesh, so the finish kernel gathers one float4 per corner
     PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nullptr, sc,
-                             s, f4));
-    return launch_force_vjp_finish(gX, gP, gF, X, m, nullptr, md, f, fac / f,
-                                   u, s);
+                             s, f4, pgd));
+    return launch_force_vjp_finish(gX, gP, gF, gc, X, m, nullptr, md, f,
+                                   fac / f, u, s);
   }
+  PMX_REQUIRE(!pgd, "PGD adjoint: particles must form the Lagrangian grid");
   for (int c = 0; c < 3; ++c) PMX_TRY(launch_fill(gms[c], M, 0.0f, s));
   PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X, m,
                                      md, f, fac / f, 1, s));
@@ -741,22 +765,64 @@ static long long tape_floats(const Plan& p, long long N, int nsteps) {
   return (long long)(nsteps + 1) * (BN3 + 4 * M);
 }
 
-static Tape tape_of(const Plan& p, float* tape, long long N, int nsteps) {
+static Tape tape_of(const Plan& p, float* tape, long long N, int nsteps,
+                    bool pgd = false) {
   const long long BN3 = 3LL * p.d.B * N;
+  const long long M4 = 4 * p.d.rows * p.d.nz;
   Tape t;
   t.xj = tape;
   t.xs = tape + BN3;
   t.fj = tape + (long long)(nsteps + 1) * BN3;
-  t.fs = t.fj + 4 * p.d.rows * p.d.nz;
+  t.fs = t.fj + M4;
+  if (pgd) {
+    t.xp = tape + (long long)(nsteps + 1) * (BN3 + M4);
+    t.fp = t.xp + (long long)nsteps * BN3;
+  }
   t.first = 0;
   return t;
 }
 
+// PGD run: the tape also keeps, per step, the pre-correction positions and
+// the PGD meshes
+static long long pgd_tape_floats(const Plan& p, long long N, int nsteps) {
+  const long long M = p.d.rows * p.d.nz;
+  const long long BN3 = 3LL * p.d.B * N;
+  return tape_floats(p, N, nsteps) + (long long)nsteps * (BN3 + 4 * M);
+}
+
+__global__ void k_zero_dbl(double* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+
 // Reverse sweep over steps [b0, b1) whose X and force meshes are in tp.
+// Gradients of <G, S> w.r.t. the PGD parameters of one step, S = alpha / f
+// readout(PGD meshes at Xpre): out[0] += <G, S> / alpha, out[1] / out[2] +=
+// alpha / f <G, readout(meshes of d PGD / d kl or d ks)>.  fpm holds the
+// PGD meshes at Xpre (x, y, z components are read).
+static int pgd_param_grads(const Plan& p, const float* Xpre, const float* G,
+                           const float* fpm, float alpha, float kl, float ks,
+                           long long N, float f, const NbodyWork& nw,
+                           double* out, hipStream_t s) {
+  MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
+  PMX_TRY(launch_readout_dot(Xpre, G, m3(fpm), md, f, out, 1.0 / f, s));
+  for (int mode = 2; mode <= 3; ++mode) {
+    const PgdTerm pt{kl, ks, mode};
+    PMX_TRY(force_at(p, Xpre, N, f, nw, s, &pt));
+    PMX_TRY(launch_readout_dot(Xpre, G, m3(nw.f4), md, f, out + mode - 1,
+                               (double)alpha / f, s));
+  }
+  return kOk;
+}
+
+// pgd / gpgd (optional): the PGD parameters of every step and where their
+// gradients are accumulated (3 float64 per step).
 static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
                          long long N, float f, float fac, const double* coef,
                          float* gstate, const NbodyWork& nw, float* gm,
-                         hipStream_t s) {
+                         hipStream_t s, const float* pgd = nullptr,
+                         double* gpgd = nullptr) {
   const long long M = p.d.rows * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
   float* gX = gstate;
@@ -770,8 +836,27 @@ static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
     u.mode = i > 0 ? 2 : 1;
     particle_grid(p, N, f, &u.gx, &u.gy, &u.gz);
     const long long slot = i - tp.first;
+    if (!pgd) {
+      PMX_TRY(force_vjp(p, tp.xs + slot * BN3, gF, gX, gP, N, f, fac, nw, gm,
+                        tp.fs + slot * 4 * M, u, s));
+      continue;
+    }
+    // force at the corrected positions (no update yet), then the PGD
+    // displacement, whose cotangent is the position cotangent itself, then
+    // the drift / kick update
+    AdjUpdate u0 = u;
+    u0.mode = 0;
     PMX_TRY(force_vjp(p, tp.xs + slot * BN3, gF, gX, gP, N, f, fac, nw, gm,
-                      tp.fs + slot * 4 * M, u, s));
+                      tp.fs + slot * 4 * M, u0, s));
+    const float alpha = pgd[3 * i], kl = pgd[3 * i + 1], ks = pgd[3 * i + 2];
+    float* xpre = tp.xp + slot * BN3;
+    float* fpm = tp.fp + slot * 4 * M;
+    if (gpgd)
+      PMX_TRY(pgd_param_grads(p, xpre, gX, fpm, alpha, kl, ks, N, f, nw,
+                              gpgd + 3 * i, s));
+    const PgdTerm pt{kl, ks, 1};
+    PMX_TRY(force_vjp(p, xpre, gF, gX, gP, N, f, alpha, nw, gm, fpm, u, s,
+                      &pt, gX));
   }
   return kOk;
 }
@@ -840,6 +925,29 @@ static int nbody_adjoint_taped(const Plan& p, float* tape, int nsteps,
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
   PMX_TRY(reverse_steps(p, tp, 0, nsteps, N, f, fac, coef, gstate, nw, gm,
                         s));
+  AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
+  particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
+  return force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
+                   fac, nw, gm, tp.fj, u0, s);
+}
+
+static int nbody_adjoint_pgd_taped(const Plan& p, float* tape, int nsteps,
+                                   long long N, float f, float fac,
+                                   const double* coef, const float* pgd,
+                                   float* gstate, double* gpgd, float* work,
+                                   hipStream_t s) {
+  const long long BN3 = (long long)p.d.B * N * 3;
+  NbodyWork nw = nbody_work(p, work);
+  float* gm = work + nbody_work_floats(p);
+  const Tape tp = tape_of(p, tape, N, nsteps, true);
+  if (gpgd && nsteps > 0) {
+    hipLaunchKernelGGL(k_zero_dbl, dim3((3 * nsteps + 255) / 256), dim3(256),
+                       0, s, gpgd, 3 * nsteps);
+    PMX_LAUNCH_CHECK();
+  }
+  PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
+  PMX_TRY(reverse_steps(p, tp, 0, nsteps, N, f, fac, coef, gstate, nw, gm, s,
+                        pgd, gpgd));
   AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
   particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
   return force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
@@ -1365,6 +1473,42 @@ int pmx_nbody_run_taped(const pmx_plan* plan, float* state, long long N,
   const Tape tp = tape_of(P(plan), tape, N, nsteps);
   return nbody_run(P(plan), state, N, f, fac, coef, 0, nsteps, 1, work,
                    S(stream), &tp);
+}
+
+long long pmx_nbody_pgd_tape_floats(const pmx_plan* plan, long long N,
+                                    int nsteps) {
+  return (plan && N >= 1 && nsteps >= 0) ? pgd_tape_floats(P(plan), N, nsteps)
+                                         : -1;
+}
+
+int pmx_nbody_run_pgd_taped(const pmx_plan* plan, float* state, long long N,
+                            float f, float fac, const double* coef,
+                            const float* pgd, int nsteps, float* tape,
+                            float* work, void* stream) {
+  PMX_REQUIRE(plan && state && tape && work && (pgd || nsteps == 0),
+              "pmx_nbody_run_pgd_taped: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_nbody_run_pgd_taped: need at least one particle");
+  PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
+              "pmx_nbody_run_pgd_taped: bad steps");
+  for (int i = 0; i < nsteps; ++i)
+    PMX_REQUIRE(pgd[3 * i + 1] != 0.0f && pgd[3 * i + 2] != 0.0f,
+                "pmx_nbody_run_pgd_taped: kl and ks must be nonzero");
+  const Tape tp = tape_of(P(plan), tape, N, nsteps, true);
+  return nbody_run(P(plan), state, N, f, fac, coef, 0, nsteps, 1, work,
+                   S(stream), &tp, pgd);
+}
+
+int pmx_nbody_adjoint_pgd_taped(const pmx_plan* plan, float* tape,
+                                int nsteps, long long N, float f, float fac,
+                                const double* coef, const float* pgd,
+                                float* gstate, double* gpgd, float* work,
+                                void* stream) {
+  PMX_REQUIRE(plan && tape && gstate && work && (pgd || nsteps == 0),
+              "pmx_nbody_adjoint_pgd_taped: null pointer");
+  PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
+              "pmx_nbody_adjoint_pgd_taped: bad steps");
+  return nbody_adjoint_pgd_taped(P(plan), tape, nsteps, N, f, fac, coef, pgd,
+                                 gstate, gpgd, work, S(stream));
 }
 
 int pmx_nbody_adjoint_taped(const pmx_plan* plan, float* tape, int nsteps,

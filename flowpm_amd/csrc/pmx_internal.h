@@ -141,8 +141,10 @@ struct Term {
   int lap;
   int e[3];
   int cic;                 // x (sinc_x sinc_y sinc_z)^-2 (cic_compensation)
-  int pgd;                 // x PGD_kernel(kl, ks) (kernels.py:105-134)
-  float pgd_kl2, pgd_ks4;  // kl^2 and ks^4
+  // x PGD_kernel(kl, ks) (kernels.py:105-134) when 1; x its derivative
+  // w.r.t. kl when 2, w.r.t. ks when 3 (the PGD parameter gradients)
+  int pgd;
+  float pgd_kl, pgd_ks, pgd_kl2, pgd_ks4;  // kl, ks, kl^2, ks^4
 };
 
 constexpr int kMaxJobs = 3;
@@ -242,6 +244,11 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
                               const MeshDims& d, const KickDrift& c,
                               hipStream_t s);
 
+// out += scale * sum_p sum_c G[p][c] readout(m_c, X_p * f) (float64)
+int launch_readout_dot(const float* X, const float* G, const Mesh3& m,
+                       const MeshDims& d, float f, double* out, double scale,
+                       hipStream_t s);
+
 // X += c * readout(m, X * f) (the PGD displacement)
 int launch_pgd_shift(float* X, const Mesh3& m, const MeshDims& d, float f,
                      float c, hipStream_t s);
@@ -285,9 +292,11 @@ struct AdjUpdate {
   int mode;  // 0 none, 1 update, 2 update + k2prev, 3 zero gF
   int gx, gy, gz;  // particle grid (tiled particle order) or 0
 };
-// m.f4 set: the density cotangent is m.f4[cell].w and rbar is unused
-int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* X,
-                            const Mesh3& m, const float* rbar,
+// m.f4 set: the density cotangent is m.f4[cell].w and rbar is unused.
+// gin: the cotangent of the readout (gF for the force; the position
+// cotangent itself for the PGD displacement); gF is the update target.
+int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* gin,
+                            const float* X, const Mesh3& m, const float* rbar,
                             const MeshDims& d, float f, float cf,
                             const AdjUpdate& u, hipStream_t s);
 

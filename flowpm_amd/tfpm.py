@@ -429,21 +429,50 @@ def _run(plan, st, N, f, fac, cptr, pgd, b0, b1, jump, work):
 
 
 class _NbodyPgd(torch.autograd.Function):
-  """nbody with the PGD correction, differentiable (placeholder until the
-  PGD adjoint kernels exist: the forward runs, the backward raises)."""
+  """nbody with the PGD correction, differentiable w.r.t. the state and the
+  PGD parameters: the forward run records a tape (positions before and
+  after each correction, PGD and force meshes) and the reverse sweep runs
+  the hand-written adjoint kernels, including d loss / d (alpha, kl, ks)."""
 
   @staticmethod
   def forward(ctx, state, pgdparams, plan, N, f, fac, coef):
     st = _f32(state).clone()
     nsteps = len(coef) // 3
-    _run(plan, st, N, f, fac, _lib.dbl(coef), _pgd_table(pgdparams, nsteps),
-         0, nsteps, 1, _work(plan))
+    pgd = _pgd_table(pgdparams, nsteps)
+    tape = torch.empty(_lib.lib().pmx_nbody_pgd_tape_floats(plan.handle, N,
+                                                            nsteps),
+                       dtype=torch.float32, device=st.device)
+    work = _work(plan)
+    _lib.call("pmx_nbody_run_pgd_taped", plan.handle, _lib.ptr(st), N, f,
+              fac, _lib.dbl(coef), pgd, nsteps, _lib.ptr(tape),
+              _lib.ptr(work), _lib.stream_of(st))
+    ctx.tape, ctx.pgd = tape, pgd
+    ctx.args = (plan, N, f, fac, coef)
+    ctx.pgd_is_tensor = isinstance(pgdparams, torch.Tensor)
+    ctx.pgd_meta = (pgdparams.dtype, pgdparams.device,
+                    tuple(pgdparams.shape)) if ctx.pgd_is_tensor else None
     return st
 
   @staticmethod
   def backward(ctx, g):
-    raise NotImplementedError("gradients through the PGD correction are not "
-                              "built yet")
+    plan, N, f, fac, coef = ctx.args
+    nsteps = len(coef) // 3
+    gs = _f32(g).clone()
+    work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
+                                                            0),
+                       dtype=torch.float32, device=gs.device)
+    want = ctx.pgd_is_tensor and ctx.needs_input_grad[1]
+    gp = torch.empty(max(3 * nsteps, 1), dtype=torch.float64,
+                     device=gs.device) if want else None
+    _lib.call("pmx_nbody_adjoint_pgd_taped", plan.handle, _lib.ptr(ctx.tape),
+              nsteps, N, f, fac, _lib.dbl(coef), ctx.pgd, _lib.ptr(gs),
+              _lib.ptr(gp), _lib.ptr(work), _lib.stream_of(gs))
+    del ctx.tape
+    gpgd = None
+    if want:
+      dt, dev, shape = ctx.pgd_meta
+      gpgd = gp[:3 * nsteps].reshape(shape).to(dtype=dt, device=dev)
+    return gs, gpgd, None, None, None, None, None
 
 
 def nbody(cosmo, state, stages, nc, pm_nc_factor=1,

@@ -269,6 +269,27 @@ int pmx_nbody_adjoint_taped(const pmx_plan* plan, float* tape, int nsteps,
                             const double* coef, float* gstate, float* work,
                             void* stream);
 
+/* Taped run and reverse sweep with the PGD correction (tfpm.py:369-422,
+ * :487-488; pgd as in pmx_nbody_run_pgd).  The tape
+ * (pmx_nbody_pgd_tape_floats floats) also keeps, per step, the positions
+ * before the correction and the PGD meshes at them.  The reverse sweep
+ * returns the state cotangent in gstate (as pmx_nbody_adjoint_taped) and,
+ * if gpgd is not NULL, d loss / d (alpha, kl, ks) of every step in
+ * gpgd[3*i + {0,1,2}] (device float64, overwritten), the gradients
+ * scripts/fit_pgd.py:77-117 fits the parameters with.  work:
+ * pmx_nbody_adj_work_floats(plan, N, 0) floats. */
+long long pmx_nbody_pgd_tape_floats(const pmx_plan* plan, long long N,
+                                    int nsteps);
+int pmx_nbody_run_pgd_taped(const pmx_plan* plan, float* state, long long N,
+                            float f, float fac, const double* coef,
+                            const float* pgd, int nsteps, float* tape,
+                            float* work, void* stream);
+int pmx_nbody_adjoint_pgd_taped(const pmx_plan* plan, float* tape,
+                                int nsteps, long long N, float f, float fac,
+                                const double* coef, const float* pgd,
+                                float* gstate, double* gpgd, float* work,
+                                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -347,3 +347,39 @@ def test_nbody_pgd_against_oracle(gpu, f):
   print("PGD f=%d: dX %.3g P %.3g (PGD moved X by %.3g)" % (f, ex, ep, moved))
   assert moved > 1e-3  # the correction matters at these parameters
   assert ex < 1e-4 and ep < 1e-4
+
+
+def test_nbody_pgd_gradients_against_oracle(gpu):
+  """Gradients through nbody with the PGD correction: w.r.t. the white
+  noise (full chain) and w.r.t. every step's (alpha, kl, ks), as
+  scripts/fit_pgd.py:77-117 uses them, against float64 autograd."""
+  n, L = 16, 64.0
+  white, pk = _ics(n, L, 51)
+  tgt = torch.rand(1, n, n, n, dtype=torch.float64,
+                   generator=torch.Generator().manual_seed(52)) * 2
+  g = obg.GrowthTable(obg.planck15())
+  stages = np.linspace(0.1, 1.0, 5)
+  p0 = np.array([[0.4, 0.7, 2.5], [0.5, 0.6, 3.0], [0.3, 0.8, 2.0],
+                 [0.45, 0.65, 2.8]])
+  wo = white.clone().requires_grad_(True)
+  po = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+  lin = opm.whitenoise_to_linear(wo, L, pk)
+  st = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=2), stages, [n] * 3,
+                 pgdparams=po)
+  rho = opm.cic_paint(torch.zeros_like(lin), st[0])
+  ((rho - tgt)**2).sum().backward()
+  c = fa.cosmology.Planck15()
+  wg = white.float().to(gpu).requires_grad_(True)
+  pg = torch.tensor(p0, dtype=torch.float32, device=gpu, requires_grad=True)
+  ling = fa.whitenoise_to_linear(n, L, wg, pk)
+  stg = fa.nbody(c, fa.lpt_init(c, ling, 0.1, order=2), stages, n,
+                 pgdparams=pg)
+  rhog = fa.cic_paint(torch.zeros_like(ling), stg[0])
+  ((rhog - tgt.float().to(gpu))**2).sum().backward()
+  cs = cosine(wg.grad, wo.grad)
+  gp, go = pg.grad.double().cpu(), po.grad
+  ep = float((gp - go).abs().max() / go.abs().max())
+  print("PGD gradients: white cosine %.7f; params max rel err %.3g" % (cs, ep))
+  print("params grad gpu", gp.numpy(), "oracle", go.numpy())
+  assert cs > 0.999
+  assert ep < 2e-2
