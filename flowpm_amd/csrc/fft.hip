@@ -790,9 +790,17 @@ static int zc2r_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   PMX_LAUNCH_CHECK();
   return kOk;
 }
+// Threads per column-tile workgroup: 512 for the large tiles (fewer
+// registers per thread: a 256 x 16 x-pass tile needs 106 VGPRs at 512
+// threads against 169 at 256, which doubles the waves a CU holds).
+template <int N, int TW>
+struct ColT {
+  static constexpr int T = (N * TW >= 4096) ? 512 : 128;
+};
+
 template <int N, int TW>
 static int y_nt(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
-  constexpr int T = (N * TW >= 4096) ? 256 : 128;
+  constexpr int T = ColT<N, TW>::T;
   const PlanDev& p = pl.d;
   const int nkt = (p.H + TW - 1) / TW;
   const int tiles = p.B * p.nx * nkt + p.B * ((p.nx + TW - 1) / TW);
@@ -818,7 +826,7 @@ static int y_nt(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
 }
 template <int N, int TW>
 static int x_nt(const Plan& pl, const XJob& job, hipStream_t s) {
-  constexpr int T = (N * TW >= 4096) ? 256 : 128;
+  constexpr int T = ColT<N, TW>::T;
   const PlanDev& p = pl.d;
   const int nkt = (p.H + TW - 1) / TW;
   const int tiles = p.B * p.ny * nkt + p.B * ((p.ny + TW - 1) / TW);

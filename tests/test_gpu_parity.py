@@ -37,7 +37,8 @@ def cosine(a, b):
 
 @pytest.mark.parametrize("shape", [(1, 16, 16, 16), (2, 32, 32, 32),
                                    (1, 16, 16, 48), (1, 96, 96, 96),
-                                   (1, 24, 48, 12), (1, 128, 64, 256)])
+                                   (1, 24, 48, 12), (1, 128, 64, 256),
+                                   (1, 256, 256, 8), (2, 512, 16, 8)])
 def test_r2c_c2r_against_numpy(gpu, shape):
   rng = np.random.default_rng(sum(shape))
   x = rng.standard_normal(shape)
@@ -142,10 +143,11 @@ def _displaced_state(n, B=1, amp=1.5, seed=3, dtype=torch.float64):
 
 @pytest.mark.parametrize("n,f,B,amp", [(32, 1, 1, 1.5), (16, 2, 2, 1.5),
                                        (64, 1, 1, 1.5), (32, 1, 2, 9.0),
-                                       (24, 1, 1, 40.0)])
+                                       (24, 1, 1, 40.0), (128, 2, 1, 3.0)])
 def test_force_against_oracle(gpu, n, f, B, amp):
   """Small displacements use the LDS brick paint; amp 9 makes some bricks
-  overflow to device atomics; amp 40 wraps particles many boxes away."""
+  overflow to device atomics; amp 40 wraps particles many boxes away;
+  128^3 particles on a 256^3 mesh runs the 512-thread column passes."""
   g = obg.GrowthTable(obg.planck15())
   st = _displaced_state(n, B, amp=amp)
   ref = opm.force(g, st, [n] * 3, pm_nc_factor=f)[2]
