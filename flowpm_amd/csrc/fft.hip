@@ -781,9 +781,11 @@ static int zc2r_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   ProfScope prof_(kProfZc2r, s);
   if (is_f4_triplet(jobs)) {
     constexpr int TW4 = TW >= 16 ? 8 : TW;
+    // measured: 256 threads beat 512 for the 256 x 8 tile of this pass
+    constexpr int T4 = (N * TW4 >= 4096) ? 512 : (N >= 256 ? 256 : 128);
     dim3 grid((unsigned)((pl.d.rows + 2 * TW4 - 1) / (2 * TW4)));
-    hipLaunchKernelGGL((k_zpass_c2r_f4<N, TW4, T>), grid, dim3(T), 0, s, jobs,
-                       pl.d);
+    hipLaunchKernelGGL((k_zpass_c2r_f4<N, TW4, T4>), grid, dim3(T4), 0, s,
+                       jobs, pl.d);
   } else {
     dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)) * jobs.n);
     hipLaunchKernelGGL((k_zpass_c2r<N, TW, T>), grid, dim3(T), 0, s, jobs,
