@@ -276,74 +276,23 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     const float fxs = shs[c][0], fxinv = shs[c][1];
     for (int i = tid; i < vol; i += 256) box[i] = 0;
     __syncthreads();
-    // x-run merging: a thread's particles (t + 256 u) are x-neighbours on
-    // the Lagrangian grid, so consecutive ones mostly deposit into the same
-    // or the next x cell.  Per (y, z) corner the two pending cells
-    // (rx, rx + 1) accumulate in registers and go to LDS when the run moves
-    // on: about 4 atomics per particle instead of 8 on a regular lattice,
-    // none for particles sharing a cell.  The deposits are integers, so the
-    // result is the same as one atomic per deposit.
-    bool run = false;
-    int rx = 0, ry = 0, rz = 0;
-    int p0[4] = {0, 0, 0, 0}, p1[4] = {0, 0, 0, 0};
-    auto flush = [&](int xx, const int* pv) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (pv[q] != 0)
-          atomicAdd(&box[(xx * e1 + ry + (q >> 1)) * e2 + rz + (q & 1)],
-                    pv[q]);
-    };
 #pragma unroll
     for (int u = 0; u < kPPT; ++u) {
       if (!ok[u]) continue;
       const float x = px[u], y = py[u], z = pz[u];
       const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
       const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
-      const float wy[2] = {1.0f - fabsf(y - fly), 1.0f - fabsf(y - (fly + 1.0f))};
-      const float wz[2] = {1.0f - fabsf(z - flz), 1.0f - fabsf(z - (flz + 1.0f))};
+      const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
+      const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
       const int bx = (int)flx - lo0, by = (int)fly - lo1, bz = (int)flz - lo2;
       const float wq = wc[u][c] * fxs;
-      int i0[4], i1[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        // the CIC weight ((wx wy) wz) of corner (dx, q >> 1, q & 1)
-        i0[q] = __float2int_rn(wq * ((w0x * wy[q >> 1]) * wz[q & 1]));
-        i1[q] = __float2int_rn(wq * ((w1x * wy[q >> 1]) * wz[q & 1]));
+      for (int k = 0; k < 8; ++k) {
+        const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
+                         (OZ[k] ? w1z : w0z);
+        const int li = ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
+        atomicAdd(&box[li], __float2int_rn(wq * kw));
       }
-      const bool same = run && by == ry && bz == rz;
-      if (same && bx == rx) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          p0[q] += i0[q];
-          p1[q] += i1[q];
-        }
-      } else if (same && bx == rx + 1) {
-        flush(rx, p0);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          p0[q] = p1[q] + i0[q];
-          p1[q] = i1[q];
-        }
-        rx = bx;
-      } else {
-        if (run) {
-          flush(rx, p0);
-          flush(rx + 1, p1);
-        }
-        rx = bx;
-        ry = by;
-        rz = bz;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          p0[q] = i0[q];
-          p1[q] = i1[q];
-        }
-        run = true;
-      }
-    }
-    if (run) {
-      flush(rx, p0);
-      flush(rx + 1, p1);
     }
     __syncthreads();
     float* dst = a.arena + off + (long long)c * vol;
@@ -526,52 +475,39 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
       for (int q = 0; q < 7; ++q) lst[pos][q] = rec[q];
     }
     __syncthreads();
-    // 2. every thread adds the bricks, in list order, to its own cells.
-    //    Branch-free loads: every load is issued (lanes outside a box read
-    //    its element 0 and drop it) before the first is consumed, so the
-    //    16 x C loads of a brick -- of UNR bricks for C = 1 -- are in flight
-    //    together instead of one round trip each.
+    // 2. every thread adds the bricks, in list order, to its own cells
     if (yz_ok) {
-      constexpr int UNR = C == 1 ? 2 : 1;
-      for (int j0 = 0; j0 < nl; j0 += UNR) {
-        float v[UNR][C][PER];
-        bool in[UNR][PER];
+      for (int j = 0; j < nl; ++j) {
+        const int e1 = lst[j][5], e2 = lst[j][6];
+        int oy = cy - lst[j][2];
+        if (oy < 0) oy += d.ny;
+        int oz = cz - lst[j][3];
+        if (oz < 0) oz += d.nz;
+        if (oy >= e1 || oz >= e2) continue;
+        const int e0 = lst[j][4];
+        const int vol = e0 * e1 * e2;
+        const float* src =
+            a.arena + ((long long)b * g.nbricks + lst[j][0]) *
+                          (long long)(C * kBoxFloats);
+        const int a0 = lst[j][1];
+        // branch-free: every load is issued (out-of-box ones read box
+        // element 0) before the first is consumed, so the 16 x C loads of
+        // one brick are in flight together instead of one round trip each
+        float v[C][PER];
+        bool in[PER];
 #pragma unroll
-        for (int r = 0; r < UNR; ++r) {
-          const int j = j0 + r;
-          const bool have = j < nl;
-          const int jj = have ? j : j0;
-          const int e1 = lst[jj][5], e2 = lst[jj][6];
-          int oy = cy - lst[jj][2];
-          if (oy < 0) oy += d.ny;
-          int oz = cz - lst[jj][3];
-          if (oz < 0) oz += d.nz;
-          const bool yzin = have && oy < e1 && oz < e2;
-          const int e0 = lst[jj][4];
-          const int vol = e0 * e1 * e2;
-          const float* src =
-              a.arena + ((long long)b * g.nbricks + lst[jj][0]) *
-                            (long long)(C * kBoxFloats);
-          const int a0 = lst[jj][1];
+        for (int k = 0; k < PER; ++k) {
+          int ox = t0x + k - a0;
+          if (ox < 0) ox += d.nx;
+          in[k] = k < Tx && ox < e0;
+          const int bi = in[k] ? (ox * e1 + oy) * e2 + oz : 0;
 #pragma unroll
-          for (int k = 0; k < PER; ++k) {
-            int ox = t0x + k - a0;
-            if (ox < 0) ox += d.nx;
-            in[r][k] = yzin && k < Tx && ox < e0;
-            const int bi = in[r][k] ? (ox * e1 + oy) * e2 + oz : 0;
-#pragma unroll
-            for (int c = 0; c < C; ++c) v[r][c][k] = src[c * vol + bi];
-          }
+          for (int c = 0; c < C; ++c) v[c][k] = src[c * vol + bi];
         }
-        // adds in list order (brick j0 before j0 + 1): the sum does not
-        // depend on the unrolling
 #pragma unroll
-        for (int r = 0; r < UNR; ++r)
+        for (int k = 0; k < PER; ++k)
 #pragma unroll
-          for (int k = 0; k < PER; ++k)
-#pragma unroll
-            for (int c = 0; c < C; ++c)
-              acc[c][k] += in[r][k] ? v[r][c][k] : 0.0f;
+          for (int c = 0; c < C; ++c) acc[c][k] += in[k] ? v[c][k] : 0.0f;
       }
     }
     __syncthreads();  // lst is rewritten by the next chunk
