@@ -97,29 +97,47 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
   const long long pb = (long long)b * a.d.N;
   const float ps = a.ps;
 
-  // one pass over the brick's particles: every load issued up front, the
-  // positions and weights stay in registers for the deposit
+  // one pass over the brick's particles: every load issued up front
+  // (branch-free: lanes past the brick's end load its first particle and
+  // drop it; the weight test is one uniform branch around the whole loop),
+  // the positions and weights stay in registers for the deposit
   float px[kPPT], py[kPPT], pz[kPPT], wc[kPPT][C];
   bool ok[kPPT];
+  long long pidx[kPPT];
 #pragma unroll
   for (int u = 0; u < kPPT; ++u) {
     const int t = tid + u * 256;
     ok[u] = t < nloc;
-    px[u] = py[u] = pz[u] = 0.0f;
-#pragma unroll
-    for (int c = 0; c < C; ++c) wc[u][c] = 0.0f;
-    if (ok[u]) {
-      const int lz = t % cz, r = t / cz, ly = r % cy, lx = r / cy;
-      const long long p =
-          pb + ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz + q0z + lz;
-      px[u] = a.X[3 * p];
-      py[u] = a.X[3 * p + 1];
-      pz[u] = a.X[3 * p + 2];
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-        wc[u][c] = a.w ? a.w[p * a.wstride + c] * a.wscale : a.wscale;
-    }
+    const int tt = ok[u] ? t : 0;
+    const int lz = tt % cz, r = tt / cz, ly = r % cy, lx = r / cy;
+    pidx[u] =
+        pb + ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz + q0z + lz;
+    px[u] = a.X[3 * pidx[u]];
+    py[u] = a.X[3 * pidx[u] + 1];
+    pz[u] = a.X[3 * pidx[u] + 2];
   }
+  if (a.w) {
+#pragma unroll
+    for (int u = 0; u < kPPT; ++u)
+#pragma unroll
+      for (int c = 0; c < C; ++c) wc[u][c] = a.w[pidx[u] * a.wstride + c];
+#pragma unroll
+    for (int u = 0; u < kPPT; ++u)
+#pragma unroll
+      for (int c = 0; c < C; ++c) wc[u][c] = wc[u][c] * a.wscale;
+  } else {
+#pragma unroll
+    for (int u = 0; u < kPPT; ++u)
+#pragma unroll
+      for (int c = 0; c < C; ++c) wc[u][c] = a.wscale;
+  }
+#pragma unroll
+  for (int u = 0; u < kPPT; ++u)
+    if (!ok[u]) {
+      px[u] = py[u] = pz[u] = 0.0f;
+#pragma unroll
+      for (int c = 0; c < C; ++c) wc[u][c] = 0.0f;
+    }
   int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
   float sw[C];  // per channel: sum over particles of |weight|
 #pragma unroll
