@@ -319,6 +319,12 @@ def batched_runs(fa, cosmo, dev):
   st = _ic_states(fa, cosmo, 64, 200.0, B, 1000, dev)
   runner = BatchedNbody([dict(cosmo=cosmo, state=st, stages=stages, nc=64)])
   t_b = _time(runner.run, 3)
+  from flowpm_amd import _lib
+  eager = BatchedNbody([dict(cosmo=cosmo, state=st, stages=stages, nc=64)],
+                       graph=False)
+  eager.run()
+  torch.cuda.synchronize()
+  kern = live_profile(_lib, eager.run)
   one = BatchedNbody([dict(cosmo=cosmo, state=st[:, :1].contiguous(),
                            stages=stages, nc=64)], graph=False)
   t_1 = _time(one.run, 5)
@@ -346,6 +352,7 @@ def batched_runs(fa, cosmo, dev):
       "uneven_mix_ms": t_mix,
       "uneven_mix_sims_per_s": 32 / (t_mix * 1e-3),
       "uneven_mix_fwd_grad_ms": t_mix_grad,
+      "sims64_batch_kernels": kern,
       "uneven_mix_fwd_grad_sims_per_s": 32 / (t_mix_grad * 1e-3),
       "note": "10-step nbody per sim, ICs excluded; forward: graph replay; "
               "fwd_grad: forward + taped adjoint per size class",
