@@ -31,12 +31,12 @@ class _Group:
     self.work = torch.empty(_lib.lib().pmx_nbody_work_floats(
         self.plan.handle), dtype=torch.float32, device=state.device)
 
-  def step(self, i, stream):
-    if i >= self.nsteps and not (i == 0 and self.nsteps == 0):
-      return
+  def run_all(self, stream):
+    """Every step in one library call: consecutive steps share the fused
+    readout + closing kick + next opening kick + drift pass."""
     _lib.call("pmx_nbody_run", self.plan.handle, _lib.ptr(self.state), self.N,
-              self.f, self.fac, self.cptr, i, min(i + 1, self.nsteps),
-              1 if i == 0 else 0, _lib.ptr(self.work), stream)
+              self.f, self.fac, self.cptr, 0, self.nsteps, 1,
+              _lib.ptr(self.work), stream)
 
 
 class BatchedNbody:
@@ -63,9 +63,7 @@ class BatchedNbody:
       st.wait_stream(main)
     for g, st in zip(self.groups, self.streams):
       with torch.cuda.stream(st):
-        sp = _lib._vp(st.cuda_stream)
-        for i in range(self.nsteps):
-          g.step(i, sp)
+        g.run_all(_lib._vp(st.cuda_stream))
     for st in self.streams:
       main.wait_stream(st)
 
