@@ -241,13 +241,14 @@ def main():
   print(json.dumps(out))
 
 
-# rocprofv3 counter kernel names of the profiler kinds
+# rocprofv3 counter kernel names (regular expressions) of the profiler
+# kinds, for the kernels the forward step launches
 _KIND_KERNEL = {
-    "readout_kick_drift": "k_readout_kick_drift", "paint": "k_brick_deposit<1>",
-    "paint_gather": "k_brick_gather<1>", "fft_z_r2c": "k_zpass_r2c",
-    "fft_y_fwd": "k_ypass<256, 16, 256, -1", "fft_x_fused": "k_xpass<256, 16",
-    "fft_y_inv": "k_ypass<256, 16, 256, 1, true", "fft_z_c2r": "k_zpass_c2r",
-    "fill": "k_fill",
+    "readout_kick_drift": r"k_readout_kick_drift<", "paint": r"k_brick_deposit<1>",
+    "paint_gather": r"k_brick_gather<1>", "fft_z_r2c": r"k_zpass_r2c<",
+    "fft_y_fwd": r"k_ypass<\d+, 16, \d+, -1, false>",
+    "fft_x_fused": r"k_xpass<\d+, 16, ", "fft_y_inv": r"k_ypass<\d+, 16, \d+, 1, true>",
+    "fft_z_c2r": r"k_zpass_c2r_f4<", "fill": r"k_fill",
 }
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "pm_step_256_hbm_traffic.md")
 
@@ -256,10 +257,12 @@ def hbm_traffic(kind, n):
   """HBM bytes per launch of this kernel kind, from the committed rocprofv3
   FETCH_SIZE / WRITE_SIZE summary of the 256^3 workload (each counter scaled
   by what it reads for a plain 1 GiB copy); None when not measured."""
+  import re
   if n != 256 or kind not in _KIND_KERNEL or not os.path.exists(TRAFFIC_TABLE):
     return None
+  pat = re.compile(_KIND_KERNEL[kind])
   for line in open(TRAFFIC_TABLE):
-    if line.startswith("| `") and _KIND_KERNEL[kind] in line:
+    if line.startswith("| `") and pat.search(line):
       cols = [c.strip() for c in line.strip().strip("|").split("|")]
       return float(cols[-1]) * 1e6
   return None
