@@ -152,6 +152,10 @@ def main():
     if b:
       v["alg_bytes"] = b
       v["GBps"] = b / (v["avg_us"] * 1e-6) / 1e9
+    h = hbm_traffic(name, n)
+    if h:
+      v["hbm_bytes"] = h  # measured (committed counter summary)
+      v["hbm_GBps"] = h / (v["avg_us"] * 1e-6) / 1e9
 
   # forward + adjoint (checkpoint every k, recompute in backward)
   fa_ms = fa8_ms = every_used = None
