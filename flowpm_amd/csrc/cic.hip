@@ -221,14 +221,22 @@ __host__ __device__ __forceinline__ bool grid_tiled(int gx, int gy, int gz,
 }
 
 // first particle of logical workgroup lb (its 4 runs are gz apart)
+// Consecutive workgroups: the G = 4 (or 2, 1) y-neighbour tiles of one
+// 16 x 64 (y, z) column at one x, then the next x: the column's mesh rows
+// are shared by the groups running together (measured at 256^3: 6 % less
+// time than G = 1).
 __device__ __forceinline__ long long grid_tile(long long lb, int gx, int gy,
                                                int gz, long long N) {
   const long long per_b = (long long)gx * (gy / kTileY) * (gz / kTileZ);
-  const long long b = lb / per_b, r = lb % per_b;
+  const long long b = lb / per_b, r0 = lb % per_b;
+  const int nty = gy / kTileY;
+  const int G = nty % 4 == 0 ? 4 : (nty % 2 == 0 ? 2 : 1);
+  const int sub = (int)(r0 % G);
+  const long long r = r0 / G;
   const int x = (int)(r % gx);
   const long long tile = r / gx;
   const int ntz = gz / kTileZ;
-  const int ty = (int)(tile / ntz), tz = (int)(tile % ntz);
+  const int ty = (int)(tile / ntz) * G + sub, tz = (int)(tile % ntz);
   return b * N + ((long long)x * gy + ty * kTileY) * gz + tz * kTileZ;
 }
 
