@@ -19,9 +19,8 @@
 // phase 2 (k_brick_gather): one workgroup per 16^3 mesh tile evaluates the
 //   candidate bricks in parallel, compacts those whose boxes meet the tile
 //   (in candidate order), and every thread adds them, in that order, to the
-//   16 cells it owns -- a z-run, read from each box row as four 16-byte
-//   vectors; the tile goes out through LDS with plain, z-contiguous stores,
-//   no barrier per brick.
+//   16 cells it owns; the tile is written with plain stores, no barrier per
+//   brick.
 //
 // Device-scope float atomics cannot complete inside one XCD's L2 (the eight
 // L2s are not coherent with each other), so the plain scatter pays a far
@@ -36,9 +35,6 @@ namespace pmx {
 
 constexpr int kBoxFloats = 16384;  // LDS budget of one brick box (64 KB)
 constexpr int kTile = 16;          // gather tile side
-// floats of padding before and after the arena: the gather's 16-float row
-// reads may start up to 15 floats before a box row and end up to 15 after
-constexpr int kArenaPad = 16;
 
 struct BrickArgs {
   float* mesh[3];        // C output meshes [B][nx][ny][nz] (zeroed)
@@ -410,7 +406,6 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
   __shared__ short cand[3][1024];
   __shared__ int ncand[3];
   __shared__ int lst[256][8];  // overlapping bricks of one chunk
-  __shared__ float otile[kTile * kTile * kTile];  // output transpose
   __shared__ int wcnt[4];
   const BrickGrid& g = a.g;
   const MeshDims& d = a.d;
@@ -450,10 +445,9 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
   const int nc0 = ncand[0], nc1 = ncand[1], nc2 = ncand[2];
   const int total = nc0 * nc1 * nc2;
   // owned cells: (x = t0x + k, y = t0y + iy, z = t0z + iz), k = 0..15
-  // owned cells: (x = t0x + ix, y = t0y + iy, z = t0z + k), k = 0..15
-  const int ix = tid / kTile, iy = tid % kTile;
-  const int cx = t0x + ix, cy = t0y + iy;
-  const bool xy_ok = ix < Tx && iy < Ty;
+  const int iy = tid / kTile, iz = tid % kTile;
+  const int cy = t0y + iy, cz = t0z + iz;
+  const bool yz_ok = iy < Ty && iz < Tz;
   float acc[C][PER];
 #pragma unroll
   for (int c = 0; c < C; ++c)
@@ -500,81 +494,52 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
     }
     __syncthreads();
     // 2. every thread adds the bricks, in list order, to its own cells
-    if (xy_ok) {
+    if (yz_ok) {
       for (int j = 0; j < nl; ++j) {
-        const int e0 = lst[j][4], e1 = lst[j][5], e2 = lst[j][6];
-        int ox = cx - lst[j][1];
-        if (ox < 0) ox += d.nx;
+        const int e1 = lst[j][5], e2 = lst[j][6];
         int oy = cy - lst[j][2];
         if (oy < 0) oy += d.ny;
-        if (ox >= e0 || oy >= e1) continue;
+        int oz = cz - lst[j][3];
+        if (oz < 0) oz += d.nz;
+        if (oy >= e1 || oz >= e2) continue;
+        const int e0 = lst[j][4];
         const int vol = e0 * e1 * e2;
         const float* src =
             a.arena + ((long long)b * g.nbricks + lst[j][0]) *
-                          (long long)(C * kBoxFloats) +
-            (ox * e1 + oy) * e2;  // this box row along z
-        // box z offset of tile cell k: dz + k, unless the box wraps round
-        // the mesh end before this tile (then per cell)
-        const int dz = t0z - lst[j][3];
-        if (dz >= 1 - PER && (dz >= 0 || dz + d.nz >= e2)) {
-          // one contiguous run: 4 x 16 bytes from src + dz (within the
-          // padded arena), cells outside [0, e2) dropped
-          float v[C][PER];
+                          (long long)(C * kBoxFloats);
+        const int a0 = lst[j][1];
+        // branch-free: every load is issued (out-of-box ones read box
+        // element 0) before the first is consumed, so the 16 x C loads of
+        // one brick are in flight together instead of one round trip each
+        float v[C][PER];
+        bool in[PER];
 #pragma unroll
-          for (int c = 0; c < C; ++c)
+        for (int k = 0; k < PER; ++k) {
+          int ox = t0x + k - a0;
+          if (ox < 0) ox += d.nx;
+          in[k] = k < Tx && ox < e0;
+          const int bi = in[k] ? (ox * e1 + oy) * e2 + oz : 0;
 #pragma unroll
-            for (int q = 0; q < PER / 4; ++q) {
-              float4 t;
-              __builtin_memcpy(&t, src + c * vol + dz + 4 * q, sizeof(t));
-              v[c][4 * q] = t.x;
-              v[c][4 * q + 1] = t.y;
-              v[c][4 * q + 2] = t.z;
-              v[c][4 * q + 3] = t.w;
-            }
-#pragma unroll
-          for (int k = 0; k < PER; ++k) {
-            const bool in = k < Tz && dz + k >= 0 && dz + k < e2;
-#pragma unroll
-            for (int c = 0; c < C; ++c) acc[c][k] += in ? v[c][k] : 0.0f;
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < PER; ++k) {
-            int oz = dz + k;
-            if (oz < 0) oz += d.nz;
-            if (k < Tz && oz < e2) {
-#pragma unroll
-              for (int c = 0; c < C; ++c) acc[c][k] += src[c * vol + oz];
-            }
-          }
+          for (int c = 0; c < C; ++c) v[c][k] = src[c * vol + bi];
         }
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+#pragma unroll
+          for (int c = 0; c < C; ++c) acc[c][k] += in[k] ? v[c][k] : 0.0f;
       }
     }
     __syncthreads();  // lst is rewritten by the next chunk
   }
-  // the tile leaves through LDS so that lanes store consecutive z cells
-  // (cell k of thread (ix, iy) at column (k + iy) mod 16: no bank conflicts)
+  if (!yz_ok) return;
   const bool rmw = a.ctr[2] != 0;  // some bricks went straight to the mesh
 #pragma unroll
-  for (int c = 0; c < C; ++c) {
-    __syncthreads();  // lst / the previous channel are no longer read
+  for (int k = 0; k < PER; ++k) {
+    if (k < Tx) {
+      const long long cell =
+          (((long long)b * d.nx + t0x + k) * d.ny + cy) * d.nz + cz;
 #pragma unroll
-    for (int k = 0; k < PER; ++k)
-      otile[(ix * kTile + iy) * kTile + ((k + iy) & (kTile - 1))] =
-          acc[c][k];
-    __syncthreads();
-    // lane (ox, oy, oz) = 16 consecutive threads per (x, y) row
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      const int e = r * 256 + tid;
-      const int oz = e % kTile, oy = (e / kTile) % kTile, ox = e / (kTile * kTile);
-      if (ox < Tx && oy < Ty && oz < Tz) {
-        const float v = otile[(ox * kTile + oy) * kTile + ((oz + oy) & (kTile - 1))];
-        const long long cell =
-            (((long long)b * d.nx + t0x + ox) * d.ny + t0y + oy) * d.nz + t0z +
-            oz;
-        a.mesh[c][cell] = rmw ? a.mesh[c][cell] + v : v;
-      }
+      for (int c = 0; c < C; ++c)
+        a.mesh[c][cell] = rmw ? a.mesh[c][cell] + acc[c][k] : acc[c][k];
     }
   }
 }
@@ -609,7 +574,7 @@ long long brick_work_floats(const MeshDims& d, const BrickGrid& g, int C) {
   // one arena slot of C boxes per brick
   const long long arena = (long long)d.B * g.nbricks * C * kBoxFloats;
   const long long meta = 8LL * d.B * g.nbricks;
-  return 2 * kArenaPad + arena + meta + 16;
+  return arena + meta + 16;
 }
 
 int launch_brick_paint(float* const* mesh, int C, const float* X,
@@ -619,9 +584,9 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
   if ((long long)d.B * d.N == 0) return kOk;
   BrickArgs a{};
   for (int c = 0; c < C; ++c) a.mesh[c] = mesh[c];
-  a.arena = work + kArenaPad;
+  a.arena = work;
   a.arena_cap = (long long)d.B * g.nbricks * C * kBoxFloats;
-  a.meta = reinterpret_cast<int*>(a.arena + a.arena_cap + kArenaPad);
+  a.meta = reinterpret_cast<int*>(work + a.arena_cap);
   a.ctr = a.meta + 8LL * d.B * g.nbricks;
   a.X = X;
   a.w = w;
