@@ -94,6 +94,46 @@ struct Dft<8, D> {
 };
 
 template <int D>
+struct Dft<16, D> {
+  // two 8-point DFTs of the even and odd inputs, odd ones twiddled by
+  // W16^m = cos(2 pi m / 16) + D i sin(2 pi m / 16)
+  static __device__ __forceinline__ void run(float2* a) {
+    constexpr float C[8] = {1.0f,
+                            0.92387953251128675613f,
+                            0.70710678118654752440f,
+                            0.38268343236508977173f,
+                            0.0f,
+                            -0.38268343236508977173f,
+                            -0.70710678118654752440f,
+                            -0.92387953251128675613f};
+    constexpr float S[8] = {0.0f,
+                            0.38268343236508977173f,
+                            0.70710678118654752440f,
+                            0.92387953251128675613f,
+                            1.0f,
+                            0.92387953251128675613f,
+                            0.70710678118654752440f,
+                            0.38268343236508977173f};
+    float2 e[8], o[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      e[m] = a[2 * m];
+      o[m] = a[2 * m + 1];
+    }
+    Dft<8, D>::run(e);
+    Dft<8, D>::run(o);
+#pragma unroll
+    for (int m = 1; m < 8; ++m)
+      o[m] = cmul(o[m], make_float2(C[m], D * S[m]));
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      a[m] = cadd(e[m], o[m]);
+      a[m + 8] = csub(e[m], o[m]);
+    }
+  }
+};
+
+template <int D>
 struct Dft<3, D> {
   static __device__ __forceinline__ void run(float2* a) {
     constexpr float h = 0.86602540378443864676f;  // sqrt(3)/2
@@ -106,10 +146,11 @@ struct Dft<3, D> {
   }
 };
 
-// radix plan: 8 first, then 4, 2, 3
+// radix plan: 16 first (fewer LDS round trips: 256 = 16 x 16 is two
+// stages instead of three), then 8, 4, 2, 3
 __host__ __device__ constexpr int pick_radix(int n) {
-  return (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2
-                                                : (n % 3 == 0) ? 3 : n;
+  return (n % 16 == 0) ? 16 : (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4
+       : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : n;
 }
 
 // LDS tile: element (line index i, column c) at s[i * P + c], P = TW + 1.
@@ -167,7 +208,8 @@ struct FftStages {
   static __device__ __forceinline__ void run(float2* s, const float2* tw) {
     if constexpr (NS < N) {
       constexpr int R = pick_radix(N / NS);
-      static_assert(R == 2 || R == 3 || R == 4 || R == 8, "unsupported radix");
+      static_assert(R == 2 || R == 3 || R == 4 || R == 8 || R == 16,
+                    "unsupported radix");
       fft_stage<N, R, NS, TW, T, D>(s, tw);
       FftStages<N, NS * R, TW, T, D>::run(s, tw);
     }
