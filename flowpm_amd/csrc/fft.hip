@@ -841,7 +841,7 @@ static int zc2r_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
 // threads against 169 at 256, which doubles the waves a CU holds).
 template <int N, int TW>
 struct ColT {
-  static constexpr int T = (N * TW >= 4096) ? 512 : 128;
+  static constexpr int T = (N * TW >= 4096) ? 512 : (N * TW >= 2048 ? 256 : 128);
 };
 
 template <int N, int TW>
