@@ -33,8 +33,10 @@ __device__ __forceinline__ Cic1 cic1(float x, int n) {
   c.w1 = 1.0f - fabsf(x - (fl + 1.0f));
   c.d0 = (x - fl > 0.0f) ? -1.0f : 0.0f;
   c.d1 = 1.0f;
-  c.i0 = wrapi(fi, n);
-  c.i1 = wrapi(fi + 1, n);
+  // floormod: a mask for power-of-two sides (two's complement makes it
+  // exact for negative fi too), the division otherwise
+  c.i0 = (n & (n - 1)) == 0 ? (fi & (n - 1)) : wrapi(fi, n);
+  c.i1 = c.i0 + 1 == n ? 0 : c.i0 + 1;
   return c;
 }
 
@@ -52,15 +54,18 @@ __device__ __forceinline__ bool pos_ok(float x, float y, float z) {
   return fabsf(x) < 1.0e9f && fabsf(y) < 1.0e9f && fabsf(z) < 1.0e9f;
 }
 
+// cell offsets within one mesh are 32-bit (sides <= 1024: < 2^30 cells);
+// the batch member's base is added once in 64 bits
 #define PMX_CORNERS(BODY)                                                 \
   {                                                                       \
     constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};                       \
     constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};                       \
     constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};                       \
+    const long long cb_ = (long long)b * ((long long)d.nx * d.ny * d.nz); \
+    const int rx0_ = cx.i0 * d.ny, rx1_ = cx.i1 * d.ny;                   \
+    const int rw_[4] = {(rx0_ + cy.i0) * d.nz, (rx0_ + cy.i1) * d.nz,     \
+                        (rx1_ + cy.i0) * d.nz, (rx1_ + cy.i1) * d.nz};    \
     _Pragma("unroll") for (int cc = 0; cc < 8; ++cc) {                    \
-      const int ix = OX[cc] ? cx.i1 : cx.i0;                              \
-      const int iy = OY[cc] ? cy.i1 : cy.i0;                              \
-      const int iz = OZ[cc] ? cz.i1 : cz.i0;                              \
       const float wx = OX[cc] ? cx.w1 : cx.w0;                            \
       const float wy = OY[cc] ? cy.w1 : cy.w0;                            \
       const float wz = OZ[cc] ? cz.w1 : cz.w0;                            \
@@ -68,7 +73,7 @@ __device__ __forceinline__ bool pos_ok(float x, float y, float z) {
       const float dy = OY[cc] ? cy.d1 : cy.d0;                            \
       const float dz = OZ[cc] ? cz.d1 : cz.d0;                            \
       const long long cell =                                              \
-          (((long long)b * d.nx + ix) * d.ny + iy) * d.nz + iz;           \
+          cb_ + (rw_[2 * OX[cc] + OY[cc]] + (OZ[cc] ? cz.i1 : cz.i0));    \
       (void)dx; (void)dy; (void)dz;                                       \
       BODY                                                                \
     }                                                                     \
@@ -227,17 +232,20 @@ __host__ __device__ __forceinline__ bool grid_tiled(int gx, int gy, int gz,
 // time than G = 1).
 __device__ __forceinline__ long long grid_tile(long long lb, int gx, int gy,
                                                int gz, long long N) {
-  const long long per_b = (long long)gx * (gy / kTileY) * (gz / kTileZ);
-  const long long b = lb / per_b, r0 = lb % per_b;
-  const int nty = gy / kTileY;
-  const int G = nty % 4 == 0 ? 4 : (nty % 2 == 0 ? 2 : 1);
-  const int sub = (int)(r0 % G);
-  const long long r = r0 / G;
-  const int x = (int)(r % gx);
-  const long long tile = r / gx;
-  const int ntz = gz / kTileZ;
-  const int ty = (int)(tile / ntz) * G + sub, tz = (int)(tile % ntz);
-  return b * N + ((long long)x * gy + ty * kTileY) * gz + tz * kTileZ;
+  // 32-bit: a launch has < 2^31 workgroups and a member < 2^31 particles
+  const unsigned per_b = (unsigned)gx * (unsigned)(gy / kTileY) *
+                         (unsigned)(gz / kTileZ);
+  const unsigned l = (unsigned)lb;
+  const unsigned b = l / per_b, r0 = l % per_b;
+  const unsigned nty = (unsigned)(gy / kTileY);
+  const unsigned G = nty % 4 == 0 ? 4 : (nty % 2 == 0 ? 2 : 1);
+  const unsigned sub = r0 % G, r = r0 / G;
+  const unsigned x = r % (unsigned)gx, tile = r / (unsigned)gx;
+  const unsigned ntz = (unsigned)(gz / kTileZ);
+  const unsigned ty = (tile / ntz) * G + sub, tz = tile % ntz;
+  return (long long)b * N +
+         (long long)((x * (unsigned)gy + ty * kTileY) * (unsigned)gz +
+                     tz * kTileZ);
 }
 
 // float offset of staged element i (0..767) of a 256-particle workgroup:
@@ -249,7 +257,7 @@ __device__ __forceinline__ long long stage_off(bool tiled, long long p0,
   return 3 * (p0 + (long long)r * gz) + o;
 }
 
-template <bool I4>
+template <bool I4, bool P2F>
 __global__ __launch_bounds__(256) void k_readout_kick_drift(
     float* __restrict__ X, float* __restrict__ P, float* __restrict__ F,
     Mesh3 m, MeshDims d, KickDrift c) {
@@ -312,8 +320,12 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
         az += v[cc].z * k;
       })
     }
-    const float Fx = (ax * c.fac) / c.f, Fy = (ay * c.fac) / c.f,
-                Fz = (az * c.fac) / c.f;
+    // (v fac) / f as tfpm.py:361-365; for a power-of-two f this equals
+    // v (fac / f) exactly (scaling by 2^-k commutes with rounding), which
+    // saves the division sequence
+    const float Fx = P2F ? ax * (c.fac / c.f) : (ax * c.fac) / c.f;
+    const float Fy = P2F ? ay * (c.fac / c.f) : (ay * c.fac) / c.f;
+    const float Fz = P2F ? az * (c.fac / c.f) : (az * c.fac) / c.f;
     sf[3 * t] = Fx;
     sf[3 * t + 1] = Fy;
     sf[3 * t + 2] = Fz;
@@ -688,12 +700,14 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
   ProfScope prof_(kProfReadoutKD, s);
   KickDrift cc = c;
   if (!grid_tiled(c.gx, c.gy, c.gz, d.N)) cc.gx = cc.gy = cc.gz = 0;
-  if (m.f4)
-    hipLaunchKernelGGL(k_readout_kick_drift<true>, dim3(nblk(tot, 256)),
-                       dim3(256), 0, s, X, P, F, m, d, cc);
-  else
-    hipLaunchKernelGGL(k_readout_kick_drift<false>, dim3(nblk(tot, 256)),
-                       dim3(256), 0, s, X, P, F, m, d, cc);
+  int ex;
+  const bool p2 = c.f > 0.0f && std::frexp(c.f, &ex) == 0.5f;
+  auto kern = m.f4 ? (p2 ? k_readout_kick_drift<true, true>
+                         : k_readout_kick_drift<true, false>)
+                   : (p2 ? k_readout_kick_drift<false, true>
+                         : k_readout_kick_drift<false, false>);
+  hipLaunchKernelGGL(kern, dim3(nblk(tot, 256)), dim3(256), 0, s, X, P, F, m,
+                     d, cc);
   PMX_LAUNCH_CHECK();
   return kOk;
 }
