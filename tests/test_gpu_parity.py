@@ -390,9 +390,13 @@ def test_nbody_pgd_gradients_against_oracle(gpu):
 def test_c2_gradient_finite_differences_128(gpu):
   """128^3, box 256, 2LPT, 20 steps: the taped adjoint's directional
   derivatives of chi^2 = sum (delta - delta_target)^2 w.r.t. the white
-  field along 16 random smooth Hermitian Fourier-mode directions vs central
-  finite differences of the GPU forward (float32 forward: relative
-  tolerance 2e-2 per direction, and the 16 pairs correlate > 0.999)."""
+  field along 16 random smooth Fourier-mode directions vs central finite
+  differences of the GPU forward.  The evolved CIC loss is only piecewise
+  smooth (particles cross cell boundaries), so the differences converge to
+  the derivative only for small steps, where float32 limits them: eps =
+  3e-4 of the field amplitude, 5e-2 of the largest derivative, and the 16
+  pairs must correlate > 0.995 (the exact check is against the float64
+  oracle's autograd, test_gradient_64_20steps_against_oracle)."""
   n, L = 128, 256.0
   c = fa.cosmology.Planck15()
   k = np.logspace(-4, 1, 256)
@@ -423,7 +427,7 @@ def test_c2_gradient_finite_differences_128(gpu):
     ph = rng.uniform(0, 2 * np.pi)
     v = np.cos(q @ (2 * np.pi * m / n) + ph).reshape(1, n, n, n)
     v = torch.tensor(v, dtype=torch.float32, device=gpu) * n**1.5
-    eps = 2e-3
+    eps = 3e-4
     with torch.no_grad():
       lp = float(loss(white0 + eps * v))
       lm = float(loss(white0 - eps * v))
@@ -433,4 +437,32 @@ def test_c2_gradient_finite_differences_128(gpu):
   rel = np.abs(ad - fd) / np.abs(fd).max()
   corr = np.corrcoef(ad, fd)[0, 1]
   print("C2 128^3 FD check: max rel %.3g, correlation %.6f" % (rel.max(), corr))
-  assert rel.max() < 2e-2 and corr > 0.999
+  assert rel.max() < 5e-2 and corr > 0.995
+
+
+def test_gradient_64_20steps_against_oracle(gpu):
+  """The deeply nonlinear case the finite differences cannot resolve:
+  64^3, box 128, 2LPT, 20 steps; taped adjoint vs float64 autograd of the
+  oracle (cosine > 0.9999)."""
+  n, L = 64, 128.0
+  white, pk = _ics(n, L, 100)
+  tgt = torch.rand(1, n, n, n, generator=torch.Generator().manual_seed(101),
+                   dtype=torch.float64) * 0.2 - 0.1
+  g = obg.GrowthTable(obg.planck15())
+  stages = np.linspace(0.1, 1.0, 21)
+  wo = white.clone().requires_grad_(True)
+  lin = opm.whitenoise_to_linear(wo, L, pk)
+  st = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=2), stages, [n] * 3)
+  rho = opm.cic_paint(torch.zeros_like(lin), st[0])
+  (((rho - 1.0) - tgt)**2).sum().backward()
+  c = fa.cosmology.Planck15()
+  wg = white.float().to(gpu).requires_grad_(True)
+  ling = fa.whitenoise_to_linear(n, L, wg, pk)
+  stg = fa.nbody(c, fa.lpt_init(c, ling, 0.1, order=2), stages, n,
+                 checkpoint_every=0)
+  rhog = fa.cic_paint(torch.zeros_like(ling), stg[0])
+  (((rhog - 1.0) - tgt.float().to(gpu))**2).sum().backward()
+  cs = cosine(wg.grad, wo.grad)
+  print("64^3 x 20 steps gradient cosine %.7f rel %.3g" %
+        (cs, rel_rms(wg.grad, wo.grad)))
+  assert cs > 0.9999
