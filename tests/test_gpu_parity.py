@@ -387,57 +387,51 @@ def test_nbody_pgd_gradients_against_oracle(gpu):
   assert ep < 2e-2
 
 
-def test_c2_gradient_finite_differences_128(gpu):
-  """128^3, box 256, 2LPT, 20 steps: the taped adjoint's directional
-  derivatives of chi^2 = sum (delta - delta_target)^2 w.r.t. the white
-  field along 16 random smooth Fourier-mode directions vs central finite
-  differences of the GPU forward.  The evolved CIC loss is only piecewise
-  smooth (particles cross cell boundaries), so the differences converge to
-  the derivative only for small steps, where float32 limits them: eps =
-  3e-4 of the field amplitude, 5e-2 of the largest derivative, and the 16
-  pairs must correlate > 0.995 (the exact check is against the float64
-  oracle's autograd, test_gradient_64_20steps_against_oracle)."""
+def test_c2_gradient_128_against_oracle(gpu):
+  """C2: 128^3, box 256, 2LPT, 20 steps, chi^2 = sum (delta - delta_t)^2
+  w.r.t. the white field: the taped adjoint against float64 autograd of the
+  oracle, over the whole gradient (cosine > 0.9999) and along 16 random
+  smooth Fourier-mode directions (directional derivatives within 1e-2).
+  Central finite differences of the float32 forward are no check here:
+  the evolved CIC loss is only piecewise smooth, and the step sizes that
+  resolve its derivative are below float32's resolution of the loss."""
   n, L = 128, 256.0
-  c = fa.cosmology.Planck15()
-  k = np.logspace(-4, 1, 256)
-  pkt = fa.tfpower.linear_matter_power(c, k)
-  pk = lambda x: fa.scipy.interpolate.interp_tf(x, k, pkt)
+  white, pk = _ics(n, L, 100)
+  tgt = torch.rand(1, n, n, n, generator=torch.Generator().manual_seed(101),
+                   dtype=torch.float64) * 0.2 - 0.1
+  g = obg.GrowthTable(obg.planck15())
   stages = np.linspace(0.1, 1.0, 21)
-  gen = torch.Generator().manual_seed(100)
-  white0 = (torch.randn(1, n, n, n, generator=gen, dtype=torch.float64) *
-            n**1.5).float().to(gpu)
-  tgen = torch.Generator().manual_seed(101)
-  tgt = (torch.rand(1, n, n, n, generator=tgen) * 0.2 - 0.1).to(gpu)
-
-  def loss(w):
-    lin = fa.whitenoise_to_linear(n, L, w, pk)
-    st = fa.nbody(c, fa.lpt_init(c, lin, 0.1, order=2), stages, n)
-    rho = fa.cic_paint(torch.zeros_like(lin), st[0])
-    return (((rho - 1.0) - tgt)**2).sum(dtype=torch.float64)
-
-  w = white0.clone().requires_grad_(True)
-  (g,) = torch.autograd.grad(loss(w), w)
+  wo = white.clone().requires_grad_(True)
+  lin = opm.whitenoise_to_linear(wo, L, pk)
+  st = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=2), stages, [n] * 3)
+  rho = opm.cic_paint(torch.zeros_like(lin), st[0])
+  (((rho - 1.0) - tgt)**2).sum().backward()
+  go = wo.grad.reshape(n, n, n)
+  c = fa.cosmology.Planck15()
+  wg = white.float().to(gpu).requires_grad_(True)
+  ling = fa.whitenoise_to_linear(n, L, wg, pk)
+  stg = fa.nbody(c, fa.lpt_init(c, ling, 0.1, order=2), stages, n)
+  rhog = fa.cic_paint(torch.zeros_like(ling), stg[0])
+  (((rhog - 1.0) - tgt.float().to(gpu))**2).sum().backward()
+  gg = wg.grad.double().cpu().reshape(n, n, n)
+  cs = cosine(gg, go)
   rng = np.random.default_rng(7)
-  q = np.indices((n, n, n)).reshape(3, -1).T.astype(np.float64)
-  ad, fd = [], []
+  q = np.indices((n, n, n)).astype(np.float64)
+  dd = []
   for _ in range(16):
     m = rng.integers(-4, 5, size=3)
     while not m.any():
       m = rng.integers(-4, 5, size=3)
     ph = rng.uniform(0, 2 * np.pi)
-    v = np.cos(q @ (2 * np.pi * m / n) + ph).reshape(1, n, n, n)
-    v = torch.tensor(v, dtype=torch.float32, device=gpu) * n**1.5
-    eps = 3e-4
-    with torch.no_grad():
-      lp = float(loss(white0 + eps * v))
-      lm = float(loss(white0 - eps * v))
-    fd.append((lp - lm) / (2 * eps))
-    ad.append(float((g.double() * v.double()).sum()))
-  ad, fd = np.array(ad), np.array(fd)
-  rel = np.abs(ad - fd) / np.abs(fd).max()
-  corr = np.corrcoef(ad, fd)[0, 1]
-  print("C2 128^3 FD check: max rel %.3g, correlation %.6f" % (rel.max(), corr))
-  assert rel.max() < 5e-2 and corr > 0.995
+    v = torch.tensor(np.cos(sum(2 * np.pi * m[d] * q[d] / n
+                                for d in range(3)) + ph))
+    dd.append((float((gg * v).sum()), float((go * v).sum())))
+  dd = np.array(dd)
+  rel = np.abs(dd[:, 0] - dd[:, 1]) / np.abs(dd[:, 1]).max()
+  print("C2 128^3 x 20 steps: gradient cosine %.7f; directional max rel %.3g"
+        % (cs, rel.max()))
+  assert cs > 0.9999
+  assert rel.max() < 1e-2
 
 
 def test_gradient_64_20steps_against_oracle(gpu):
