@@ -43,6 +43,8 @@ _SIGS = {
     "pmx_spec_to_full": (_i, [_vp, _vp, _vp, _vp]),
     "pmx_full_to_spec": (_i, [_vp, _vp, _vp, _vp]),
     "pmx_fft_c2r_poisson_grad": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _vp]),
+    "pmx_fft_c2r_longrange_grad": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f,
+                                        _c.c_double, _vp]),
     "pmx_force_meshes": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _vp]),
     "pmx_force_meshes_adj": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _vp]),
     "pmx_fft_poisson_grad_adj": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _vp]),

@@ -237,7 +237,11 @@ __device__ __forceinline__ float2 apply_term(float2 v, const Term& t, int kx,
     // the reference sets the zero mode's kk to 1 before its mask is formed,
     // so the mask keeps everything (kernels.py:43-47, :128-133)
     const double kk1 = kk == 0.0 ? 1.0 : kk;
-    if (t.lap) c *= (float)(1.0 / kk1);
+    if (t.lap) {
+      double lapd = 1.0 / kk1;
+      if (t.lr_r2 > 0.0) lapd *= exp(-kk * t.lr_r2);
+      c *= (float)lapd;
+    }
     if (t.pgd) {
       // float32 as TF evaluates it: exp(-kl^2 / kk) exp(-kk^2 / ks^4)
       const float kkf = (float)kk1, kk2f = (float)(kk1 * kk1);

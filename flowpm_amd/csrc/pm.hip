@@ -197,7 +197,8 @@ static inline Term with_pgd(Term t, const PgdTerm* pgd) {
 static int poisson_grad(const Plan& p, const float2* spec, int fwd,
                         float2* w0, float2* w1, float2* w2, float* fx,
                         float* fy, float* fz, float scale, hipStream_t s,
-                        float* f4 = nullptr, const PgdTerm* pgd = nullptr) {
+                        float* f4 = nullptr, const PgdTerm* pgd = nullptr,
+                        double lr_r2 = 0.0) {
   XJob xj{};
   xj.nin = 1;
   xj.nout = 2;
@@ -206,6 +207,8 @@ static int poisson_grad(const Plan& p, const float2* spec, int fwd,
   xj.out[1] = w1;  // lap * i gx
   xj.term[0][0] = with_pgd(term(scale, 1, 0, 0, 0), pgd);
   xj.term[0][1] = with_pgd(term(scale, 1, 1, 0, 0), pgd);
+  xj.term[0][0].lr_r2 = lr_r2;
+  xj.term[0][1].lr_r2 = lr_r2;
   xj.do_fwd = fwd;
   xj.do_inv = 1;
   PMX_TRY(launch_xpass(p, xj, s));
@@ -1162,6 +1165,18 @@ int pmx_fft_c2r_poisson_grad(const pmx_plan* plan, const float* dk,
   float2* w = reinterpret_cast<float2*>(work);
   return poisson_grad(p, reinterpret_cast<const float2*>(dk), 0, w,
                       w + p.d.S, w + 2 * p.d.S, fx, fy, fz, scale, S(stream));
+}
+
+int pmx_fft_c2r_longrange_grad(const pmx_plan* plan, const float* dk,
+                               float* work, float* fx, float* fy, float* fz,
+                               float scale, double r_split, void* stream) {
+  PMX_REQUIRE(plan && dk && work && fx && fy && fz,
+              "pmx_fft_c2r_longrange_grad: null pointer");
+  const Plan& p = P(plan);
+  float2* w = reinterpret_cast<float2*>(work);
+  return poisson_grad(p, reinterpret_cast<const float2*>(dk), 0, w,
+                      w + p.d.S, w + 2 * p.d.S, fx, fy, fz, scale, S(stream),
+                      nullptr, nullptr, r_split * r_split);
 }
 
 int pmx_force_meshes(const pmx_plan* plan, const float* rho, float* work,

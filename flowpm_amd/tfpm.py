@@ -11,7 +11,7 @@ in the HIP library (libpmx_hip.so):
   lpt2_source    tfpm.py:98-145   six phi_ij by one fan-out inverse, source,
                                   r2c
   lpt_init       tfpm.py:148-180  differentiable (hand-written adjoint)
-  apply_longrange tfpm.py:183-219 (split = 0, as on the path)
+  apply_longrange tfpm.py:183-219 (split = 0 on the path; split != 0 too)
   kick / drift   tfpm.py:273-314
   force          tfpm.py:317-366  paint -> fused r2c/Poisson/gradient/c2r ->
                                   readout
@@ -132,14 +132,20 @@ def linear_field(nc, boxsize, pk, kvec=None, batch_size=1, seed=None,
 
 
 # -------------------------------------------------------------------- LPT --
-def _poisson_grad_meshes(plan, spec):
-  """3 meshes c2r(dk * lap * i g_d), as one (3, B, nx, ny, nz) tensor."""
+def _poisson_grad_meshes(plan, spec, split=0.0):
+  """3 meshes c2r(dk * lap [* exp(-k^2 split^2)] * i g_d), as one
+  (3, B, nx, ny, nz) tensor."""
   out = torch.empty((3, plan.B, plan.nx, plan.ny, plan.nz),
                     dtype=torch.float32, device=spec.device)
   work = plan.spec(3)
-  _lib.call("pmx_fft_c2r_poisson_grad", plan.handle, _lib.ptr(spec),
-            _lib.ptr(work), _lib.ptr(out[0]), _lib.ptr(out[1]),
-            _lib.ptr(out[2]), 1.0, _lib.stream_of(spec))
+  if split == 0:
+    _lib.call("pmx_fft_c2r_poisson_grad", plan.handle, _lib.ptr(spec),
+              _lib.ptr(work), _lib.ptr(out[0]), _lib.ptr(out[1]),
+              _lib.ptr(out[2]), 1.0, _lib.stream_of(spec))
+  else:
+    _lib.call("pmx_fft_c2r_longrange_grad", plan.handle, _lib.ptr(spec),
+              _lib.ptr(work), _lib.ptr(out[0]), _lib.ptr(out[1]),
+              _lib.ptr(out[2]), 1.0, float(split), _lib.stream_of(spec))
   return out
 
 
@@ -256,12 +262,13 @@ def lpt_init(cosmo, linear, a, order=2, kvec=None, name="LPTInit"):
 # ------------------------------------------------------------------ nbody --
 def apply_longrange(x, delta_k, split=0, factor=1, kvec=None,
                     name="ApplyLongrange"):
-  """Long-range force at positions x from delta_k (tfpm.py:183-219)."""
-  if split != 0:
-    raise NotImplementedError("only split=0 (the path's setting) is built")
+  """Long-range force at positions x from delta_k (tfpm.py:183-219); with
+  split != 0 the Laplacian is filtered by exp(-k^2 split^2)
+  (kernels.py:81-102)."""
   B, nx, ny, nz = [int(s) for s in delta_k.shape]
   plan = _plans.get(delta_k.device, B, nx, ny, nz)
-  meshes = _poisson_grad_meshes(plan, _full_to_real_half(plan, delta_k))
+  meshes = _poisson_grad_meshes(plan, _full_to_real_half(plan, delta_k),
+                                split=float(split))
   f = torch.stack([cic_readout(meshes[d], x) for d in range(3)], dim=2)
   return f * factor
 

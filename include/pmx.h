@@ -106,6 +106,12 @@ int pmx_full_to_spec(const pmx_plan* plan, const float* full, float* spec,
 int pmx_fft_c2r_poisson_grad(const pmx_plan* plan, const float* dk,
                              float* work, float* fx, float* fy, float* fz,
                              float scale, void* stream);
+/* The same with the long-range filter of apply_longrange(split = r_split)
+ * (kernels.py:81-102): the multiplier is laplace_kernel *
+ * exp(-k^2 r_split^2), formed in float64 and rounded once. */
+int pmx_fft_c2r_longrange_grad(const pmx_plan* plan, const float* dk,
+                               float* work, float* fx, float* fy, float* fz,
+                               float scale, double r_split, void* stream);
 /* Fused density -> force meshes: r2c3d(rho * rho_scale) followed by the
  * Poisson x gradient inverse above, with the forward x transform, the
  * multiply and the inverse x transform in one pass (tfpm.py:344-354). */

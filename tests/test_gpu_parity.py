@@ -206,6 +206,11 @@ def test_lpt_ops_dropins(gpu):
   f_o = opm.apply_longrange(q, dk_o, factor=0.4)
   f_g = fa.tfpm.apply_longrange(q.float().to(gpu), dk_g, factor=0.4)
   close(f_g, f_o, 1e-5)
+  # the long-range filter exp(-k^2 split^2) (kernels.py:81-102)
+  f_o = opm.apply_longrange(q, dk_o, split=0.7, factor=0.4)
+  f_g = fa.tfpm.apply_longrange(q.float().to(gpu), dk_g, split=0.7,
+                                factor=0.4)
+  close(f_g, f_o, 1e-5)
 
 
 def test_nbody_readme_workflow_32(gpu):
