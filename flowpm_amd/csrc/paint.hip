@@ -76,13 +76,18 @@ __device__ __forceinline__ int imod(int a, int n) {
   return r < 0 ? r + n : r;
 }
 
-constexpr int kPPT = 8;  // particles per thread: bricks hold <= 2048
+// deposit workgroup: 512 threads (8 waves: with two 64 KB boxes per CU that
+// is 16 waves to hide the loads and the LDS atomics), 4 particles per
+// thread; bricks hold <= 2048 particles
+constexpr int kDT = 512;
+constexpr int kDW = kDT / 64;
+constexpr int kPPT = 2048 / kDT;
 
 template <int C>
-__global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
+__global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
   __shared__ int box[kBoxFloats];  // fixed-point accumulators
-  __shared__ int red[4][6];
-  __shared__ float redw[4][C];
+  __shared__ int red[kDW][6];
+  __shared__ float redw[kDW][C];
   __shared__ int sh[10];
   __shared__ float shs[C][2];
   const BrickGrid& g = a.g;
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
   long long pidx[kPPT];
 #pragma unroll
   for (int u = 0; u < kPPT; ++u) {
-    const int t = tid + u * 256;
+    const int t = tid + u * kDT;
     ok[u] = t < nloc;
     const int tt = ok[u] ? t : 0;
     const int lz = tt % cz, r = tt / cz, ly = r % cy, lx = r / cy;
@@ -184,9 +189,12 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     int lo[3], hi[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      lo[d] = min(min(red[0][d], red[1][d]), min(red[2][d], red[3][d]));
-      hi[d] = max(max(red[0][3 + d], red[1][3 + d]),
-                  max(red[2][3 + d], red[3][3 + d]));
+      lo[d] = red[0][d];
+      hi[d] = red[0][3 + d];
+      for (int w = 1; w < kDW; ++w) {
+        lo[d] = min(lo[d], red[w][d]);
+        hi[d] = max(hi[d], red[w][3 + d]);
+      }
     }
     const int n[3] = {a.d.nx, a.d.ny, a.d.nz};
     int ext[3] = {0, 0, 0};
@@ -210,7 +218,8 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     // total (nor a partial sum) can leave the int32 range
     bool wfin = true;
     for (int c = 0; c < C; ++c) {
-      const float swt = (redw[0][c] + redw[1][c]) + (redw[2][c] + redw[3][c]);
+      float swt = redw[0][c];
+      for (int w = 1; w < kDW; ++w) swt += redw[w][c];
       int e = 0;
       wfin = wfin && swt <= 3.0e38f;  // false for inf / nan weights
       if (swt <= 3.0e38f && swt > 0.0f) {
@@ -292,7 +301,7 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const float fxs = shs[c][0], fxinv = shs[c][1];
-    for (int i = tid; i < vol; i += 256) box[i] = 0;
+    for (int i = tid; i < vol; i += kDT) box[i] = 0;
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kPPT; ++u) {
@@ -314,7 +323,7 @@ __global__ __launch_bounds__(256) void k_brick_deposit(BrickArgs a) {
     }
     __syncthreads();
     float* dst = a.arena + off + (long long)c * vol;
-    for (int i = tid; i < vol; i += 256) dst[i] = (float)box[i] * fxinv;
+    for (int i = tid; i < vol; i += kDT) dst[i] = (float)box[i] * fxinv;
     if (c + 1 < C) __syncthreads();  // the box is cleared for the next one
   }
 }
@@ -564,7 +573,7 @@ BrickGrid make_brick_grid(int gx, int gy, int gz, int f, int C) {
 }
 
 bool brick_paint_ok(const MeshDims& d, const BrickGrid& g) {
-  return g.ex * g.ey * g.ez <= 256 * kPPT && g.gx * g.f == d.nx &&
+  return g.ex * g.ey * g.ez <= kDT * kPPT && g.gx * g.f == d.nx &&
          g.gy * g.f == d.ny && g.gz * g.f == d.nz &&
          (long long)g.gx * g.gy * g.gz == d.N && g.nbx <= 1024 &&
          g.nby <= 1024 && g.nbz <= 1024;
@@ -599,9 +608,9 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
     ProfScope prof_(C == 1 ? kProfPaint : kProfPaintW, s);
     dim3 grid(g.nbricks, d.B);
     if (C == 1)
-      hipLaunchKernelGGL(k_brick_deposit<1>, grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_brick_deposit<1>, grid, dim3(kDT), 0, s, a);
     else
-      hipLaunchKernelGGL(k_brick_deposit<3>, grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_brick_deposit<3>, grid, dim3(kDT), 0, s, a);
     PMX_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_brick_reduce, dim3(1), dim3(1024), 0, s, a.meta,
