@@ -94,3 +94,30 @@ def test_white_noise_philox_against_oracle(gpu, shape, seed):
   w2 = fa.white_noise(shape[1:], batch_size=shape[0], seed=seed + 1,
                       type="real").cpu().numpy()
   assert np.abs(w2 - w).max() > 1.0
+
+
+def test_save_state_of_gpu_state(gpu, tmp_path):
+  """io.save_state on a device-resident 2LPT state (tests/test_io.py:16-62
+  of the reference: B = 2, 16^3, box 100): the written rows equal the
+  state copied to the host, scaled in float32."""
+  n, L = 16, 100.0
+  c = fa.cosmology.Planck15()
+  gen = torch.Generator().manual_seed(0)
+  white = (torch.randn(2, n, n, n, generator=gen, dtype=torch.float64) *
+           n**1.5).float().to(gpu)
+  k = np.logspace(-4, 1, 256)
+  pkt = fa.tfpower.linear_matter_power(c, k)
+  lin = fa.whitenoise_to_linear(
+      [n] * 3, [L] * 3, white,
+      lambda x: fa.scipy.interpolate.interp_tf(x, k, pkt))
+  st = fa.lpt_init(c, lin, 0.1, order=2)
+  fn = str(tmp_path / "snap")
+  fa.io.save_state(c, st, 0.1, [n] * 3, [L] * 3, fn)
+  host = st.cpu().numpy()
+  scale = np.float32(np.float32(1.0) / np.float32(n) * np.float32(L))
+  for b in range(2):
+    np.testing.assert_array_equal(
+        fa.io.read_block(fn, "%d/Position" % b), host[0, b] * scale)
+    np.testing.assert_array_equal(
+        fa.io.read_block(fn, "%d/Velocity" % b), host[1, b] * scale)
+  np.testing.assert_allclose(fa.io.read_attrs(fn, "Header")["Time"], 0.1)
