@@ -221,7 +221,7 @@ def main():
           "a0": 0.1, "checkpoint_every": every_used,
           "fft_tiles": {"y": tiles[0], "x": tiles[1]},
       },
-      "autotune_force_mesh_ms": {"%dx%d" % k: v for k, v in tile_ms.items()},
+      "autotune_two_steps_ms": {"%dx%d" % k: v for k, v in tile_ms.items()},
       "fwd_adjoint_ms_per_step": fa_ms,
       "adjoint_over_forward": (fa_ms / ms_step) if fa_ms else None,
       "fwd_adjoint_recompute8_ms_per_step": fa8_ms,
@@ -321,8 +321,14 @@ def batched_runs(fa, cosmo, dev):
   one at a time; the uneven mix of 8 each of 32/64/96/128^3 (boxes
   100/200/300/400) forward under one graph, and forward + gradient."""
   from flowpm_amd.batched import BatchedNbody
+  from flowpm_amd import autotune
   stages = np.linspace(0.1, 1.0, 11)
   B = 64
+  # tile widths of every plan from the autotuner (untimed)
+  tiles = {"64x64^3": autotune.tune(dev, B, 64, 64, 64)[0],
+           "1x64^3": autotune.tune(dev, 1, 64, 64, 64)[0]}
+  for n in (32, 64, 96, 128):
+    tiles["8x%d^3" % n] = autotune.tune(dev, 8, n, n, n)[0]
   st = _ic_states(fa, cosmo, 64, 200.0, B, 1000, dev)
   runner = BatchedNbody([dict(cosmo=cosmo, state=st, stages=stages, nc=64)])
   t_b = _time(runner.run, 3)
@@ -360,6 +366,7 @@ def batched_runs(fa, cosmo, dev):
       "uneven_mix_sims_per_s": 32 / (t_mix * 1e-3),
       "uneven_mix_fwd_grad_ms": t_mix_grad,
       "sims64_batch_kernels": kern,
+      "fft_tiles": {k: "%dx%d" % v for k, v in tiles.items()},
       "uneven_mix_fwd_grad_sims_per_s": 32 / (t_mix_grad * 1e-3),
       "note": "10-step nbody per sim, ICs excluded; forward: graph replay; "
               "fwd_grad: forward + taped adjoint per size class",

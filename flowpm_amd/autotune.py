@@ -1,11 +1,16 @@
 """Tile-shape autotuner for the FFT line passes.
 
-For a plan (batch, nx, ny, nz) it times the fused density -> force-mesh
-pipeline (z r2c, y fwd, fused x, y inv, z c2r) with HIP events for every
-allowed (tile_y, tile_x) column width and keeps the fastest.  Choices are
-cached per process and, when ``FLOWPM_AMD_TUNE_CACHE`` names a file, in that
-JSON file across processes.
+For a plan (batch, nx, ny, nz) it times two PM steps of the library's
+time loop (paint, the force-mesh transforms, fused readout + kick + drift)
+on particles near the plan's own grid, with HIP events, for every allowed
+(tile_y, tile_x) column width and keeps the fastest.  The steps are timed
+rather than the transforms alone because the tile width that is faster on
+cache-resident spectra need not be faster inside the step, where the
+particle streams keep the spectra out of the 256 MB last-level cache.
+Choices are cached per process and, when ``FLOWPM_AMD_TUNE_CACHE`` names a
+file, in that JSON file across processes.
 """
+import numpy as np
 import json
 import os
 
@@ -61,19 +66,79 @@ def time_force_meshes(plan, reps=5):
   return e0.elapsed_time(e1) / reps
 
 
-def tune(device, B, nx, ny, nz, reps=5, verbose=False):
+class _StepBench:
+  """Two KDK steps (early-time coefficients) of the library's time loop on
+  particles at the plan's grid points displaced by 0.3 cells rms."""
+
+  def __init__(self, plan, nsteps=2):
+    from . import cosmology, tfpm
+    dev = plan.device
+    g = torch.Generator(device="cpu").manual_seed(0)
+    q = torch.stack(torch.meshgrid(
+        torch.arange(plan.nx, dtype=torch.float32),
+        torch.arange(plan.ny, dtype=torch.float32),
+        torch.arange(plan.nz, dtype=torch.float32), indexing="ij"),
+                    dim=-1).reshape(1, -1, 3)
+    self.N = q.shape[1]
+    st = torch.zeros((3, plan.B, self.N, 3), dtype=torch.float32)
+    st[0] = q + 0.3 * torch.randn((plan.B, self.N, 3), generator=g)
+    self.state = st.to(dev)
+    c = cosmology.Planck15()
+    self.coef = _lib.dbl(
+        tfpm.step_coefficients(c, np.linspace(0.1, 0.12, nsteps + 1)))
+    self.fac = float(np.float32(1.5 * c.Omega_m))
+    self.nsteps = nsteps
+    self.work = torch.empty(_lib.lib().pmx_nbody_work_floats(plan.handle),
+                            dtype=torch.float32, device=dev)
+    self.plan = plan
+    self.stream = _lib.stream_of(self.work)
+
+  def __call__(self):
+    _lib.call("pmx_nbody_run", self.plan.handle, _lib.ptr(self.state),
+              self.N, 1.0, self.fac, self.coef, 0, self.nsteps, 1,
+              _lib.ptr(self.work), self.stream)
+
+
+def _time(fn, dev, reps):
+  fn()
+  torch.cuda.synchronize(dev)
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(reps):
+    fn()
+  e1.record()
+  torch.cuda.synchronize(dev)
+  return e0.elapsed_time(e1) / reps
+
+
+def time_steps(plan, reps=3, bench=None):
+  """Mean ms of two PM steps of the time loop on this plan."""
+  return _time(bench or _StepBench(plan), plan.device, reps)
+
+
+def tune(device, B, nx, ny, nz, reps=3, rounds=3, verbose=False):
   """Measures every tile choice for this shape, applies and returns the
-  fastest (tile_y, tile_x) and the timings."""
+  fastest (tile_y, tile_x) and the timings (ms of two PM steps).
+
+  The choices are timed in turn, ``rounds`` times over, and each keeps its
+  best time: the differences between tile widths are a few per cent of a
+  step, the size of the drift of the GPU's clocks between consecutive
+  measurements."""
   key = (int(B), int(nx), int(ny), int(nz))
   plan = _plans.get(device, *key)
   table = _load()
   results = {}
-  for ty in _WIDTHS:
-    for tx in _WIDTHS:
-      _lib.call("pmx_plan_set_tiles", plan.handle, ty, tx)
-      results[(ty, tx)] = time_force_meshes(plan, reps)
-      if verbose:
-        print("tiles y=%d x=%d: %.4f ms" % (ty, tx, results[(ty, tx)]))
+  bench = _StepBench(plan)
+  for _ in range(rounds):
+    for ty in _WIDTHS:
+      for tx in _WIDTHS:
+        _lib.call("pmx_plan_set_tiles", plan.handle, ty, tx)
+        t = time_steps(plan, reps, bench)
+        results[(ty, tx)] = min(results.get((ty, tx), t), t)
+  if verbose:
+    for (ty, tx), t in sorted(results.items()):
+      print("tiles y=%d x=%d: %.4f ms" % (ty, tx, t))
   best = min(results, key=results.get)
   _lib.call("pmx_plan_set_tiles", plan.handle, *best)
   _plans.TILE_CHOICES[key] = best

@@ -238,9 +238,13 @@ __device__ __forceinline__ float2 apply_term(float2 v, const Term& t, int kx,
     // so the mask keeps everything (kernels.py:43-47, :128-133)
     const double kk1 = kk == 0.0 ? 1.0 : kk;
     if (t.lap) {
-      double lapd = 1.0 / kk1;
-      if (t.lr_r2 > 0.0) lapd *= exp(-kk * t.lr_r2);
-      c *= (float)lapd;
+      c *= (float)(1.0 / kk1);
+      // long-range filter as the reference forms it: its float64 exp is
+      // cast to complex64 and multiplies the complex64 Laplacian
+      // (tfpm.py:205-207).  Evaluated in float (expf of the float64
+      // exponent): a float64 exp here costs the 256-point x pass 19 VGPRs,
+      // above the 128 that let two 512-thread workgroups share a CU.
+      if (t.lr_r2 > 0.0) c *= expf((float)(-kk * t.lr_r2));
     }
     if (t.pgd) {
       // float32 as TF evaluates it: exp(-kl^2 / kk) exp(-kk^2 / ks^4)

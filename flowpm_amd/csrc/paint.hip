@@ -14,13 +14,14 @@
 //   fixed-point scale is a power of two chosen per brick and channel from
 //   the sum of the channel's |weights| so no cell can overflow: one deposit
 //   is rounded to at most 2^-31 of that sum.
-//   A brick whose box does not fit the LDS budget or the arena falls back
-//   to device-scope float atomics straight into the mesh.
+//   A brick whose box does not fit the LDS budget falls back to
+//   device-scope float atomics into the finished mesh (k_brick_spill, after
+//   phase 2).
 // phase 2 (k_brick_gather): one workgroup per 16^3 mesh tile evaluates the
 //   candidate bricks in parallel, compacts those whose boxes meet the tile
 //   (in candidate order), and every thread adds them, in that order, to the
 //   16 cells it owns; the tile is written with plain stores, no barrier per
-//   brick.
+//   brick, so the mesh is never cleared first.
 //
 // Device-scope float atomics cannot complete inside one XCD's L2 (the eight
 // L2s are not coherent with each other), so the plain scatter pays a far
@@ -37,7 +38,7 @@ constexpr int kBoxFloats = 16384;  // LDS budget of one brick box (64 KB)
 constexpr int kTile = 16;          // gather tile side
 
 struct BrickArgs {
-  float* mesh[3];        // C output meshes [B][nx][ny][nz] (zeroed)
+  float* mesh[3];        // C output meshes [B][nx][ny][nz] (overwritten)
   float* arena;          // box arena
   long long arena_cap;   // floats
   int* meta;             // 8 ints per brick: lo[3], ext[3], off lo/hi
@@ -269,34 +270,10 @@ __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
   if (mode == 0) return;
   const int lo0 = sh[1], lo1 = sh[2], lo2 = sh[3];
   const int e1 = sh[5], e2 = sh[6], vol = sh[7];
+  if (mode == 2) return;  // k_brick_spill adds it after the gather
   constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};
   constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};
   constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};
-  if (mode == 2) {
-    const MeshDims& d = a.d;
-#pragma unroll
-    for (int u = 0; u < kPPT; ++u) {
-      if (!ok[u]) continue;
-      const float x = px[u], y = py[u], z = pz[u];
-      const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
-      const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
-      const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
-      const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
-      const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
-                         (OZ[k] ? w1z : w0z);
-        const long long cell =
-            (((long long)b * d.nx + imod(ix + OX[k], d.nx)) * d.ny +
-             imod(iy + OY[k], d.ny)) * d.nz + imod(iz + OZ[k], d.nz);
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-          atomicAdd(a.mesh[c] + cell, wc[u][c] * kw);
-      }
-    }
-    return;
-  }
   const long long off = (long long)sh[8] | ((long long)sh[9] << 31);
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -540,15 +517,65 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
     __syncthreads();  // lst is rewritten by the next chunk
   }
   if (!yz_ok) return;
-  const bool rmw = a.ctr[2] != 0;  // some bricks went straight to the mesh
+  // every cell of the mesh is written here (the mesh needs no clearing)
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     if (k < Tx) {
       const long long cell =
           (((long long)b * d.nx + t0x + k) * d.ny + cy) * d.nz + cz;
 #pragma unroll
+      for (int c = 0; c < C; ++c) a.mesh[c][cell] = acc[c][k];
+    }
+  }
+}
+
+// Bricks whose box did not fit (box over the LDS budget, positions spread
+// over more than the mesh, non-finite weights) are added to the gathered
+// mesh with device-scope float atomics, one thread per particle.  Without
+// such bricks (k_brick_reduce counted none) every workgroup returns after
+// one load.
+template <int C>
+__global__ __launch_bounds__(256) void k_brick_spill(BrickArgs a) {
+  if (a.ctr[2] == 0) return;
+  const BrickGrid& g = a.g;
+  const MeshDims& d = a.d;
+  const long long nq = (long long)d.B * g.nbricks;
+  for (long long q = blockIdx.x; q < nq; q += gridDim.x) {
+    if (a.meta[8 * q + 7] == 0) continue;
+    const int b = (int)(q / g.nbricks), bid = (int)(q % g.nbricks);
+    const int kz = bid % g.nbz, ky = (bid / g.nbz) % g.nby,
+              kx = bid / (g.nbz * g.nby);
+    const int q0x = kx * g.ex, q0y = ky * g.ey, q0z = kz * g.ez;
+    const int cx = min(g.ex, g.gx - q0x), cy = min(g.ey, g.gy - q0y),
+              cz = min(g.ez, g.gz - q0z);
+    const int nloc = cx * cy * cz;
+    for (int t = threadIdx.x; t < nloc; t += blockDim.x) {
+      const int lz = t % cz, r = t / cz, ly = r % cy, lx = r / cy;
+      const long long pi = (long long)b * d.N +
+                           ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz +
+                           q0z + lz;
+      const float x = a.X[3 * pi] * a.ps, y = a.X[3 * pi + 1] * a.ps,
+                  z = a.X[3 * pi + 2] * a.ps;
+      if (!ok_pos(x, y, z)) continue;
+      float wc[C];
+#pragma unroll
       for (int c = 0; c < C; ++c)
-        a.mesh[c][cell] = rmw ? a.mesh[c][cell] + acc[c][k] : acc[c][k];
+        wc[c] = a.w ? a.w[pi * a.wstride + c] * a.wscale : a.wscale;
+      const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+      const float wx[2] = {1.0f - fabsf(x - flx), 1.0f - fabsf(x - (flx + 1.0f))};
+      const float wy[2] = {1.0f - fabsf(y - fly), 1.0f - fabsf(y - (fly + 1.0f))};
+      const float wz[2] = {1.0f - fabsf(z - flz), 1.0f - fabsf(z - (flz + 1.0f))};
+      const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+        const float kw = (wx[ox] * wy[oy]) * wz[oz];
+        const long long cell =
+            (((long long)b * d.nx + imod(ix + ox, d.nx)) * d.ny +
+             imod(iy + oy, d.ny)) * d.nz + imod(iz + oz, d.nz);
+#pragma unroll
+        for (int c = 0; c < C; ++c) atomicAdd(a.mesh[c] + cell, wc[c] * kw);
+      }
     }
   }
 }
@@ -625,6 +652,13 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
       hipLaunchKernelGGL(k_brick_gather<1>, grid, dim3(256), 0, s, a);
     else
       hipLaunchKernelGGL(k_brick_gather<3>, grid, dim3(256), 0, s, a);
+    PMX_LAUNCH_CHECK();
+    const long long nq = (long long)d.B * g.nbricks;
+    const dim3 sgrid((unsigned)std::min<long long>(nq, 256));
+    if (C == 1)
+      hipLaunchKernelGGL(k_brick_spill<1>, sgrid, dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_brick_spill<3>, sgrid, dim3(256), 0, s, a);
     PMX_LAUNCH_CHECK();
   }
   return kOk;

@@ -541,14 +541,13 @@ static long long nbody_work_floats(const Plan& p) {
   return 5 * M + 2 * 3 * p.d.S + brick_floats(p);
 }
 
-// Paint C meshes (zeroed here) with the brick paint when the particles form
-// the grid mesh/f, else with the plain atomic scatter.
+// Paint C meshes (overwritten) with the brick paint when the particles form
+// the grid mesh/f, else with the plain atomic scatter into cleared meshes.
 static int paint_meshes(const Plan& p, float* const* mesh, int C,
                         const float* X, const float* w, int wstride,
                         float wscale, long long N, float f, float* brick,
                         hipStream_t s) {
   const long long M = p.d.rows * p.d.nz;
-  for (int c = 0; c < C; ++c) PMX_TRY(launch_fill(mesh[c], M, 0.0f, s));
   const int fi = (int)f;
   const MeshDims md = paint_dims(p, N);
   if ((float)fi == f && fi >= 1 && p.d.nx % fi == 0 && p.d.ny % fi == 0 &&
@@ -559,8 +558,9 @@ static int paint_meshes(const Plan& p, float* const* mesh, int C,
       return launch_brick_paint(mesh, C, X, w, wstride, wscale, f, md, g,
                                 brick, s);
   }
-  if (C == 1) return launch_paint(mesh[0], X, w, wscale, md, f, 0, s);
-  return kUnsupported;  // C = 3 only arises on the grid path
+  if (C != 1) return kUnsupported;  // C = 3 only arises on the grid path
+  PMX_TRY(launch_fill(mesh[0], M, 0.0f, s));
+  return launch_paint(mesh[0], X, w, wscale, md, f, 0, s);
 }
 
 // force meshes at X (positions * f), density scaled by 1/nbar and 1/M

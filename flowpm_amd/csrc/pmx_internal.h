@@ -146,8 +146,7 @@ struct Term {
   int pgd;
   float pgd_kl, pgd_ks, pgd_kl2, pgd_ks4;  // kl, ks, kl^2, ks^4
   // x longrange_kernel = exp(-k^2 r_split^2) (kernels.py:81-102) when > 0,
-  // folded into the Laplacian in float64 (one rounding, as the reference's
-  // float64 product cast to complex64)
+  // with k^2 r_split^2 in float64 and the exp in float32
   double lr_r2;
 };
 

@@ -77,3 +77,25 @@ def test_batched_step_entry_matches_nbody(gpu):
   torch.cuda.synchronize()
   for o, r in zip(out, ref):
     assert _rel(o, r) < 1e-6
+
+
+def test_autotune_picks_a_tile_and_results_do_not_depend_on_it(gpu):
+  """The tuner times every (tile_y, tile_x) on two PM steps and applies the
+  fastest; every choice gives the same trajectory (the tile width only
+  changes which workgroup transforms a line, not the arithmetic)."""
+  from flowpm_amd import _plans, autotune
+  c = fa.cosmology.Planck15()
+  n, B = 32, 2
+  st = _state(c, n, B, 41, gpu)
+  stages = np.linspace(0.1, 1.0, 4)
+  plan = _plans.get(gpu, B, n, n, n)
+  outs = {}
+  for tiles in ((8, 8), (8, 16), (16, 8), (16, 16)):
+    fa._lib.call("pmx_plan_set_tiles", plan.handle, *tiles)
+    outs[tiles] = fa.nbody(c, st, stages, n)
+  for tiles, o in outs.items():
+    assert _rel(o, outs[(16, 16)]) < 1e-6, tiles
+  best, ms = autotune.tune(gpu, B, n, n, n)
+  assert sorted(ms) == sorted(outs)
+  assert ms[best] == min(ms.values()) and all(v > 0 for v in ms.values())
+  assert _plans.TILE_CHOICES[(B, n, n, n)] == best
