@@ -92,7 +92,10 @@ __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
   __shared__ int sh[10];
   __shared__ float shs[C][2];
   const BrickGrid& g = a.g;
-  const int bid = blockIdx.x, b = blockIdx.y;
+  // XCD-aware: each XCD deposits one contiguous run of bricks, so the
+  // cache lines that z-neighbouring bricks share are fetched into one L2
+  const int bid = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int b = blockIdx.y;
   const int tid = threadIdx.x;
   const int kz = bid % g.nbz, ky = (bid / g.nbz) % g.nby,
             kx = bid / (g.nbz * g.nby);
