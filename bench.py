@@ -111,7 +111,9 @@ def main():
   coef = fa.tfpm.step_coefficients(cosmo, stages)
   plan = _plans.get(dev, 1, n, n, n)
   from flowpm_amd import autotune
-  tiles, tile_ms = autotune.tune(dev, 1, n, n, n)  # untimed
+  # untimed: the tuner runs this workload (same ICs and stages) per choice
+  tiles, tile_ms = autotune.tune(dev, 1, n, n, n, state=state0,
+                                 stages=stages, cosmo=cosmo)
   fac = float(np.float32(1.5 * cosmo.Omega_m))
   work = torch.empty(_lib.lib().pmx_nbody_work_floats(plan.handle),
                      dtype=torch.float32, device=dev)
@@ -221,7 +223,7 @@ def main():
           "a0": 0.1, "checkpoint_every": every_used,
           "fft_tiles": {"y": tiles[0], "x": tiles[1]},
       },
-      "autotune_two_steps_ms": {"%dx%d" % k: v for k, v in tile_ms.items()},
+      "autotune_run_ms": {"%dx%d" % k: v for k, v in tile_ms.items()},
       "fwd_adjoint_ms_per_step": fa_ms,
       "adjoint_over_forward": (fa_ms / ms_step) if fa_ms else None,
       "fwd_adjoint_recompute8_ms_per_step": fa8_ms,
@@ -324,12 +326,14 @@ def batched_runs(fa, cosmo, dev):
   from flowpm_amd import autotune
   stages = np.linspace(0.1, 1.0, 11)
   B = 64
-  # tile widths of every plan from the autotuner (untimed)
-  tiles = {"64x64^3": autotune.tune(dev, B, 64, 64, 64)[0],
-           "1x64^3": autotune.tune(dev, 1, 64, 64, 64)[0]}
-  for n in (32, 64, 96, 128):
-    tiles["8x%d^3" % n] = autotune.tune(dev, 8, n, n, n)[0]
   st = _ic_states(fa, cosmo, 64, 200.0, B, 1000, dev)
+  # tile widths of every plan from the autotuner, on the timed workloads
+  # (untimed)
+  tiles = {"64x64^3": autotune.tune(dev, B, 64, 64, 64, state=st,
+                                    stages=stages, cosmo=cosmo)[0],
+           "1x64^3": autotune.tune(dev, 1, 64, 64, 64,
+                                   state=st[:, :1].contiguous(),
+                                   stages=stages, cosmo=cosmo)[0]}
   runner = BatchedNbody([dict(cosmo=cosmo, state=st, stages=stages, nc=64)])
   t_b = _time(runner.run, 3)
   from flowpm_amd import _lib
@@ -344,9 +348,10 @@ def batched_runs(fa, cosmo, dev):
   groups = []
   for i, (n, L) in enumerate(((32, 100.), (64, 200.), (96, 300.),
                               (128, 400.))):
-    groups.append(dict(cosmo=cosmo, state=_ic_states(fa, cosmo, n, L, 8,
-                                                     2000 + i, dev),
-                       stages=stages, nc=n))
+    gst = _ic_states(fa, cosmo, n, L, 8, 2000 + i, dev)
+    tiles["8x%d^3" % n] = autotune.tune(dev, 8, n, n, n, state=gst,
+                                        stages=stages, cosmo=cosmo)[0]
+    groups.append(dict(cosmo=cosmo, state=gst, stages=stages, nc=n))
   mix = BatchedNbody(groups)
   t_mix = _time(mix.run, 3)
 
