@@ -256,6 +256,58 @@ def test_nbody_64_batch2_against_oracle(gpu):
   assert ex < 1e-4 and ep < 1e-4 and ef < 1e-4
 
 
+@pytest.mark.parametrize("nc,box", [([16, 16, 48], [50.0, 50.0, 150.0]),
+                                    ([32, 64, 128], [100.0, 200.0, 400.0])])
+def test_nbody_rectangular_against_oracle(gpu, nc, box):
+  """End-to-end nbody on rectangular meshes (tests/test_tfpm.py:186-212 of
+  the reference: [nc, nc, 3 nc], box [bs, bs, 3 bs], 1LPT at 0.1, stages
+  linspace(0.1, 1, 10)); the second shape runs the tiled particle passes
+  (32-row slabs) on a non-cubic grid."""
+  gen = torch.Generator().manual_seed(100)
+  white = torch.randn([1] + nc, generator=gen, dtype=torch.float64) * (
+      nc[0] * nc[1] * nc[2])**0.5
+  _, pk = _ics(8, 1.0, 0)
+  g = obg.GrowthTable(obg.planck15())
+  stages = np.linspace(0.1, 1.0, 10)
+  lin = opm.whitenoise_to_linear(white, box, pk)
+  st_o = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=1), stages, nc)
+  c = fa.cosmology.Planck15()
+  lin_g = fa.whitenoise_to_linear(nc, box, white.float().to(gpu), pk)
+  close(lin_g, lin, 2e-6)
+  st_g = fa.nbody(c, fa.lpt_init(c, lin_g, 0.1, order=1), stages, nc)
+  q = opm.lagrangian_grid(nc, 1, torch.float64)
+  ex = rel_rms(st_g[0] - q.float().to(gpu), st_o[0] - q)
+  ep = rel_rms(st_g[1], st_o[1])
+  rho_o = opm.cic_paint(torch.zeros([1] + nc, dtype=torch.float64), st_o[0])
+  rho_g = fa.cic_paint(torch.zeros([1] + nc, device=gpu), st_g[0])
+  er = rel_rms(rho_g - 1, rho_o - 1)
+  print("rectangular %s nbody rel RMS: dX %.3g P %.3g delta %.3g" %
+        (nc, ex, ep, er))
+  assert ex < 1e-4 and ep < 1e-4 and er < 1e-4
+
+
+def test_nbody_pm_nc_factor2_against_oracle(gpu):
+  """End-to-end nbody with the force mesh twice the particle grid
+  (tests/test_tfpm.py:158-183 of the reference: pm_nc_factor = 2): 32^3
+  particles on a 64^3 force mesh."""
+  n, L = 32, 100.0
+  white, pk = _ics(n, L, 100)
+  g = obg.GrowthTable(obg.planck15())
+  stages = np.linspace(0.1, 1.0, 10)
+  lin = opm.whitenoise_to_linear(white, L, pk)
+  st_o = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=1), stages, [n] * 3,
+                   pm_nc_factor=2)
+  c = fa.cosmology.Planck15()
+  st_g = fa.nbody(c, fa.lpt_init(c, lin.float().to(gpu), 0.1, order=1),
+                  stages, n, pm_nc_factor=2)
+  q = opm.lagrangian_grid([n] * 3, 1, torch.float64)
+  ex = rel_rms(st_g[0] - q.float().to(gpu), st_o[0] - q)
+  ep = rel_rms(st_g[1], st_o[1])
+  ef = rel_rms(st_g[2], st_o[2])
+  print("pm_nc_factor 2 nbody rel RMS: dX %.3g P %.3g F %.3g" % (ex, ep, ef))
+  assert ex < 1e-4 and ep < 1e-4 and ef < 1e-4
+
+
 def test_nbody_gradient_against_oracle(gpu):
   n, L = 16, 64.0
   white, pk = _ics(n, L, 11)
