@@ -79,9 +79,10 @@ def main():
                   "FETCH_SIZE / WRITE_SIZE counters)")
   args = ap.parse_args()
   if args.gpus != 1:
-    print("bench.py measures one GPU: this build has no multi-GPU path "
-          "(simulations shard by batch member with no communication; see "
-          "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+    print("bench.py measures one GPU: there is no multi-GPU path (batched "
+          "simulations would shard by member with no communication, one "
+          "large simulation by slabs with all-to-all transposes).  Run with "
+          "--gpus 1.", file=sys.stderr)
     sys.exit(2)
 
   import flowpm_amd as fa
