@@ -84,6 +84,47 @@ constexpr int kDT = 512;
 constexpr int kDW = kDT / 64;
 constexpr int kPPT = 2048 / kDT;
 
+// one brick's particles in registers, kPPT per thread (raw positions and
+// weights; lanes past the brick's end hold ok = false)
+template <int C>
+struct BrickRegs {
+  float px[kPPT], py[kPPT], pz[kPPT], wc[kPPT][C];
+  bool ok[kPPT];
+};
+
+// every load issued up front, branch-free: lanes past the brick's end load
+// its first particle and drop it; the weight test is one uniform branch
+template <int C>
+__device__ __forceinline__ void load_brick(const BrickArgs& a, int bid, int b,
+                                           int tid, BrickRegs<C>& r) {
+  const BrickGrid& g = a.g;
+  const int kz = bid % g.nbz, ky = (bid / g.nbz) % g.nby,
+            kx = bid / (g.nbz * g.nby);
+  const int q0x = kx * g.ex, q0y = ky * g.ey, q0z = kz * g.ez;
+  const int cy = min(g.ey, g.gy - q0y), cz = min(g.ez, g.gz - q0z);
+  const int nloc = min(g.ex, g.gx - q0x) * cy * cz;
+  const long long pb = (long long)b * a.d.N;
+  long long pidx[kPPT];
+#pragma unroll
+  for (int u = 0; u < kPPT; ++u) {
+    const int t = tid + u * kDT;
+    r.ok[u] = t < nloc;
+    const int tt = r.ok[u] ? t : 0;
+    const int lz = tt % cz, q = tt / cz, ly = q % cy, lx = q / cy;
+    pidx[u] =
+        pb + ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz + q0z + lz;
+    r.px[u] = a.X[3 * pidx[u]];
+    r.py[u] = a.X[3 * pidx[u] + 1];
+    r.pz[u] = a.X[3 * pidx[u] + 2];
+  }
+  if (a.w) {
+#pragma unroll
+    for (int u = 0; u < kPPT; ++u)
+#pragma unroll
+      for (int c = 0; c < C; ++c) r.wc[u][c] = a.w[pidx[u] * a.wstride + c];
+  }
+}
+
 template <int C>
 __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
   __shared__ int box[kBoxFloats];  // fixed-point accumulators
@@ -97,214 +138,192 @@ __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
   const int bid = (int)xcd_block(blockIdx.x, gridDim.x);
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
-  const int kz = bid % g.nbz, ky = (bid / g.nbz) % g.nby,
-            kx = bid / (g.nbz * g.nby);
-  const int q0x = kx * g.ex, q0y = ky * g.ey, q0z = kz * g.ez;
-  const int cx = min(g.ex, g.gx - q0x), cy = min(g.ey, g.gy - q0y),
-            cz = min(g.ez, g.gz - q0z);
-  const int nloc = cx * cy * cz;
-  const long long pb = (long long)b * a.d.N;
   const float ps = a.ps;
-
-  // one pass over the brick's particles: every load issued up front
-  // (branch-free: lanes past the brick's end load its first particle and
-  // drop it; the weight test is one uniform branch around the whole loop),
-  // the positions and weights stay in registers for the deposit
-  float px[kPPT], py[kPPT], pz[kPPT], wc[kPPT][C];
-  bool ok[kPPT];
-  long long pidx[kPPT];
-#pragma unroll
-  for (int u = 0; u < kPPT; ++u) {
-    const int t = tid + u * kDT;
-    ok[u] = t < nloc;
-    const int tt = ok[u] ? t : 0;
-    const int lz = tt % cz, r = tt / cz, ly = r % cy, lx = r / cy;
-    pidx[u] =
-        pb + ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz + q0z + lz;
-    px[u] = a.X[3 * pidx[u]];
-    py[u] = a.X[3 * pidx[u] + 1];
-    pz[u] = a.X[3 * pidx[u] + 2];
-  }
-  if (a.w) {
-#pragma unroll
-    for (int u = 0; u < kPPT; ++u)
-#pragma unroll
-      for (int c = 0; c < C; ++c) wc[u][c] = a.w[pidx[u] * a.wstride + c];
-#pragma unroll
-    for (int u = 0; u < kPPT; ++u)
-#pragma unroll
-      for (int c = 0; c < C; ++c) wc[u][c] = wc[u][c] * a.wscale;
-  } else {
-#pragma unroll
-    for (int u = 0; u < kPPT; ++u)
-#pragma unroll
-      for (int c = 0; c < C; ++c) wc[u][c] = a.wscale;
-  }
-#pragma unroll
-  for (int u = 0; u < kPPT; ++u)
-    if (!ok[u]) {
-      px[u] = py[u] = pz[u] = 0.0f;
-#pragma unroll
-      for (int c = 0; c < C; ++c) wc[u][c] = 0.0f;
-    }
-  int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
-  float sw[C];  // per channel: sum over particles of |weight|
-#pragma unroll
-  for (int c = 0; c < C; ++c) sw[c] = 0.0f;
-#pragma unroll
-  for (int u = 0; u < kPPT; ++u) {
-    px[u] *= ps;
-    py[u] *= ps;
-    pz[u] *= ps;
-    ok[u] = ok[u] && ok_pos(px[u], py[u], pz[u]);
-    if (ok[u]) {
-      const int f[3] = {(int)floorf(px[u]), (int)floorf(py[u]),
-                        (int)floorf(pz[u])};
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        mn[d] = min(mn[d], f[d]);
-        mx[d] = max(mx[d], f[d]);
-      }
-#pragma unroll
-      for (int c = 0; c < C; ++c) sw[c] += fabsf(wc[u][c]);
-    }
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    mn[d] = wave_min(mn[d]);
-    mx[d] = wave_max(mx[d]);
-  }
-#pragma unroll
-  for (int c = 0; c < C; ++c)
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) sw[c] += __shfl_xor(sw[c], o);
-  const int wv = tid >> 6;
-  if ((tid & 63) == 0) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      red[wv][d] = mn[d];
-      red[wv][3 + d] = mx[d];
-    }
-#pragma unroll
-    for (int c = 0; c < C; ++c) redw[wv][c] = sw[c];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    int lo[3], hi[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = red[0][d];
-      hi[d] = red[0][3 + d];
-      for (int w = 1; w < kDW; ++w) {
-        lo[d] = min(lo[d], red[w][d]);
-        hi[d] = max(hi[d], red[w][3 + d]);
-      }
-    }
-    const int n[3] = {a.d.nx, a.d.ny, a.d.nz};
-    int ext[3] = {0, 0, 0};
-    bool any = lo[0] <= hi[0];
-    long long vol = 0;
-    bool small = true;
-    if (any) {
-      for (int d = 0; d < 3; ++d) {
-        // long long: positions may span up to 2e9 cells
-        const long long e = (long long)hi[d] - lo[d] + 2;
-        small = small && e <= n[d];
-        ext[d] = small ? (int)e : 0;
-      }
-      if (small) vol = (long long)ext[0] * ext[1] * ext[2];
-    }
-    int mode = 0;  // 0: nothing, 1: LDS box, 2: device atomics
-    const long long off =
-        ((long long)b * g.nbricks + bid) * (long long)(C * kBoxFloats);
-    int R = 0;
-    // fixed-point scale 2^e with (sum of |weights|) * 2^e < 2^30: no cell
-    // total (nor a partial sum) can leave the int32 range
-    bool wfin = true;
-    for (int c = 0; c < C; ++c) {
-      float swt = redw[0][c];
-      for (int w = 1; w < kDW; ++w) swt += redw[w][c];
-      int e = 0;
-      wfin = wfin && swt <= 3.0e38f;  // false for inf / nan weights
-      if (swt <= 3.0e38f && swt > 0.0f) {
-        int ex;
-        frexpf(swt, &ex);  // swt = m 2^ex, m in [0.5, 1)
-        e = min(max(30 - ex, -120), 120);
-      }
-      shs[c][0] = ldexpf(1.0f, e);
-      shs[c][1] = ldexpf(1.0f, -e);
-    }
-    if (any) {
-      // every brick owns a fixed arena slot: no shared allocator, and no
-      // shared counter either (k_brick_reduce takes the radius maximum)
-      mode = (small && vol <= kBoxFloats && wfin) ? 1 : 2;
-      if (mode == 1) {
-        // box radius around the brick's own mesh interval, for the gather
-        const int q0[3] = {q0x, q0y, q0z}, cc[3] = {cx, cy, cz};
-        for (int d = 0; d < 3; ++d) {
-          const int f = g.f;
-          R = max(R, q0[d] * f - lo[d]);
-          R = max(R, lo[d] + ext[d] - (q0[d] + cc[d]) * f);
-        }
-      }
-    }
-    int* m = a.meta + 8LL * ((long long)b * g.nbricks + bid);
-    m[0] = lo[0];
-    m[1] = lo[1];
-    m[2] = lo[2];
-    m[3] = mode == 1 ? ext[0] : 0;
-    m[4] = mode == 1 ? ext[1] : 0;
-    m[5] = mode == 1 ? ext[2] : 0;
-    m[6] = R;
-    m[7] = mode == 2 ? 1 : 0;
-    sh[0] = mode;
-    sh[1] = lo[0];
-    sh[2] = lo[1];
-    sh[3] = lo[2];
-    sh[4] = ext[0];
-    sh[5] = ext[1];
-    sh[6] = ext[2];
-    sh[7] = (int)vol;
-    sh[8] = (int)(off & 0x7fffffff);
-    sh[9] = (int)(off >> 31);
-  }
-  __syncthreads();
-  const int mode = sh[0];
-  if (mode == 0) return;
-  const int lo0 = sh[1], lo1 = sh[2], lo2 = sh[3];
-  const int e1 = sh[5], e2 = sh[6], vol = sh[7];
-  if (mode == 2) return;  // k_brick_spill adds it after the gather
-  constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};
-  constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};
-  constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};
-  const long long off = (long long)sh[8] | ((long long)sh[9] << 31);
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const float fxs = shs[c][0], fxinv = shs[c][1];
-    for (int i = tid; i < vol; i += kDT) box[i] = 0;
-    __syncthreads();
+  BrickRegs<C> cur;
+  load_brick<C>(a, bid, b, tid, cur);
+  {
+    const int kz = bid % g.nbz, ky = (bid / g.nbz) % g.nby,
+              kx = bid / (g.nbz * g.nby);
+    const int q0x = kx * g.ex, q0y = ky * g.ey, q0z = kz * g.ez;
+    const int cx = min(g.ex, g.gx - q0x), cy = min(g.ey, g.gy - q0y),
+              cz = min(g.ez, g.gz - q0z);
+    // the brick's registers, modified in place
+    float(&px)[kPPT] = cur.px;
+    float(&py)[kPPT] = cur.py;
+    float(&pz)[kPPT] = cur.pz;
+    float(&wc)[kPPT][C] = cur.wc;
+    bool(&ok)[kPPT] = cur.ok;
 #pragma unroll
     for (int u = 0; u < kPPT; ++u) {
-      if (!ok[u]) continue;
-      const float x = px[u], y = py[u], z = pz[u];
-      const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
-      const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
-      const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
-      const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
-      const int bx = (int)flx - lo0, by = (int)fly - lo1, bz = (int)flz - lo2;
-      const float wq = wc[u][c] * fxs;
+      px[u] = ok[u] ? px[u] : 0.0f;
+      py[u] = ok[u] ? py[u] : 0.0f;
+      pz[u] = ok[u] ? pz[u] : 0.0f;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
-                         (OZ[k] ? w1z : w0z);
-        const int li = ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
-        atomicAdd(&box[li], __float2int_rn(wq * kw));
+      for (int c = 0; c < C; ++c)
+        wc[u][c] = ok[u] ? (a.w ? wc[u][c] * a.wscale : a.wscale) : 0.0f;
+    }
+    int mn[3] = {INT_MAX, INT_MAX, INT_MAX},
+        mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+    float sw[C];  // per channel: sum over particles of |weight|
+#pragma unroll
+    for (int c = 0; c < C; ++c) sw[c] = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kPPT; ++u) {
+      px[u] *= ps;
+      py[u] *= ps;
+      pz[u] *= ps;
+      ok[u] = ok[u] && ok_pos(px[u], py[u], pz[u]);
+      if (ok[u]) {
+        const int f[3] = {(int)floorf(px[u]), (int)floorf(py[u]),
+                          (int)floorf(pz[u])};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          mn[d] = min(mn[d], f[d]);
+          mx[d] = max(mx[d], f[d]);
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) sw[c] += fabsf(wc[u][c]);
       }
     }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = wave_min(mn[d]);
+      mx[d] = wave_max(mx[d]);
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) sw[c] += __shfl_xor(sw[c], o);
+    const int wv = tid >> 6;
+    if ((tid & 63) == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        red[wv][d] = mn[d];
+        red[wv][3 + d] = mx[d];
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) redw[wv][c] = sw[c];
+    }
     __syncthreads();
-    float* dst = a.arena + off + (long long)c * vol;
-    for (int i = tid; i < vol; i += kDT) dst[i] = (float)box[i] * fxinv;
-    if (c + 1 < C) __syncthreads();  // the box is cleared for the next one
+    if (tid == 0) {
+      int lo[3], hi[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = red[0][d];
+        hi[d] = red[0][3 + d];
+        for (int w = 1; w < kDW; ++w) {
+          lo[d] = min(lo[d], red[w][d]);
+          hi[d] = max(hi[d], red[w][3 + d]);
+        }
+      }
+      const int n[3] = {a.d.nx, a.d.ny, a.d.nz};
+      int ext[3] = {0, 0, 0};
+      bool any = lo[0] <= hi[0];
+      long long vol = 0;
+      bool small = true;
+      if (any) {
+        for (int d = 0; d < 3; ++d) {
+          // long long: positions may span up to 2e9 cells
+          const long long e = (long long)hi[d] - lo[d] + 2;
+          small = small && e <= n[d];
+          ext[d] = small ? (int)e : 0;
+        }
+        if (small) vol = (long long)ext[0] * ext[1] * ext[2];
+      }
+      int mode = 0;  // 0: nothing, 1: LDS box, 2: device atomics (spill)
+      const long long off =
+          ((long long)b * g.nbricks + bid) * (long long)(C * kBoxFloats);
+      int R = 0;
+      // fixed-point scale 2^e with (sum of |weights|) * 2^e < 2^30: no cell
+      // total (nor a partial sum) can leave the int32 range
+      bool wfin = true;
+      for (int c = 0; c < C; ++c) {
+        float swt = redw[0][c];
+        for (int w = 1; w < kDW; ++w) swt += redw[w][c];
+        int e = 0;
+        wfin = wfin && swt <= 3.0e38f;  // false for inf / nan weights
+        if (swt <= 3.0e38f && swt > 0.0f) {
+          int ex;
+          frexpf(swt, &ex);  // swt = m 2^ex, m in [0.5, 1)
+          e = min(max(30 - ex, -120), 120);
+        }
+        shs[c][0] = ldexpf(1.0f, e);
+        shs[c][1] = ldexpf(1.0f, -e);
+      }
+      if (any) {
+        // every brick owns a fixed arena slot: no shared allocator, and no
+        // shared counter either (k_brick_reduce takes the radius maximum)
+        mode = (small && vol <= kBoxFloats && wfin) ? 1 : 2;
+        if (mode == 1) {
+          // box radius around the brick's own mesh interval, for the gather
+          const int q0[3] = {q0x, q0y, q0z}, cc[3] = {cx, cy, cz};
+          for (int d = 0; d < 3; ++d) {
+            const int f = g.f;
+            R = max(R, q0[d] * f - lo[d]);
+            R = max(R, lo[d] + ext[d] - (q0[d] + cc[d]) * f);
+          }
+        }
+      }
+      int* m = a.meta + 8LL * ((long long)b * g.nbricks + bid);
+      m[0] = lo[0];
+      m[1] = lo[1];
+      m[2] = lo[2];
+      m[3] = mode == 1 ? ext[0] : 0;
+      m[4] = mode == 1 ? ext[1] : 0;
+      m[5] = mode == 1 ? ext[2] : 0;
+      m[6] = R;
+      m[7] = mode == 2 ? 1 : 0;
+      sh[0] = mode;
+      sh[1] = lo[0];
+      sh[2] = lo[1];
+      sh[3] = lo[2];
+      sh[4] = ext[0];
+      sh[5] = ext[1];
+      sh[6] = ext[2];
+      sh[7] = (int)vol;
+      sh[8] = (int)(off & 0x7fffffff);
+      sh[9] = (int)(off >> 31);
+    }
+    __syncthreads();
+    const int mode = sh[0];
+    const int lo0 = sh[1], lo1 = sh[2], lo2 = sh[3];
+    const int e1 = sh[5], e2 = sh[6], vol = sh[7];
+    const long long off = (long long)sh[8] | ((long long)sh[9] << 31);
+    // mode 0: no particles; mode 2: k_brick_spill adds it after the gather
+    if (mode != 1) return;
+    {
+      constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};
+      constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};
+      constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const float fxs = shs[c][0], fxinv = shs[c][1];
+        for (int i = tid; i < vol; i += kDT) box[i] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPPT; ++u) {
+          if (!ok[u]) continue;
+          const float x = px[u], y = py[u], z = pz[u];
+          const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+          const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
+          const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
+          const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
+          const int bx = (int)flx - lo0, by = (int)fly - lo1, bz = (int)flz - lo2;
+          const float wq = wc[u][c] * fxs;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
+                             (OZ[k] ? w1z : w0z);
+            const int li = ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
+            atomicAdd(&box[li], __float2int_rn(wq * kw));
+          }
+        }
+        __syncthreads();
+        float* dst = a.arena + off + (long long)c * vol;
+        for (int i = tid; i < vol; i += kDT) dst[i] = (float)box[i] * fxinv;
+        if (c + 1 < C) __syncthreads();  // the box is cleared for the next one
+      }
+    }
   }
 }
 
