@@ -42,7 +42,8 @@ void prof_record(int kind, hipStream_t s, hipEvent_t* ev, bool end) {
 static const char* kProfNames[kProfKinds] = {
     "fill", "paint", "paint_gather", "fft_z_r2c", "fft_y_fwd", "fft_x_fused", "fft_y_inv",
     "fft_z_c2r", "readout_kick_drift", "kick_drift", "readout", "cic_adjoint",
-    "paint_w3", "paint_w3_gather", "adjoint_update", "other"};
+    "paint_w3", "paint_w3_gather", "adjoint_update", "grad", "grad_adjoint",
+    "other"};
 
 // ------------------------------------------------------------------ plan --
 static int plan_create(Plan** out, int B, int nx, int ny, int nz) {
@@ -242,9 +243,45 @@ static int poisson_grad(const Plan& p, const float2* spec, int fwd,
   return kOk;
 }
 
+// The interleaved force mesh f4 from one potential: phi = c2r(lap * r2c(rho)
+// [* PGD]) into the real scratch mesh phi, then the real-space gradient
+// (the reference's i g_d multiplier is the fourth-order central difference,
+// stencil.hip): one transform chain instead of three.
+static int force_mesh_f4(const Plan& p, const float* rho, float2* w,
+                         float* phi, float* f4, float rho_scale,
+                         hipStream_t s, const PgdTerm* pgd) {
+  PMX_TRY(r2c_zy(p, &rho, &w, 1, rho_scale, s));
+  XJob xj{};
+  xj.nin = 1;
+  xj.nout = 1;
+  xj.in[0] = w;
+  xj.out[0] = w;
+  xj.term[0][0] = with_pgd(term(1.0f, 1, 0, 0, 0), pgd);
+  xj.do_fwd = 1;
+  xj.do_inv = 1;
+  PMX_TRY(launch_xpass(p, xj, s));
+  YJobs yj{};
+  yj.n = 1;
+  yj.j[0].in = w;
+  yj.j[0].nout = 1;
+  yj.j[0].out[0] = w;
+  yj.j[0].scale[0] = 1.0f;
+  PMX_TRY(launch_ypass(p, yj, 1, s));
+  ZJobs zj{};
+  zj.n = 1;
+  zj.j[0] = zc2r(w, phi, 1.0f, 0);
+  PMX_TRY(launch_zpass_c2r(p, zj, s));
+  return launch_grad_f4(p, phi, f4, s);
+}
+
+// phi (optional): a real scratch mesh; with it and f4 the force mesh comes
+// from one potential and the real-space gradient (force_mesh_f4)
 static int force_meshes(const Plan& p, const float* rho, float2* w, float* fx,
                         float* fy, float* fz, float rho_scale, hipStream_t s,
-                        float* f4 = nullptr, const PgdTerm* pgd = nullptr) {
+                        float* f4 = nullptr, const PgdTerm* pgd = nullptr,
+                        float* phi = nullptr) {
+  if (f4 && phi && grad_stencil_ok(p))
+    return force_mesh_f4(p, rho, w, phi, f4, rho_scale, s, pgd);
   float2* w0 = w;
   float2* w1 = w + p.d.S;
   float2* w2 = w + 2 * p.d.S;
@@ -256,11 +293,44 @@ static int force_meshes(const Plan& p, const float* rho, float2* w, float* fx,
 // grho = IFFT3(sum_d conj(lap i g_d) FFT3(g_d) * rho_scale).
 // f4w (optional): write grho into component 3 of this interleaved
 // (x, y, z, -) mesh instead of to grho.
+// phibar (optional): a real scratch mesh; with it the three cotangents are
+// first contracted by the adjoint real-space gradient,
+// phibar = -(D_x gx + D_y gy + D_z gz), and one transform chain follows.
 static int force_meshes_adj(const Plan& p, const float* gx, const float* gy,
                             const float* gz, float2* w, float* grho,
                             float rho_scale, hipStream_t s,
                             float* f4w = nullptr,
-                            const PgdTerm* pgd = nullptr) {
+                            const PgdTerm* pgd = nullptr,
+                            float* phibar = nullptr) {
+  if (phibar && grad_stencil_ok(p)) {
+    PMX_TRY(launch_grad_adj(p, gx, gy, gz, phibar, s));
+    const float* pb = phibar;
+    PMX_TRY(r2c_zy(p, &pb, &w, 1, rho_scale, s));
+    XJob xj{};
+    xj.nin = 1;
+    xj.nout = 1;
+    xj.in[0] = w;
+    xj.out[0] = w;
+    xj.term[0][0] = with_pgd(term(1.0f, 1, 0, 0, 0), pgd);
+    xj.do_fwd = 1;
+    xj.do_inv = 1;
+    PMX_TRY(launch_xpass(p, xj, s));
+    YJobs yj{};
+    yj.n = 1;
+    yj.j[0].in = w;
+    yj.j[0].nout = 1;
+    yj.j[0].out[0] = w;
+    yj.j[0].scale[0] = 1.0f;
+    PMX_TRY(launch_ypass(p, yj, 1, s));
+    ZJobs zj{};
+    zj.n = 1;
+    zj.j[0] = zc2r(w, f4w ? f4w : grho, 1.0f, 0);
+    if (f4w) {
+      zj.j[0].ostride = 4;
+      zj.j[0].ocomp = 3;
+    }
+    return launch_zpass_c2r(p, zj, s);
+  }
   float2* ws[3] = {w, w + p.d.S, w + 2 * p.d.S};
   const float* in[3] = {gx, gy, gz};
   PMX_TRY(r2c_zy(p, in, ws, 3, rho_scale, s));
@@ -578,8 +648,9 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
   // rho * (1/nbar) with nbar = N / Mb (tfpm.py:346-351), then r2c / Mb
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
+  // the density buffer takes the potential once it is transformed
   return force_meshes(p, nw.rho, nw.w, nullptr, nullptr, nullptr, sc, s,
-                      nw.f4, pgd);
+                      nw.f4, pgd, nw.rho);
 }
 
 // Where a run leaves what the reverse sweep needs of each force evaluation
@@ -725,8 +796,9 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
     // mesh cotangents by the brick paint; one fused particle pass at the end
     // the density cotangent goes to the unused fourth component of the
     // force mesh, so the finish kernel gathers one float4 per corner
+    // nw.rho is free here: it takes the potential's cotangent
     PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nullptr, sc,
-                             s, f4, pgd));
+                             s, f4, pgd, nw.rho));
     return launch_force_vjp_finish(gX, gP, gF, gc, X, m, nullptr, md, f,
                                    fac / f, u, s);
   }

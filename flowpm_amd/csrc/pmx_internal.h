@@ -82,6 +82,8 @@ enum ProfKind : int {
   kProfPaintW,        // weighted three-channel brick deposit (force VJP)
   kProfPaintWGather,  // its gather
   kProfAdjUpdate,     // KDK cotangent updates and checkpoint copies
+  kProfGrad,          // real-space gradient: potential -> force mesh
+  kProfGradAdj,       // its adjoint: three mesh cotangents -> potential
   kProfOther,
   kProfKinds
 };
@@ -209,6 +211,13 @@ struct MeshDims {
   long long N;  // particles per batch member
 };
 int launch_fill(float* p, long long n, float v, hipStream_t s);
+// Fourth-order central-difference gradient of a real mesh (stencil.hip):
+// f4 = (D_x phi, D_y phi, D_z phi, 0) per cell, and the adjoint
+// out = -(D_x gx + D_y gy + D_z gz).  Needs nz % 4 == 0.
+bool grad_stencil_ok(const Plan& p);
+int launch_grad_f4(const Plan& p, const float* phi, float* f4, hipStream_t s);
+int launch_grad_adj(const Plan& p, const float* gx, const float* gy,
+                    const float* gz, float* out, hipStream_t s);
 int launch_paint(float* mesh, const float* pos, const float* w, float wscale,
                  const MeshDims& d, float pscale, int mode, hipStream_t s);
 int launch_readout(float* out, const float* mesh, const float* pos,
