@@ -104,24 +104,42 @@ __device__ __forceinline__ Nbr nbr_of(const Cell& c, const GridDims& g) {
   return n;
 }
 
-// f4[cell] = (D_x phi, D_y phi, D_z phi, 0)
+// f4[cell] = (D_x phi, D_y phi, D_z phi, 0).  A thread's 4 cells are
+// consecutive, so its own 4 float4 stores would leave each store
+// instruction spread over 64-byte strides; they go through LDS instead and
+// every store instruction writes one contiguous 1 KB run per wave (measured
+// at 256^3: 78 us against 103).
 __global__ __launch_bounds__(256) void k_grad_f4(const float* __restrict__ phi,
                                                  float4* __restrict__ f4,
                                                  GridDims g) {
   const Cell c = cell_of(g);
-  if (!c.ok) return;
-  const Nbr n = nbr_of(c, g);
-  const int z0 = c.z0;
-  const float4 gz = dz_row(phi + n.c, z0, g.nz);
-  const float4 gy = dvec(ld4(phi + n.ym2 + z0), ld4(phi + n.ym1 + z0),
-                         ld4(phi + n.yp1 + z0), ld4(phi + n.yp2 + z0));
-  const float4 gx = dvec(ld4(phi + n.xm2 + z0), ld4(phi + n.xm1 + z0),
-                         ld4(phi + n.xp1 + z0), ld4(phi + n.xp2 + z0));
-  float4* o = f4 + n.c + z0;
-  o[0] = make_float4(gx.x, gy.x, gz.x, 0.0f);
-  o[1] = make_float4(gx.y, gy.y, gz.y, 0.0f);
-  o[2] = make_float4(gx.z, gy.z, gz.z, 0.0f);
-  o[3] = make_float4(gx.w, gy.w, gz.w, 0.0f);
+  float4 gx = make_float4(0.f, 0.f, 0.f, 0.f), gy = gx, gz = gx;
+  if (c.ok) {
+    const Nbr n = nbr_of(c, g);
+    const int z0 = c.z0;
+    gz = dz_row(phi + n.c, z0, g.nz);
+    gy = dvec(ld4(phi + n.ym2 + z0), ld4(phi + n.ym1 + z0),
+              ld4(phi + n.yp1 + z0), ld4(phi + n.yp2 + z0));
+    gx = dvec(ld4(phi + n.xm2 + z0), ld4(phi + n.xm1 + z0),
+              ld4(phi + n.xp1 + z0), ld4(phi + n.xp2 + z0));
+  }
+  __shared__ float4 st[256 * 4];
+  const int t = threadIdx.x;
+  st[4 * t] = make_float4(gx.x, gy.x, gz.x, 0.0f);
+  st[4 * t + 1] = make_float4(gx.y, gy.y, gz.y, 0.0f);
+  st[4 * t + 2] = make_float4(gx.z, gy.z, gz.z, 0.0f);
+  st[4 * t + 3] = make_float4(gx.w, gy.w, gz.w, 0.0f);
+  __syncthreads();
+  // the block's threads cover 4 * 256 consecutive cells from its first one
+  const long long tot = (long long)g.B * g.nx * g.ny * g.nq;
+  const long long i0 =
+      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x;
+  const long long ncell = 4 * min((long long)blockDim.x, tot - i0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int q = t + 256 * k;
+    if (q < ncell) f4[4 * i0 + q] = st[q];
+  }
 }
 
 // out = -(D_x gx + D_y gy + D_z gz)
