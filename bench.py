@@ -50,9 +50,12 @@ def algorithmic_bytes(kind, n, N):
       "paint_gather": 4 * M,
       "fft_z_r2c": 4 * M + S8,
       "fft_y_fwd": 2 * S8,
-      "fft_x_fused": S8 + 2 * S8,
-      "fft_y_inv": 2 * S8 + 3 * S8,
-      "fft_z_c2r": 3 * S8 + 3 * 4 * M,
+      # the potential: x forward, multiply, x inverse; y inverse; z c2r
+      "fft_x_fused": 2 * S8,
+      "fft_y_inv": 2 * S8,
+      "fft_z_c2r": S8 + 4 * M,
+      # potential in, the three force meshes out
+      "grad": 4 * M + 3 * 4 * M,
       # X, P read and written; F stays in registers inside fused steps
       "readout_kick_drift": 3 * 4 * M + 24 * N + 24 * N,
   }
@@ -254,8 +257,8 @@ _KIND_KERNEL = {
     "readout_kick_drift": r"k_readout_kick_drift<", "paint": r"k_brick_deposit<1>",
     "paint_gather": r"k_brick_gather<1>", "fft_z_r2c": r"k_zpass_r2c<",
     "fft_y_fwd": r"k_ypass<\d+, 16, \d+, -1, false>",
-    "fft_x_fused": r"k_xpass<\d+, 16, ", "fft_y_inv": r"k_ypass<\d+, 16, \d+, 1, true>",
-    "fft_z_c2r": r"k_zpass_c2r_f4<", "fill": r"k_fill",
+    "fft_x_fused": r"k_xpass<\d+, 16, ", "fft_y_inv": r"k_ypass<\d+, 16, \d+, 1, false>",
+    "fft_z_c2r": r"k_zpass_c2r<\d+, 16, ", "grad": r"k_grad_f4", "fill": r"k_fill",
 }
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "pm_step_256_hbm_traffic.md")
 

@@ -20,23 +20,21 @@
 
 namespace pmx {
 
-namespace {
-
 struct GridDims {
   int B, nx, ny, nz, nq;  // nq = nz / 4
 };
 
-__device__ __forceinline__ float4 ld4(const float* p) {
+static __device__ __forceinline__ float4 ld4(const float* p) {
   return *reinterpret_cast<const float4*>(p);
 }
-__device__ __forceinline__ float2 ld2(const float* p) {
+static __device__ __forceinline__ float2 ld2(const float* p) {
   return *reinterpret_cast<const float2*>(p);
 }
-__device__ __forceinline__ float dcoef(float m2, float m1, float p1,
+static __device__ __forceinline__ float dcoef(float m2, float m1, float p1,
                                        float p2) {
   return (2.0f / 3.0f) * (p1 - m1) - (1.0f / 12.0f) * (p2 - m2);
 }
-__device__ __forceinline__ float4 dvec(float4 m2, float4 m1, float4 p1,
+static __device__ __forceinline__ float4 dvec(float4 m2, float4 m1, float4 p1,
                                        float4 p2) {
   return make_float4(dcoef(m2.x, m1.x, p1.x, p2.x),
                      dcoef(m2.y, m1.y, p1.y, p2.y),
@@ -49,7 +47,7 @@ struct Cell {
   int b, x, y, z0;
   bool ok;
 };
-__device__ __forceinline__ Cell cell_of(const GridDims& g) {
+static __device__ __forceinline__ Cell cell_of(const GridDims& g) {
   const long long tot = (long long)g.B * g.nx * g.ny * g.nq;
   const long long i =
       xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
@@ -67,7 +65,7 @@ __device__ __forceinline__ Cell cell_of(const GridDims& g) {
 }
 
 // D_z of the 4 cells z0 .. z0 + 3 of the row at p (row length nz)
-__device__ __forceinline__ float4 dz_row(const float* row, int z0, int nz) {
+static __device__ __forceinline__ float4 dz_row(const float* row, int z0, int nz) {
   const int zm = z0 == 0 ? nz - 2 : z0 - 2;
   const int zp = z0 + 4 == nz ? 0 : z0 + 4;
   const float2 a = ld2(row + zm);  // z0 - 2, z0 - 1
@@ -77,7 +75,7 @@ __device__ __forceinline__ float4 dz_row(const float* row, int z0, int nz) {
                      dcoef(c.x, c.y, c.w, e.x), dcoef(c.y, c.z, e.x, e.y));
 }
 
-__device__ __forceinline__ int wrap(int v, int n) {
+static __device__ __forceinline__ int wrap(int v, int n) {
   return v < 0 ? v + n : (v >= n ? v - n : v);
 }
 
@@ -85,7 +83,7 @@ __device__ __forceinline__ int wrap(int v, int n) {
 struct Nbr {
   long long ym2, ym1, yp1, yp2, xm2, xm1, xp1, xp2, c;
 };
-__device__ __forceinline__ Nbr nbr_of(const Cell& c, const GridDims& g) {
+static __device__ __forceinline__ Nbr nbr_of(const Cell& c, const GridDims& g) {
   const long long nz = g.nz;
   const long long plane = (long long)g.ny * nz;
   const long long xb = ((long long)c.b * g.nx + c.x) * plane;
@@ -162,16 +160,14 @@ __global__ __launch_bounds__(256) void k_grad_adj(const float* __restrict__ gx,
                   -((d.z + b.z) + a.z), -((d.w + b.w) + a.w));
 }
 
-GridDims grid_dims(const Plan& p) {
+static GridDims grid_dims(const Plan& p) {
   return GridDims{p.d.B, p.d.nx, p.d.ny, p.d.nz, p.d.nz / 4};
 }
 
-unsigned nblocks(const GridDims& g) {
+static unsigned nblocks(const GridDims& g) {
   const long long tot = (long long)g.B * g.nx * g.ny * g.nq;
   return (unsigned)((tot + 255) / 256);
 }
-
-}  // namespace
 
 bool grad_stencil_ok(const Plan& p) {
   return p.d.nz % 4 == 0 && p.d.nx >= 4 && p.d.ny >= 4 && p.d.nz >= 8;
