@@ -422,51 +422,6 @@ __global__ __launch_bounds__(T) void k_zpass_c2r(ZJobs jobs, PlanDev p) {
   }
 }
 
-// The three force-mesh rows of one chunk in one workgroup: jobs 0, 1, 2 are
-// transformed one after the other through the same LDS tile, their outputs
-// kept in registers, and each cell is written once as the float4
-// (job 0, job 1, job 2, 0) -- whole-line stores instead of three strided
-// partial ones.
-template <int N, int TW, int T>
-__global__ __launch_bounds__(T) void k_zpass_c2r_f4(ZJobs jobs, PlanDev p) {
-  constexpr int P = Pitch<TW>::P;
-  constexpr int NO = 2 * TW * N;  // real outputs per job
-  constexpr int EO = (NO + T - 1) / T;
-  __shared__ float2 s[N * P];
-  __shared__ float2 tw[N];
-  const long long lb = xcd_block(blockIdx.x, gridDim.x);
-  load_twiddles<N, T>(tw, p.ax[2].tw);
-  const long long r0 = lb * (2 * TW);
-  const int nr = (int)min((long long)(2 * TW), p.rows - r0);
-  float res[3][EO];
-#pragma unroll
-  for (int jb = 0; jb < 3; ++jb) {
-    const ZJob& J = jobs.j[jb];
-    if (jb > 0) __syncthreads();  // every read of the previous tile is done
-    c2r_load<N, TW, T>(s, J, p, r0, nr);
-    __syncthreads();
-    fft_tile<N, TW, T, +1>(s, tw);
-#pragma unroll
-    for (int u = 0; u < EO; ++u) {
-      const int q = threadIdx.x + u * T;
-      const int row = q / N, z = q % N;
-      float r = 0.0f;
-      if (NO % T == 0 || q < NO) {
-        const float2 v = s[z * P + (row >> 1)];
-        r = J.scale * ((row & 1) ? v.y : v.x);
-      }
-      res[jb][u] = r;
-    }
-  }
-  float4* __restrict__ dst = reinterpret_cast<float4*>(jobs.j[0].rout) + r0 * N;
-#pragma unroll
-  for (int u = 0; u < EO; ++u) {
-    const int q = threadIdx.x + u * T;
-    if ((NO % T == 0 || q < NO) && q / N < nr)
-      dst[q] = make_float4(res[0][u], res[1][u], res[2][u], 0.0f);
-  }
-}
-
 // ------------------------------------------------------- column tiles ----
 struct Tile {
   long long base;
@@ -815,32 +770,13 @@ static int zr2c_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   PMX_LAUNCH_CHECK();
   return kOk;
 }
-// jobs 0..2 writing components 0..2 of one float4 mesh, plain stores
-static bool is_f4_triplet(const ZJobs& jobs) {
-  if (jobs.n != 3) return false;
-  for (int d = 0; d < 3; ++d)
-    if (jobs.j[d].rout != jobs.j[0].rout || jobs.j[d].ostride != 4 ||
-        jobs.j[d].ocomp != d || jobs.j[d].accum)
-      return false;
-  return (reinterpret_cast<uintptr_t>(jobs.j[0].rout) & 15) == 0;
-}
-
 template <int N>
 static int zc2r_n(const Plan& pl, const ZJobs& jobs, hipStream_t s) {
   constexpr int TW = Cfg<N>::ZTW, T = Cfg<N>::T;
   ProfScope prof_(kProfZc2r, s);
-  if (is_f4_triplet(jobs)) {
-    constexpr int TW4 = TW >= 16 ? 8 : TW;
-    // measured: 256 threads beat 512 for the 256 x 8 tile of this pass
-    constexpr int T4 = (N * TW4 >= 4096) ? 512 : (N >= 256 ? 256 : 128);
-    dim3 grid((unsigned)((pl.d.rows + 2 * TW4 - 1) / (2 * TW4)));
-    hipLaunchKernelGGL((k_zpass_c2r_f4<N, TW4, T4>), grid, dim3(T4), 0, s,
-                       jobs, pl.d);
-  } else {
-    dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)) * jobs.n);
-    hipLaunchKernelGGL((k_zpass_c2r<N, TW, T>), grid, dim3(T), 0, s, jobs,
-                       pl.d);
-  }
+  dim3 grid((unsigned)((pl.d.rows + 2 * TW - 1) / (2 * TW)) * jobs.n);
+  hipLaunchKernelGGL((k_zpass_c2r<N, TW, T>), grid, dim3(T), 0, s, jobs,
+                     pl.d);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

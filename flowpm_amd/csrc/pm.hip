@@ -198,8 +198,7 @@ static inline Term with_pgd(Term t, const PgdTerm* pgd) {
 static int poisson_grad(const Plan& p, const float2* spec, int fwd,
                         float2* w0, float2* w1, float2* w2, float* fx,
                         float* fy, float* fz, float scale, hipStream_t s,
-                        float* f4 = nullptr, const PgdTerm* pgd = nullptr,
-                        double lr_r2 = 0.0) {
+                        const PgdTerm* pgd = nullptr, double lr_r2 = 0.0) {
   XJob xj{};
   xj.nin = 1;
   xj.nout = 2;
@@ -231,14 +230,9 @@ static int poisson_grad(const Plan& p, const float2* spec, int fwd,
   PMX_TRY(launch_ypass(p, yj, 1, s));
   ZJobs zj{};
   zj.n = 3;
-  zj.j[0] = zc2r(w1, f4 ? f4 : fx, 1.0f, 0);
-  zj.j[1] = zc2r(w2, f4 ? f4 : fy, 1.0f, 0);
-  zj.j[2] = zc2r(w0, f4 ? f4 : fz, 1.0f, 1);
-  if (f4)
-    for (int d = 0; d < 3; ++d) {
-      zj.j[d].ostride = 4;
-      zj.j[d].ocomp = d;
-    }
+  zj.j[0] = zc2r(w1, fx, 1.0f, 0);
+  zj.j[1] = zc2r(w2, fy, 1.0f, 0);
+  zj.j[2] = zc2r(w0, fz, 1.0f, 1);
   PMX_TRY(launch_zpass_c2r(p, zj, s));
   return kOk;
 }
@@ -274,19 +268,15 @@ static int force_mesh_f4(const Plan& p, const float* rho, float2* w,
   return launch_grad_f4(p, phi, f4, s);
 }
 
-// phi (optional): a real scratch mesh; with it and f4 the force mesh comes
-// from one potential and the real-space gradient (force_mesh_f4)
+// Three separate force meshes (the drop-in force / apply_longrange path).
 static int force_meshes(const Plan& p, const float* rho, float2* w, float* fx,
                         float* fy, float* fz, float rho_scale, hipStream_t s,
-                        float* f4 = nullptr, const PgdTerm* pgd = nullptr,
-                        float* phi = nullptr) {
-  if (f4 && phi && grad_stencil_ok(p))
-    return force_mesh_f4(p, rho, w, phi, f4, rho_scale, s, pgd);
+                        const PgdTerm* pgd = nullptr) {
   float2* w0 = w;
   float2* w1 = w + p.d.S;
   float2* w2 = w + 2 * p.d.S;
   PMX_TRY(r2c_zy(p, &rho, &w0, 1, rho_scale, s));
-  return poisson_grad(p, w0, 1, w0, w1, w2, fx, fy, fz, 1.0f, s, f4, pgd);
+  return poisson_grad(p, w0, 1, w0, w1, w2, fx, fy, fz, 1.0f, s, pgd);
 }
 
 // Adjoint of the map (g_x, g_y, g_z) <- force_meshes(rho):
@@ -649,8 +639,8 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
   // the density buffer takes the potential once it is transformed
-  return force_meshes(p, nw.rho, nw.w, nullptr, nullptr, nullptr, sc, s,
-                      nw.f4, pgd, nw.rho);
+  PMX_REQUIRE(grad_stencil_ok(p), "force: mesh sides must be multiples of 4");
+  return force_mesh_f4(p, nw.rho, nw.w, nw.rho, nw.f4, sc, s, pgd);
 }
 
 // Where a run leaves what the reverse sweep needs of each force evaluation
@@ -1248,7 +1238,7 @@ int pmx_fft_c2r_longrange_grad(const pmx_plan* plan, const float* dk,
   float2* w = reinterpret_cast<float2*>(work);
   return poisson_grad(p, reinterpret_cast<const float2*>(dk), 0, w,
                       w + p.d.S, w + 2 * p.d.S, fx, fy, fz, scale, S(stream),
-                      nullptr, nullptr, r_split * r_split);
+                      nullptr, r_split * r_split);
 }
 
 int pmx_force_meshes(const pmx_plan* plan, const float* rho, float* work,
