@@ -1250,6 +1250,19 @@ int pmx_force_meshes(const pmx_plan* plan, const float* rho, float* work,
                       fz, rho_scale, S(stream));
 }
 
+int pmx_force_mesh_f4(const pmx_plan* plan, const float* rho, float* work,
+                      float* phi, float* f4, float rho_scale, void* stream) {
+  PMX_REQUIRE(plan && rho && work && phi && f4,
+              "pmx_force_mesh_f4: null pointer");
+  PMX_REQUIRE((reinterpret_cast<uintptr_t>(f4) & 15) == 0,
+              "pmx_force_mesh_f4: f4 must be 16-byte aligned");
+  const Plan& p = P(plan);
+  PMX_REQUIRE(grad_stencil_ok(p),
+              "pmx_force_mesh_f4: mesh sides must be multiples of 4");
+  return force_mesh_f4(p, rho, reinterpret_cast<float2*>(work), phi, f4,
+                       rho_scale, S(stream), nullptr);
+}
+
 int pmx_force_meshes_adj(const pmx_plan* plan, const float* gfx,
                          const float* gfy, const float* gfz, float* work,
                          float* grho, float rho_scale, void* stream) {

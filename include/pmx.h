@@ -107,8 +107,9 @@ int pmx_fft_c2r_poisson_grad(const pmx_plan* plan, const float* dk,
                              float* work, float* fx, float* fy, float* fz,
                              float scale, void* stream);
 /* The same with the long-range filter of apply_longrange(split = r_split)
- * (kernels.py:81-102): the multiplier is laplace_kernel *
- * exp(-k^2 r_split^2), formed in float64 and rounded once. */
+ * (kernels.py:81-102): the multiplier is laplace_kernel (float64, rounded
+ * to float) times exp(-k^2 r_split^2) (float64 exponent, float exp), as the
+ * reference multiplies the two after casting each to complex64. */
 int pmx_fft_c2r_longrange_grad(const pmx_plan* plan, const float* dk,
                                float* work, float* fx, float* fy, float* fz,
                                float scale, double r_split, void* stream);
@@ -118,6 +119,15 @@ int pmx_fft_c2r_longrange_grad(const pmx_plan* plan, const float* dk,
 int pmx_force_meshes(const pmx_plan* plan, const float* rho, float* work,
                      float* fx, float* fy, float* fz, float rho_scale,
                      void* stream);
+/* The force mesh of the time loop, interleaved (tfpm.py:344-354 with the
+ * three meshes in one float4 per cell): phi = c2r3d(r2c3d(rho * rho_scale)
+ * * laplace_kernel) into the real mesh phi (which may be rho itself), then
+ * f4[cell] = (D_x phi, D_y phi, D_z phi, 0), D the periodic fourth-order
+ * central difference 2/3 (x[n+1] - x[n-1]) - 1/12 (x[n+2] - x[n-2]) whose
+ * Fourier multiplier is exactly gradient_kernel (kernels.py:51-78).
+ * work: 1 spec buffer; nz % 4 == 0; f4 16-byte aligned. */
+int pmx_force_mesh_f4(const pmx_plan* plan, const float* rho, float* work,
+                      float* phi, float* f4, float rho_scale, void* stream);
 /* Its adjoint: grho = sum_d T_d^T g_d, T_d^T = c2r(conj(lap i g_d) r2c(.)),
  * the conjugate transform (tfpm.py:183-219 reversed). */
 int pmx_force_meshes_adj(const pmx_plan* plan, const float* gfx,

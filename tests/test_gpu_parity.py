@@ -541,3 +541,28 @@ def test_gradient_64_20steps_against_oracle(gpu):
   print("64^3 x 20 steps gradient cosine %.7f rel %.3g" %
         (cs, rel_rms(wg.grad, wo.grad)))
   assert cs > 0.9999
+
+
+@pytest.mark.parametrize("shape", [(2, 32, 16, 24), (1, 64, 64, 64)])
+def test_force_mesh_f4_matches_fourier_gradient(gpu, shape):
+  """The time loop's force mesh (one potential, real-space fourth-order
+  gradient) against the three Fourier-space gradient meshes of
+  pmx_force_meshes on the same density."""
+  from flowpm_amd import _lib, _plans
+  B, nx, ny, nz = shape
+  plan = _plans.get(gpu, B, nx, ny, nz)
+  gen = torch.Generator().manual_seed(sum(shape))
+  rho = (torch.rand(shape, generator=gen) * 2).to(gpu)
+  sc = 1.0 / (nx * ny * nz)
+  f3 = torch.empty((3,) + shape, device=gpu)
+  work = plan.spec(3)
+  _lib.call("pmx_force_meshes", plan.handle, _lib.ptr(rho), _lib.ptr(work),
+            _lib.ptr(f3[0]), _lib.ptr(f3[1]), _lib.ptr(f3[2]), sc,
+            _lib.stream_of(rho))
+  f4 = torch.empty(shape + (4,), device=gpu)
+  phi = torch.empty(shape, device=gpu)
+  _lib.call("pmx_force_mesh_f4", plan.handle, _lib.ptr(rho), _lib.ptr(work),
+            _lib.ptr(phi), _lib.ptr(f4), sc, _lib.stream_of(rho))
+  for d in range(3):
+    close(f4[..., d], f3[d], 2e-6, "component %d" % d)
+  assert float(f4[..., 3].abs().max()) == 0.0

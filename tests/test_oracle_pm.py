@@ -251,3 +251,21 @@ def test_pgd_limits_and_momentum():
   shift = (out[0] - x)[0]
   assert shift.abs().max() > 1e-3
   assert shift.sum(0).abs().max() < 1e-10 * shift.abs().sum()
+
+
+def test_gradient_kernel_is_fourth_order_central_difference():
+  """gradient_kernel (kernels.py:51-78) multiplies mode k by
+  i (8 sin w - sin 2w) / 6; under numpy's forward transform that is the
+  transfer function of the periodic stencil
+  2/3 (x[n+1] - x[n-1]) - 1/12 (x[n+2] - x[n-2]), which the GPU time loop
+  applies in real space."""
+  rng = np.random.default_rng(0)
+  for n in ([16, 12, 20], [8, 8, 8], [32, 24, 16]):
+    phi = rng.normal(size=n)
+    kvec = pm.fftk(n, symmetric=False)
+    for d in range(3):
+      g = pm.gradient_kernel(kvec, d)
+      ref = np.real(np.fft.ifftn(np.fft.fftn(phi) * g))
+      st = (2 / 3) * (np.roll(phi, -1, d) - np.roll(phi, 1, d)) - (1 / 12) * (
+          np.roll(phi, -2, d) - np.roll(phi, 2, d))
+      np.testing.assert_allclose(st, ref, atol=1e-13 * np.abs(ref).max())
