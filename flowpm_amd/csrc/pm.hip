@@ -537,6 +537,37 @@ __global__ void k_spec_mul_real(float2* __restrict__ spec,
   }
 }
 
+// amp[mode] = table[mx^2 + my^2 + mz^2], m_d the signed frequency index
+// of the mode (numpy fftfreq order), over the half-spectrum layout of one
+// mesh (main then Nyquist plane): P(k) evaluated once per distinct |k| of a
+// box with equal sides instead of once per mode.
+__global__ void k_amp_from_table(const float* __restrict__ table,
+                                 long long ntable, float* __restrict__ amp,
+                                 PlanDev p) {
+  const long long per_main = (long long)p.nx * p.ny * p.H;
+  const long long tot = per_main + (long long)p.nx * p.ny;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       e < tot; e += (long long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    if (e < per_main) {
+      z = (int)(e % p.H);
+      const long long r = e / p.H;
+      y = (int)(r % p.ny);
+      x = (int)(r / p.ny);
+    } else {
+      const long long r = e - per_main;
+      y = (int)(r % p.ny);
+      x = (int)(r / p.ny);
+      z = p.H;
+    }
+    const long long mx = x < (p.nx + 1) / 2 ? x : x - p.nx;
+    const long long my = y < (p.ny + 1) / 2 ? y : y - p.ny;
+    const long long mz = z < (p.nz + 1) / 2 ? z : z - p.nz;
+    const long long sq = mx * mx + my * my + mz * mz;
+    amp[e] = sq < ntable ? table[sq] : __int_as_float(0x7fc00000);
+  }
+}
+
 static inline unsigned gridfor(long long n) {
   return (unsigned)std::max<long long>(
       1, std::min<long long>((n + 255) / 256, 16384));
@@ -1441,6 +1472,23 @@ int pmx_power_spectrum_bins(const pmx_plan* plan, const float* spec,
               "pmx_power_spectrum_bins: null pointer");
   return launch_power_bins(P(plan), reinterpret_cast<const float2*>(spec), kx,
                            ky, kz, edges, nedges, psum, S(stream));
+}
+
+int pmx_amp_from_table(const pmx_plan* plan, const float* table,
+                       long long ntable, float* amp, void* stream) {
+  PMX_REQUIRE(plan && table && amp, "pmx_amp_from_table: null pointer");
+  const PlanDev& d = P(plan).d;
+  const long long need = (long long)(d.nx / 2) * (d.nx / 2) +
+                         (long long)(d.ny / 2) * (d.ny / 2) +
+                         (long long)(d.nz / 2) * (d.nz / 2) + 1;
+  PMX_REQUIRE(ntable >= need,
+              "pmx_amp_from_table: table shorter than the largest mx^2 + my^2 "
+              "+ mz^2 + 1");
+  const long long tot = (long long)d.nx * d.ny * (d.H + 1);
+  hipLaunchKernelGGL(k_amp_from_table, dim3(gridfor(tot)), dim3(256), 0,
+                     S(stream), table, ntable, amp, d);
+  PMX_LAUNCH_CHECK();
+  return 0;
 }
 
 int pmx_spec_mul_real(const pmx_plan* plan, float* spec, const float* amp,

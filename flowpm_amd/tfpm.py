@@ -114,12 +114,34 @@ def whitenoise_to_linear(nc, boxsize, white, pk, kvec=None, name=None):
   nc = _nc3(nc)
   if np.isscalar(boxsize):
     boxsize = [float(boxsize)] * 3
-  kmesh = _half_kmesh(nc, boxsize)
-  pkv = np.asarray(pk(kmesh), dtype=np.float64).reshape(-1)
-  amp = (pkv / (boxsize[0] * boxsize[1] * boxsize[2]))**0.5
-  amp = torch.as_tensor(amp.astype(np.float32), device=white.device)
   _lib.require_cuda(white)
+  amp = _linear_amp(nc, [float(b) for b in boxsize], pk, white.device)
   return _WhiteToLinear.apply(white, amp)
+
+
+def _linear_amp(nc, boxsize, pk, device):
+  """sqrt(pk(|k|) / V) over the half spectrum of one mesh, float32 on the
+  device (tfpm.py:56-62).  With equal box sides |k| = 2 pi sqrt(s) / L,
+  s = mx^2 + my^2 + mz^2, so pk is evaluated once per s (at most
+  3 (n/2)^2 + 1 values instead of n^3 / 2 modes: 49153 against 8.4 M at
+  256^3) and the table is expanded over the modes on the GPU; other boxes
+  evaluate pk on every mode on the host."""
+  V = boxsize[0] * boxsize[1] * boxsize[2]
+  smax = sum((n // 2)**2 for n in nc)
+  modes = nc[0] * nc[1] * (nc[2] // 2 + 1)
+  if boxsize[0] == boxsize[1] == boxsize[2] and smax + 1 < modes:
+    s = np.arange(smax + 1, dtype=np.float64)
+    k = 2 * np.pi * np.sqrt(s) / boxsize[0]
+    tab = (np.asarray(pk(k), dtype=np.float64).reshape(-1) / V)**0.5
+    tab = torch.as_tensor(tab.astype(np.float32), device=device)
+    amp = torch.empty(modes, dtype=torch.float32, device=device)
+    plan = _plans.get(device, 1, *nc)
+    _lib.call("pmx_amp_from_table", plan.handle, _lib.ptr(tab), smax + 1,
+              _lib.ptr(amp), _lib.stream_of(amp))
+    return amp
+  pkv = np.asarray(pk(_half_kmesh(nc, boxsize)), dtype=np.float64).reshape(-1)
+  amp = (pkv / V)**0.5
+  return torch.as_tensor(amp.astype(np.float32), device=device)
 
 
 def linear_field(nc, boxsize, pk, kvec=None, batch_size=1, seed=None,

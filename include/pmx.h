@@ -202,6 +202,15 @@ int pmx_power_spectrum_bins(const pmx_plan* plan, const float* spec,
                             const double* kz, const double* edges, int nedges,
                             double* psum, void* stream);
 
+/* The linear-field amplitude of a box with equal sides (tfpm.py:56-62 with
+ * |k| = 2 pi sqrt(mx^2 + my^2 + mz^2) / L): amp[mode] = table[mx^2 + my^2 +
+ * mz^2] over the half-spectrum layout below, m_d the signed frequency index
+ * (numpy fftfreq order).  table[s] = sqrt(P(2 pi sqrt(s) / L) / V) is
+ * evaluated once per distinct |k| on the host; ntable must exceed the
+ * largest (nx/2)^2 + (ny/2)^2 + (nz/2)^2.  No counterpart in the reference
+ * (which evaluates pk on every mode). */
+int pmx_amp_from_table(const pmx_plan* plan, const float* table,
+                       long long ntable, float* amp, void* stream);
 /* spec *= amp, amp a real half-spectrum array of one mesh (nx*ny*nz/2 main
  * values then nx*ny Nyquist-plane values), applied to every batch member:
  * the sqrt(P(k)/V) weighting of tfpm.py:56-62. */

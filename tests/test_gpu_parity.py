@@ -566,3 +566,20 @@ def test_force_mesh_f4_matches_fourier_gradient(gpu, shape):
   for d in range(3):
     close(f4[..., d], f3[d], 2e-6, "component %d" % d)
   assert float(f4[..., 3].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("nc,box", [([32, 32, 32], 100.0),
+                                    ([16, 24, 32], 60.0),
+                                    ([64, 64, 64], 200.0)])
+def test_linear_amplitude_table_matches_per_mode(gpu, nc, box):
+  """The equal-sides shortcut of the linear-field amplitude (P(k) once per
+  distinct |k|, expanded on the GPU) against pk evaluated on every mode of
+  the half spectrum (tfpm.py:56-62)."""
+  _, pk = _ics(8, 1.0, 0)
+  amp = fa.tfpm._linear_amp(nc, [box] * 3, pk, gpu).cpu().double().numpy()
+  km = fa.tfpm._half_kmesh(nc, [box] * 3)
+  ref = (np.asarray(pk(km), dtype=np.float64) / box**3)**0.5
+  ref32 = ref.astype(np.float32).astype(np.float64)
+  assert amp.shape == ref.shape
+  err = np.abs(amp - ref32) / np.abs(ref32).max()
+  assert err.max() < 1e-6, err.max()
