@@ -49,6 +49,7 @@ _SIGS = {
     "pmx_force_meshes_adj": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _vp]),
     "pmx_force_mesh_f4": (_i, [_vp, _vp, _vp, _vp, _vp, _f, _vp]),
     "pmx_amp_from_table": (_i, [_vp, _vp, _ll, _vp, _vp]),
+    "pmx_power_spectrum_counts": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "pmx_fft_poisson_grad_adj": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _vp]),
     "pmx_lpt2_source": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "pmx_lpt2_work_floats": (_ll, [_vp]),

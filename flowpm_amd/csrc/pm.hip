@@ -1474,6 +1474,16 @@ int pmx_power_spectrum_bins(const pmx_plan* plan, const float* spec,
                            ky, kz, edges, nedges, psum, S(stream));
 }
 
+int pmx_power_spectrum_counts(const pmx_plan* plan, const double* kx,
+                              const double* ky, const double* kz,
+                              const double* edges, int nedges, double* nsum,
+                              void* stream) {
+  PMX_REQUIRE(plan && kx && ky && kz && edges && nsum,
+              "pmx_power_spectrum_counts: null pointer");
+  return launch_power_bins(P(plan), nullptr, kx, ky, kz, edges, nedges, nsum,
+                           S(stream), 1);
+}
+
 int pmx_amp_from_table(const pmx_plan* plan, const float* table,
                        long long ntable, float* amp, void* stream) {
   PMX_REQUIRE(plan && table && amp, "pmx_amp_from_table: null pointer");

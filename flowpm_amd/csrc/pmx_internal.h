@@ -278,7 +278,7 @@ int launch_white_noise(float* out, long long n, unsigned long long k0,
 constexpr int kMaxPkBins = 8192;
 int launch_power_bins(const Plan& p, const float2* spec, const double* kx,
                       const double* ky, const double* kz, const double* edges,
-                      int nedges, double* psum, hipStream_t s);
+                      int nedges, double* psum, hipStream_t s, int count = 0);
 
 // Lagrangian-brick paint (paint.hip).
 struct BrickGrid {

@@ -11,7 +11,9 @@
 //   j = 0:          W(0)
 //   0 < j < nz/2:   W(j) + W(nz - j)
 //   j = nz/2:       W(nz/2)
-// which reproduces the reference's sums exactly.
+// which reproduces the reference's sums exactly.  With count = 1 the same
+// pass sums W alone: the reference's weighted mode count Nsum (pk.py:32-35,
+// a bincount over the full spectrum), exact in float64.
 //
 // |k| = sqrt((kx^2 + ky^2) + kz^2) is evaluated in float64 without
 // contraction from the same per-axis tables numpy builds, so the bin of
@@ -56,6 +58,7 @@ __global__ void k_zero_d(double* p, long long n) {
   if (i < n) p[i] = 0.0;
 }
 
+template <bool COUNT>
 __global__ __launch_bounds__(256) void k_pk_bin(PkArgs a) {
   // numpy rounds every product and sum: no fused multiply-adds here
 #pragma clang fp contract(off)
@@ -76,14 +79,15 @@ __global__ __launch_bounds__(256) void k_pk_bin(PkArgs a) {
     int cur = -1;
     double acc = 0.0;
     for (int j = 0; j <= H; ++j) {
-      const float2 v = j < H ? src[j] : a.spec[d.nyq0 + row];
+      float2 v = make_float2(0.f, 0.f);
+      if constexpr (!COUNT) v = j < H ? src[j] : a.spec[d.nyq0 + row];
       const double kk = kxy + a.kz[j] * a.kz[j];
       const int bin = digitize(sqrt(kk), a.edges, a.nedges);
       int w = pk_w(j, d.nz);
       if (j > 0 && j < H) w += pk_w(d.nz - j, d.nz);
       // float32 as the reference forms it: real(F conj F) times W
       const float p2 = v.x * v.x + v.y * v.y;
-      const float pw = (float)w * p2;
+      const float pw = COUNT ? (float)w : (float)w * p2;
       if (bin != cur) {
         if (cur >= 0) atomicAdd(&bins[cur], acc);
         cur = bin;
@@ -101,10 +105,11 @@ __global__ __launch_bounds__(256) void k_pk_bin(PkArgs a) {
 
 int launch_power_bins(const Plan& p, const float2* spec, const double* kx,
                       const double* ky, const double* kz, const double* edges,
-                      int nedges, double* psum, hipStream_t s) {
+                      int nedges, double* psum, hipStream_t s, int count) {
   PMX_REQUIRE(nedges >= 1 && nedges + 1 <= kMaxPkBins,
               "power spectrum: 1 <= number of edges < 8192");
-  const long long nout = (long long)p.d.B * (nedges + 1);
+  const int nb_members = count ? 1 : p.d.B;
+  const long long nout = (long long)nb_members * (nedges + 1);
   hipLaunchKernelGGL(k_zero_d, dim3((unsigned)((nout + 255) / 256)),
                      dim3(256), 0, s, psum, nout);
   PMX_LAUNCH_CHECK();
@@ -119,7 +124,8 @@ int launch_power_bins(const Plan& p, const float2* spec, const double* kx,
   a.d = p.d;
   const long long rows = (long long)p.d.nx * p.d.ny;
   const unsigned g = (unsigned)std::min<long long>((rows + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_pk_bin, dim3(g, p.d.B), dim3(256),
+  hipLaunchKernelGGL(count ? k_pk_bin<true> : k_pk_bin<false>,
+                     dim3(g, nb_members), dim3(256),
                      sizeof(double) * (nedges + 1), s, a);
   PMX_LAUNCH_CHECK();
   return kOk;

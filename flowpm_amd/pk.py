@@ -1,12 +1,15 @@
 """Power spectrum (flowpm/pk.py:7-103) on the GPU.
 
 ``power_spectrum(field, kmin, dk, boxsize)`` keeps the reference's
-signature and its arithmetic: the host builds the bin edges and the
-weighted mode counts exactly as ``_initialize_pk`` does (NumPy float64), the
-HIP library transforms the field (unnormalised half spectrum) and sums
-W |F|^2 per bin (``pmx_power_spectrum_bins``, the reference's full-spectrum
-weights accounted on the half spectrum), and the normalisation follows the
-reference's float32 steps.  Not differentiable (the reference's bincount
+signature and its arithmetic: the host builds the bin edges and per-axis
+wavenumbers as ``_initialize_pk`` does (NumPy float64), the HIP library
+transforms the field (unnormalised half spectrum) and sums W |F|^2 per bin
+(``pmx_power_spectrum_bins``, the reference's full-spectrum weights
+accounted on the half spectrum); the weighted mode counts come from the
+same binning pass over W alone (``pmx_power_spectrum_counts``, exact
+integers in float64, equal to the reference's bincount) instead of a
+full-mesh NumPy digitize; the normalisation follows the reference's float32
+steps.  Not differentiable (the reference's bincount
 path is used as a diagnostic / loss on fixed bins; this one returns a
 detached tensor).
 """
@@ -20,9 +23,21 @@ from .utils import _f32, _mesh_dims, fwd_real
 __all__ = ["power_spectrum"]
 
 
+def _edges(shape, boxsize, kmin, dk):
+  """Per-axis wavenumbers and bin edges (pk.py:7-37), NumPy float64."""
+  shape = [int(s) for s in shape]
+  boxsize = np.asarray(boxsize, dtype=np.float64)
+  kmax = np.pi * np.min(shape) / np.max(boxsize) + dk / 2
+  kedges = np.arange(kmin, kmax, dk)
+  k = [np.fft.fftfreq(N, 1. / (N * 2 * np.pi / L)) for N, L in
+       zip(shape, boxsize)]
+  return k, kedges
+
+
 def _initialize_pk(shape, boxsize, kmin, dk):
   """Bin edges, per-axis wavenumbers and weighted mode counts
-  (pk.py:7-37), NumPy float64."""
+  (pk.py:7-37), NumPy float64, on the host over the full mesh (the
+  reference's own procedure; power_spectrum counts on the GPU)."""
   shape = [int(s) for s in shape]
   boxsize = np.asarray(boxsize, dtype=np.float64)
   W = np.empty(shape, dtype="f4")
@@ -51,7 +66,7 @@ def power_spectrum(field, kmin=5, dk=0.5, boxsize=False):
     raise ValueError("power_spectrum needs the box size per dimension "
                      "(boxsize.prod() normalises the power, pk.py:90)")
   boxsize = np.broadcast_to(np.asarray(boxsize, dtype=np.float64), (3,))
-  k, kedges, Nsum = _initialize_pk((nx, ny, nz), boxsize, kmin, dk)
+  k, kedges = _edges((nx, ny, nz), boxsize, kmin, dk)
   dev = field.device
   if len(kedges) < 2:  # no complete bin: empty, as the reference returns
     return kedges[:0].astype(np.float64), torch.zeros(
@@ -65,10 +80,13 @@ def power_spectrum(field, kmin=5, dk=0.5, boxsize=False):
   _lib.call("pmx_power_spectrum_bins", plan.handle, _lib.ptr(spec),
             _lib.ptr(kx), _lib.ptr(ky), _lib.ptr(kz), _lib.ptr(edges),
             len(kedges), _lib.ptr(psum), _lib.stream_of(field))
+  nsum = torch.empty(len(kedges) + 1, dtype=torch.float64, device=dev)
+  _lib.call("pmx_power_spectrum_counts", plan.handle, _lib.ptr(kx),
+            _lib.ptr(ky), _lib.ptr(kz), _lib.ptr(edges), len(kedges),
+            _lib.ptr(nsum), _lib.stream_of(field))
   # float32 steps of pk.py:82-103: Psum (complex64) / Nsum (complex64),
   # [1:-1], * boxsize.prod(), / float32(M)**2
-  nsum = torch.as_tensor(Nsum.astype(np.float32), device=dev)
-  power = psum.to(torch.float32) / nsum
+  power = psum.to(torch.float32) / nsum.to(torch.float32)
   power = power[:, 1:-1] * np.float32(boxsize.prod())
   norm = np.float32(nx * ny * nz)**2
   kbins = kedges[:-1] + (kedges[1:] - kedges[:-1]) / 2

@@ -201,6 +201,13 @@ int pmx_power_spectrum_bins(const pmx_plan* plan, const float* spec,
                             const double* kx, const double* ky,
                             const double* kz, const double* edges, int nedges,
                             double* psum, void* stream);
+/* The weighted mode count of the same bins (pk.py:32-35: bincount of the
+ * full-spectrum digitize with the weights W), one array nsum[nedges + 1]
+ * (float64, exact), without touching any field. */
+int pmx_power_spectrum_counts(const pmx_plan* plan, const double* kx,
+                              const double* ky, const double* kz,
+                              const double* edges, int nedges, double* nsum,
+                              void* stream);
 
 /* The linear-field amplitude of a box with equal sides (tfpm.py:56-62 with
  * |k| = 2 pi sqrt(mx^2 + my^2 + mz^2) / L): amp[mode] = table[mx^2 + my^2 +
