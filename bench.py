@@ -258,7 +258,7 @@ _KIND_KERNEL = {
     "paint_gather": r"k_brick_gather<1>", "fft_z_r2c": r"k_zpass_r2c<",
     "fft_y_fwd": r"k_ypass<\d+, 16, \d+, -1, false>",
     "fft_x_fused": r"k_xpass<\d+, 16, ", "fft_y_inv": r"k_ypass<\d+, 16, \d+, 1, false>",
-    "fft_z_c2r": r"k_zpass_c2r<\d+, 16, ", "grad": r"k_grad_f4", "fill": r"k_fill",
+    "fft_z_c2r": r"k_zpass_c2r<\d+, 16, ", "grad": r"k_grad_mesh<", "fill": r"k_fill",
 }
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "pm_step_256_hbm_traffic.md")
 

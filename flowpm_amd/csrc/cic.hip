@@ -40,13 +40,24 @@ __device__ __forceinline__ Cic1 cic1(float x, int n) {
   return c;
 }
 
-template <bool I4>
-__device__ __forceinline__ float3 load3(const Mesh3& m, long long cell) {
-  if (I4) {
+// force mesh layouts: kMeshSep three meshes, kMeshF3 packed (x, y, z) per
+// cell, kMeshF4 float4 (x, y, z, -) per cell
+constexpr int kMeshSep = 0, kMeshF3 = 3, kMeshF4 = 4;
+template <int L>
+__device__ __forceinline__ float3 load_cell(const Mesh3& m, long long cell) {
+  if (L == kMeshF4) {
     const float4 v = m.f4[cell];
     return make_float3(v.x, v.y, v.z);
   }
+  if (L == kMeshF3) {
+    const float* q = m.f3 + 3 * cell;
+    return make_float3(q[0], q[1], q[2]);
+  }
   return make_float3(m.x[cell], m.y[cell], m.z[cell]);
+}
+template <bool I4>
+__device__ __forceinline__ float3 load3(const Mesh3& m, long long cell) {
+  return load_cell<I4 ? kMeshF4 : kMeshSep>(m, cell);
 }
 
 __device__ __forceinline__ bool pos_ok(float x, float y, float z) {
@@ -260,7 +271,7 @@ __device__ __forceinline__ long long stage_off(bool tiled, long long p0,
   return 3 * (p0 + (long long)r * gz) + o;
 }
 
-template <bool I4, bool P2F>
+template <int L, bool P2F>
 __global__ __launch_bounds__(256) void k_readout_kick_drift(
     float* __restrict__ X, float* __restrict__ P, float* __restrict__ F,
     Mesh3 m, MeshDims d, KickDrift c) {
@@ -315,7 +326,7 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
     if (pos_ok(x, y, z)) {
       const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
       float3 v[8];
-      PMX_CORNERS({ v[cc] = load3<I4>(m, cell); })
+      PMX_CORNERS({ v[cc] = load_cell<L>(m, cell); })
       PMX_CORNERS({
         const float k = (wx * wy) * wz;
         ax += v[cc].x * k;
@@ -705,10 +716,12 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
   if (!grid_tiled(c.gx, c.gy, c.gz, d.N)) cc.gx = cc.gy = cc.gz = 0;
   int ex;
   const bool p2 = c.f > 0.0f && std::frexp(c.f, &ex) == 0.5f;
-  auto kern = m.f4 ? (p2 ? k_readout_kick_drift<true, true>
-                         : k_readout_kick_drift<true, false>)
-                   : (p2 ? k_readout_kick_drift<false, true>
-                         : k_readout_kick_drift<false, false>);
+  auto kern = m.f4 ? (p2 ? k_readout_kick_drift<kMeshF4, true>
+                         : k_readout_kick_drift<kMeshF4, false>)
+              : m.f3 ? (p2 ? k_readout_kick_drift<kMeshF3, true>
+                           : k_readout_kick_drift<kMeshF3, false>)
+                     : (p2 ? k_readout_kick_drift<kMeshSep, true>
+                           : k_readout_kick_drift<kMeshSep, false>);
   hipLaunchKernelGGL(kern, dim3(nblk(tot, 256)), dim3(256), 0, s, X, P, F, m,
                      d, cc);
   PMX_LAUNCH_CHECK();
@@ -721,6 +734,7 @@ int launch_readout_dot(const float* X, const float* G, const Mesh3& m,
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   ProfScope prof_(kProfAdjCic, s);
+  PMX_REQUIRE(!m.f3, "readout_dot: no packed-mesh variant");
   if (m.f4)
     hipLaunchKernelGGL(k_readout_dot<true>, dim3(nblk(tot, 256)), dim3(256),
                        0, s, X, G, m, d, f, out, scale);
@@ -736,6 +750,7 @@ int launch_pgd_shift(float* X, const Mesh3& m, const MeshDims& d, float f,
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   ProfScope prof_(kProfReadout, s);
+  PMX_REQUIRE(!m.f3, "pgd_shift: no packed-mesh variant");
   if (m.f4)
     hipLaunchKernelGGL(k_pgd_shift<true>, dim3(nblk(tot, 256)), dim3(256), 0,
                        s, X, m, d, f, c);
@@ -767,6 +782,7 @@ int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* gin,
   ProfScope prof_(kProfAdjCic, s);
   AdjUpdate uu = u;
   if (!grid_tiled(u.gx, u.gy, u.gz, d.N)) uu.gx = uu.gy = uu.gz = 0;
+  PMX_REQUIRE(!m.f3, "force_vjp_finish: no packed-mesh variant");
   if (m.f4)
     hipLaunchKernelGGL(k_force_vjp_finish<true>, dim3(nblk(tot, 256)),
                        dim3(256), 0, s, gX, gP, gF, gin ? gin : gF, X, m,
@@ -786,6 +802,7 @@ int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz, float* dX,
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   ProfScope prof_(kProfAdjCic, s);
+  PMX_REQUIRE(!m.f3, "force_vjp_particles: no packed-mesh variant");
   if (m.f4)
     hipLaunchKernelGGL(k_force_vjp_particles<true>, dim3(nblk(tot, 256)),
                        dim3(256), 0, s, gmx, gmy, gmz, dX, gF, X, m, d, f, c,

@@ -237,13 +237,14 @@ static int poisson_grad(const Plan& p, const float2* spec, int fwd,
   return kOk;
 }
 
-// The interleaved force mesh f4 from one potential: phi = c2r(lap * r2c(rho)
+// The interleaved force mesh from one potential: phi = c2r(lap * r2c(rho)
 // [* PGD]) into the real scratch mesh phi, then the real-space gradient
 // (the reference's i g_d multiplier is the fourth-order central difference,
-// stencil.hip): one transform chain instead of three.
+// stencil.hip): one transform chain instead of three.  nc = 4: float4
+// cells (x, y, z, -), whose fourth slot the adjoint uses; nc = 3: packed.
 static int force_mesh_f4(const Plan& p, const float* rho, float2* w,
                          float* phi, float* f4, float rho_scale,
-                         hipStream_t s, const PgdTerm* pgd) {
+                         hipStream_t s, const PgdTerm* pgd, int nc = 4) {
   PMX_TRY(r2c_zy(p, &rho, &w, 1, rho_scale, s));
   XJob xj{};
   xj.nin = 1;
@@ -265,7 +266,7 @@ static int force_mesh_f4(const Plan& p, const float* rho, float2* w,
   zj.n = 1;
   zj.j[0] = zc2r(w, phi, 1.0f, 0);
   PMX_TRY(launch_zpass_c2r(p, zj, s));
-  return launch_grad_f4(p, phi, f4, s);
+  return launch_grad_mesh(p, phi, f4, nc, s);
 }
 
 // Three separate force meshes (the drop-in force / apply_longrange path).
@@ -590,7 +591,12 @@ static void particle_grid(const Plan& p, long long N, float f, int* gx,
 }
 
 static inline Mesh3 m3(const float* f4) {
-  return Mesh3{nullptr, nullptr, nullptr, reinterpret_cast<const float4*>(f4)};
+  return Mesh3{nullptr, nullptr, nullptr, reinterpret_cast<const float4*>(f4),
+               nullptr};
+}
+// force mesh of nc floats per cell (3: packed, 4: float4)
+static inline Mesh3 mesh_nc(const float* fm, int nc) {
+  return nc == 4 ? m3(fm) : Mesh3{nullptr, nullptr, nullptr, nullptr, fm};
 }
 
 struct NbodyWork {
@@ -655,10 +661,10 @@ static int paint_meshes(const Plan& p, float* const* mesh, int C,
 }
 
 // force meshes at X (positions * f), density scaled by 1/nbar and 1/M
-// (with pgd: the PGD meshes of the same density).
+// (with pgd: the PGD meshes of the same density); nc floats per cell.
 static int force_at(const Plan& p, const float* X, long long N, float f,
                     const NbodyWork& nw, hipStream_t s,
-                    const PgdTerm* pgd = nullptr) {
+                    const PgdTerm* pgd = nullptr, int nc = 4) {
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   (void)M;
@@ -671,7 +677,7 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
   const float sc = (float)(inv_nbar / (double)Mb);
   // the density buffer takes the potential once it is transformed
   PMX_REQUIRE(grad_stencil_ok(p), "force: mesh sides must be multiples of 4");
-  return force_mesh_f4(p, nw.rho, nw.w, nw.rho, nw.f4, sc, s, pgd);
+  return force_mesh_f4(p, nw.rho, nw.w, nw.rho, nw.f4, sc, s, pgd, nc);
 }
 
 // Where a run leaves what the reverse sweep needs of each force evaluation
@@ -731,10 +737,14 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     return base ? base + (long long)(i - tape.first) * BN3 : nullptr;
   };
   bool x_kept = false;  // X of the coming force is already in its slot
+  // a force mesh the tape keeps is float4 (the reverse sweep stores the
+  // density cotangent in its fourth slot); one only the readout reads is
+  // packed, 12 bytes per cell
   if (jump) {
     PMX_TRY(keep_x(tape.xj, X));
     const NbodyWork nw = at(tape.fj);
-    PMX_TRY(force_at(p, X, N, f, nw, s));
+    const int nc = tape.fj ? 4 : 3;
+    PMX_TRY(force_at(p, X, N, f, nw, s, nullptr, nc));
     kd.do_k2 = 0;
     kd.do_k1d = (b0 < b1);
     if (kd.do_k1d) {
@@ -743,7 +753,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
       kd.xcopy = xslot(b0);  // the drifted X goes to the tape as well
       x_kept = kd.xcopy != nullptr;
     }
-    PMX_TRY(launch_readout_kick_drift(X, P, F, m3(nw.f4), md,
+    PMX_TRY(launch_readout_kick_drift(X, P, F, mesh_nc(nw.f4, nc), md,
                                       kd, s));
   } else if (b0 < b1) {
     // opening kick + drift of the first step from the stored force
@@ -769,7 +779,8 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
       PMX_TRY(keep_x(xslot(i), X));
     }
     const NbodyWork nw = at(tape.fs ? tape.fs + slot * 4 * M : nullptr);
-    PMX_TRY(force_at(p, X, N, f, nw, s));
+    const int nc = tape.fs ? 4 : 3;
+    PMX_TRY(force_at(p, X, N, f, nw, s, nullptr, nc));
     kd.do_k2 = 1;
     kd.k2 = (float)coef[3 * i + 2];
     kd.do_k1d = (i + 1 < b1);
@@ -783,7 +794,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     }
     // only the last force of the run has to stay in the state
     kd.skip_f = kd.do_k1d;
-    PMX_TRY(launch_readout_kick_drift(X, P, F, m3(nw.f4), md,
+    PMX_TRY(launch_readout_kick_drift(X, P, F, mesh_nc(nw.f4, nc), md,
                                       kd, s));
   }
   return kOk;
@@ -1537,7 +1548,7 @@ int pmx_readout_kick_drift(float* X, float* P, float* F, const float* fx,
   c.d = d;
   c.do_k2 = flags & 1;
   c.do_k1d = (flags >> 1) & 1;
-  return launch_readout_kick_drift(X, P, F, Mesh3{fx, fy, fz, nullptr}, md, c,
+  return launch_readout_kick_drift(X, P, F, Mesh3{fx, fy, fz, nullptr, nullptr}, md, c,
                                    S(stream));
 }
 

@@ -212,10 +212,12 @@ struct MeshDims {
 };
 int launch_fill(float* p, long long n, float v, hipStream_t s);
 // Fourth-order central-difference gradient of a real mesh (stencil.hip):
-// f4 = (D_x phi, D_y phi, D_z phi, 0) per cell, and the adjoint
+// fm = (D_x phi, D_y phi, D_z phi, 0) per cell as float4 (nc = 4) or the
+// first three packed (nc = 3), and the adjoint
 // out = -(D_x gx + D_y gy + D_z gz).  Needs nz % 4 == 0.
 bool grad_stencil_ok(const Plan& p);
-int launch_grad_f4(const Plan& p, const float* phi, float* f4, hipStream_t s);
+int launch_grad_mesh(const Plan& p, const float* phi, float* fm, int nc,
+                     hipStream_t s);
 int launch_grad_adj(const Plan& p, const float* gx, const float* gy,
                     const float* gz, float* out, hipStream_t s);
 int launch_paint(float* mesh, const float* pos, const float* w, float wscale,
@@ -245,12 +247,14 @@ struct KickDrift {
   int gx, gy, gz;
   float* xcopy;   // if set (with do_k1d): drifted X also stored here
 };
-// The three force meshes: separate (x, y, z) or interleaved float4 (f4).
+// The three force meshes: separate (x, y, z), interleaved float4 (f4) or
+// interleaved packed (x, y, z) floats (f3).
 struct Mesh3 {
   const float* x;
   const float* y;
   const float* z;
   const float4* f4;
+  const float* f3;
 };
 int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
                               const MeshDims& d, const KickDrift& c,

@@ -10,7 +10,7 @@
 // written in Fourier space; at the Nyquist mode both are zero), so the three
 // force meshes are D_x, D_y, D_z of ONE inverse-transformed potential: one
 // transform chain instead of three, and this pass writes the interleaved
-// (F_x, F_y, F_z, -) mesh the particle passes gather.  D is antisymmetric,
+// (F_x, F_y, F_z[, -]) mesh the particle passes gather.  D is antisymmetric,
 // so the adjoint map from the three mesh cotangents to the potential's
 // cotangent is -(D_x g_x + D_y g_y + D_z g_z), also one pass.
 //
@@ -20,8 +20,15 @@
 
 namespace pmx {
 
+// Threads visit the cells in y bands: band of yb rows, then x, then band,
+// then member (yb = ny is plain C order).  A row's x neighbours are read
+// again 2 x steps later, i.e. 2 * yb rows later in this order; with whole
+// planes at 512^3 (1 MB per plane and mesh) that reuse distance no longer
+// fits one XCD's 4 MB L2.
 struct GridDims {
   int B, nx, ny, nz, nq;  // nq = nz / 4
+  int yb;                 // band height in rows, divides ny
+  unsigned tot;           // B * nx * ny * nq threads (< 2^31, host-checked)
 };
 
 static __device__ __forceinline__ float4 ld4(const float* p) {
@@ -47,21 +54,28 @@ struct Cell {
   int b, x, y, z0;
   bool ok;
 };
-static __device__ __forceinline__ Cell cell_of(const GridDims& g) {
-  const long long tot = (long long)g.B * g.nx * g.ny * g.nq;
-  const long long i =
-      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
+// index arithmetic in 32 bits: the launcher checks tot < 2^31
+static __device__ __forceinline__ Cell cell_at(const GridDims& g, unsigned i) {
   Cell c;
-  c.ok = i < tot;
-  const long long j = c.ok ? i : 0;
-  const int q = (int)(j % g.nq);
-  const long long r = j / g.nq;
-  c.y = (int)(r % g.ny);
-  const long long r2 = r / g.ny;
-  c.x = (int)(r2 % g.nx);
-  c.b = (int)(r2 / g.nx);
-  c.z0 = 4 * q;
+  c.ok = i < g.tot;
+  const unsigned j = c.ok ? i : 0u;
+  const unsigned q = j % (unsigned)g.nq;
+  unsigned r = j / (unsigned)g.nq;  // row in band order
+  const unsigned yy = r % (unsigned)g.yb;
+  r /= (unsigned)g.yb;
+  c.x = (int)(r % (unsigned)g.nx);
+  r /= (unsigned)g.nx;
+  const unsigned nband = (unsigned)(g.ny / g.yb);
+  c.y = (int)((r % nband) * (unsigned)g.yb + yy);
+  c.b = (int)(r / nband);
+  c.z0 = (int)(4 * q);
   return c;
+}
+static __device__ __forceinline__ unsigned block_first(void) {
+  return (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+}
+static __device__ __forceinline__ Cell cell_of(const GridDims& g) {
+  return cell_at(g, block_first() + threadIdx.x);
 }
 
 // D_z of the 4 cells z0 .. z0 + 3 of the row at p (row length nz)
@@ -102,14 +116,17 @@ static __device__ __forceinline__ Nbr nbr_of(const Cell& c, const GridDims& g) {
   return n;
 }
 
-// f4[cell] = (D_x phi, D_y phi, D_z phi, 0).  A thread's 4 cells are
-// consecutive, so its own 4 float4 stores would leave each store
-// instruction spread over 64-byte strides; they go through LDS instead and
-// every store instruction writes one contiguous 1 KB run per wave (measured
-// at 256^3: 78 us against 103).
-__global__ __launch_bounds__(256) void k_grad_f4(const float* __restrict__ phi,
-                                                 float4* __restrict__ f4,
-                                                 GridDims g) {
+// NC = 4: fm[cell] = (D_x phi, D_y phi, D_z phi, 0) as one float4;
+// NC = 3: (D_x phi, D_y phi, D_z phi) packed, 12 bytes per cell (the
+// forward step's mesh: a quarter fewer bytes written here and gathered by
+// the readout).  A thread's 4 cells are consecutive, so its own stores
+// would leave each store instruction spread over 4 * NC-float strides; they
+// go through LDS instead and every store instruction writes one contiguous
+// 1 KB run per wave (measured at 256^3 with NC = 4: 78 us against 103).
+template <int NC>
+__global__ __launch_bounds__(256) void k_grad_mesh(const float* __restrict__ phi,
+                                                   float* __restrict__ fm,
+                                                   GridDims g) {
   const Cell c = cell_of(g);
   float4 gx = make_float4(0.f, 0.f, 0.f, 0.f), gy = gx, gz = gx;
   if (c.ok) {
@@ -121,22 +138,34 @@ __global__ __launch_bounds__(256) void k_grad_f4(const float* __restrict__ phi,
     gx = dvec(ld4(phi + n.xm2 + z0), ld4(phi + n.xm1 + z0),
               ld4(phi + n.xp1 + z0), ld4(phi + n.xp2 + z0));
   }
-  __shared__ float4 st[256 * 4];
+  __shared__ float4 st[256 * NC];
   const int t = threadIdx.x;
-  st[4 * t] = make_float4(gx.x, gy.x, gz.x, 0.0f);
-  st[4 * t + 1] = make_float4(gx.y, gy.y, gz.y, 0.0f);
-  st[4 * t + 2] = make_float4(gx.z, gy.z, gz.z, 0.0f);
-  st[4 * t + 3] = make_float4(gx.w, gy.w, gz.w, 0.0f);
+  if (NC == 4) {
+    st[4 * t] = make_float4(gx.x, gy.x, gz.x, 0.0f);
+    st[4 * t + 1] = make_float4(gx.y, gy.y, gz.y, 0.0f);
+    st[4 * t + 2] = make_float4(gx.z, gy.z, gz.z, 0.0f);
+    st[4 * t + 3] = make_float4(gx.w, gy.w, gz.w, 0.0f);
+  } else {
+    // cells 0..3 of the thread: x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3
+    st[3 * t] = make_float4(gx.x, gy.x, gz.x, gx.y);
+    st[3 * t + 1] = make_float4(gy.y, gz.y, gx.z, gy.z);
+    st[3 * t + 2] = make_float4(gz.z, gx.w, gy.w, gz.w);
+  }
   __syncthreads();
-  // the block's threads cover 4 * 256 consecutive cells from its first one
-  const long long tot = (long long)g.B * g.nx * g.ny * g.nq;
-  const long long i0 =
-      xcd_block(blockIdx.x, gridDim.x) * (long long)blockDim.x;
-  const long long ncell = 4 * min((long long)blockDim.x, tot - i0);
+  // the block's threads cover 4 * 256 cells consecutive in memory from its
+  // first one (the host keeps every block inside one band row group):
+  // NC * 256 float4, 16-byte aligned (the first cell's offset is a multiple
+  // of 4 * 256, or of the row length when rows are longer)
+  const unsigned i0 = block_first();
+  const Cell c0 = cell_at(g, i0);
+  const long long cell0 =
+      (((long long)c0.b * g.nx + c0.x) * g.ny + c0.y) * g.nz + c0.z0;
+  const int nv = NC * (int)min(blockDim.x, g.tot - i0);
+  float4* out = reinterpret_cast<float4*>(fm + NC * cell0);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < NC; ++k) {
     const int q = t + 256 * k;
-    if (q < ncell) f4[4 * i0 + q] = st[q];
+    if (q < nv) out[q] = st[q];
   }
 }
 
@@ -160,26 +189,47 @@ __global__ __launch_bounds__(256) void k_grad_adj(const float* __restrict__ gx,
                   -((d.z + b.z) + a.z), -((d.w + b.w) + a.w));
 }
 
-static GridDims grid_dims(const Plan& p) {
-  return GridDims{p.d.B, p.d.nx, p.d.ny, p.d.nz, p.d.nz / 4};
+// Band height for nmesh input meshes: the largest divisor of ny whose
+// reuse distance (2 x steps either way: 4 * yb rows of each input mesh)
+// stays within 1 MB, among the heights that keep every 256-thread block
+// inside one (band, x) row group, so each block's cells stay consecutive in
+// memory; whole planes when they already fit.
+static GridDims grid_dims(const Plan& p, int nmesh) {
+  GridDims g{p.d.B, p.d.nx, p.d.ny, p.d.nz, p.d.nz / 4, p.d.ny, 0u};
+  g.tot = (unsigned)((long long)g.B * g.nx * g.ny * g.nq);
+  const long long budget = 1 << 20;
+  auto dist = [&](long long yb) { return 4 * yb * p.d.nz * 4LL * nmesh; };
+  if (dist(g.ny) <= budget) return g;
+  int rpb = 0;  // rows per block
+  if (g.nq >= 256 && g.nq % 256 == 0) rpb = 1;
+  else if (g.nq < 256 && 256 % g.nq == 0) rpb = 256 / g.nq;
+  if (rpb == 0 || g.ny % rpb != 0) return g;
+  int yb = rpb;  // smallest legal band if none fits the budget
+  for (int h = rpb; h < g.ny; h += rpb)
+    if (g.ny % h == 0 && dist(h) <= budget) yb = h;
+  g.yb = yb;
+  return g;
 }
 
-static unsigned nblocks(const GridDims& g) {
-  const long long tot = (long long)g.B * g.nx * g.ny * g.nq;
-  return (unsigned)((tot + 255) / 256);
-}
+static unsigned nblocks(const GridDims& g) { return (g.tot + 255u) / 256u; }
 
 bool grad_stencil_ok(const Plan& p) {
-  return p.d.nz % 4 == 0 && p.d.nx >= 4 && p.d.ny >= 4 && p.d.nz >= 8;
+  return p.d.nz % 4 == 0 && p.d.nx >= 4 && p.d.ny >= 4 && p.d.nz >= 8 &&
+         (long long)p.d.B * p.d.nx * p.d.ny * (p.d.nz / 4) < (1LL << 31);
 }
 
-int launch_grad_f4(const Plan& p, const float* phi, float* f4,
-                   hipStream_t s) {
+int launch_grad_mesh(const Plan& p, const float* phi, float* fm, int nc,
+                     hipStream_t s) {
   PMX_REQUIRE(grad_stencil_ok(p), "gradient stencil: unsupported mesh");
-  const GridDims g = grid_dims(p);
+  PMX_REQUIRE(nc == 3 || nc == 4, "gradient stencil: 3 or 4 floats per cell");
+  const GridDims g = grid_dims(p, 1);
   ProfScope prof_(kProfGrad, s);
-  hipLaunchKernelGGL(k_grad_f4, dim3(nblocks(g)), dim3(256), 0, s, phi,
-                     reinterpret_cast<float4*>(f4), g);
+  if (nc == 4)
+    hipLaunchKernelGGL(k_grad_mesh<4>, dim3(nblocks(g)), dim3(256), 0, s, phi,
+                       fm, g);
+  else
+    hipLaunchKernelGGL(k_grad_mesh<3>, dim3(nblocks(g)), dim3(256), 0, s, phi,
+                       fm, g);
   PMX_LAUNCH_CHECK();
   return kOk;
 }
@@ -187,7 +237,7 @@ int launch_grad_f4(const Plan& p, const float* phi, float* f4,
 int launch_grad_adj(const Plan& p, const float* gx, const float* gy,
                     const float* gz, float* out, hipStream_t s) {
   PMX_REQUIRE(grad_stencil_ok(p), "gradient stencil: unsupported mesh");
-  const GridDims g = grid_dims(p);
+  const GridDims g = grid_dims(p, 3);
   ProfScope prof_(kProfGradAdj, s);
   hipLaunchKernelGGL(k_grad_adj, dim3(nblocks(g)), dim3(256), 0, s, gx, gy,
                      gz, out, g);
