@@ -284,6 +284,7 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   const int np = tiled ? 256 : (int)min((long long)256, tot - p0);
   const int nf = 3 * np;
   const bool kick = c.do_k2 || c.do_k1d;
+  float* const Xo = c.xout ? c.xout : X;  // where the drifted X goes
   const int t = threadIdx.x;
   auto goff = [&](int i) { return stage_off(tiled, p0, c.gz, i); };
   // particle state is streamed with non-temporal loads / stores so it does
@@ -374,14 +375,14 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
       const long long o = goff(i);
       if (!c.skip_f) __builtin_nontemporal_store(sf[i], F + o);
       if (kick) __builtin_nontemporal_store(sp[i], P + o);
-      if (c.do_k1d) __builtin_nontemporal_store(sx[i], X + o);
+      if (c.do_k1d) __builtin_nontemporal_store(sx[i], Xo + o);
       if (c.do_k1d && c.xcopy) __builtin_nontemporal_store(sx[i], c.xcopy + o);
     }
   } else {
     for (int i = t; i < nf; i += 256) {
       if (!c.skip_f) __builtin_nontemporal_store(sf[i], F + 3 * p0 + i);
       if (kick) __builtin_nontemporal_store(sp[i], P + 3 * p0 + i);
-      if (c.do_k1d) __builtin_nontemporal_store(sx[i], X + 3 * p0 + i);
+      if (c.do_k1d) __builtin_nontemporal_store(sx[i], Xo + 3 * p0 + i);
       if (c.do_k1d && c.xcopy)
         __builtin_nontemporal_store(sx[i], c.xcopy + 3 * p0 + i);
     }

@@ -737,6 +737,18 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     return base ? base + (long long)(i - tape.first) * BN3 : nullptr;
   };
   bool x_kept = false;  // X of the coming force is already in its slot
+  // Without PGD a taped run keeps the drifted positions only in the tape:
+  // the readout writes them into the next slot (xout) and the next force
+  // reads them there, and one copy at the end returns the last positions
+  // to the state.  With PGD the drifted positions are corrected in place,
+  // so the tape gets a second copy (xcopy).
+  float* Xc = X;  // the current positions: the state's or a tape slot
+  auto drift_to = [&](int i) {
+    float* slot = xslot(i);
+    kd.xout = pgd ? nullptr : slot;
+    kd.xcopy = pgd ? slot : nullptr;
+    x_kept = slot != nullptr;
+  };
   // a force mesh the tape keeps is float4 (the reverse sweep stores the
   // density cotangent in its fourth slot); one only the readout reads is
   // packed, 12 bytes per cell
@@ -750,11 +762,11 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     if (kd.do_k1d) {
       kd.k1 = (float)coef[3 * b0];
       kd.d = (float)coef[3 * b0 + 1];
-      kd.xcopy = xslot(b0);  // the drifted X goes to the tape as well
-      x_kept = kd.xcopy != nullptr;
+      drift_to(b0);
     }
     PMX_TRY(launch_readout_kick_drift(X, P, F, mesh_nc(nw.f4, nc), md,
                                       kd, s));
+    if (kd.do_k1d && kd.xout) Xc = kd.xout;
   } else if (b0 < b1) {
     // opening kick + drift of the first step from the stored force
     KickDrift c{};
@@ -766,36 +778,41 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   for (int i = b0; i < b1; ++i) {
     const long long slot = i - tape.first;
     if (pgd) {
-      if (!x_kept) PMX_TRY(keep_x(xslot(i), X));
+      if (!x_kept) PMX_TRY(keep_x(xslot(i), Xc));
       const float alpha = pgd[3 * i], kl = pgd[3 * i + 1],
                   ks = pgd[3 * i + 2];
       const PgdTerm pt{kl, ks, 1};
       const NbodyWork nwp = at(tape.fp ? tape.fp + slot * 4 * M : nullptr);
-      PMX_TRY(force_at(p, X, N, f, nwp, s, &pt));
-      PMX_TRY(launch_pgd_shift(X, m3(nwp.f4), md, f, alpha / f, s));
+      PMX_TRY(force_at(p, Xc, N, f, nwp, s, &pt));
+      PMX_TRY(launch_pgd_shift(Xc, m3(nwp.f4), md, f, alpha / f, s));
       // the corrected positions are the force's
-      PMX_TRY(keep_x(tape.xs ? tape.xs + slot * BN3 : nullptr, X));
+      PMX_TRY(keep_x(tape.xs ? tape.xs + slot * BN3 : nullptr, Xc));
     } else if (!x_kept) {
-      PMX_TRY(keep_x(xslot(i), X));
+      PMX_TRY(keep_x(xslot(i), Xc));
     }
     const NbodyWork nw = at(tape.fs ? tape.fs + slot * 4 * M : nullptr);
     const int nc = tape.fs ? 4 : 3;
-    PMX_TRY(force_at(p, X, N, f, nw, s, nullptr, nc));
+    PMX_TRY(force_at(p, Xc, N, f, nw, s, nullptr, nc));
     kd.do_k2 = 1;
     kd.k2 = (float)coef[3 * i + 2];
     kd.do_k1d = (i + 1 < b1);
-    kd.xcopy = nullptr;
+    kd.xout = kd.xcopy = nullptr;
     x_kept = false;
     if (kd.do_k1d) {
       kd.k1 = (float)coef[3 * (i + 1)];
       kd.d = (float)coef[3 * (i + 1) + 1];
-      kd.xcopy = xslot(i + 1);
-      x_kept = kd.xcopy != nullptr;
+      drift_to(i + 1);
     }
     // only the last force of the run has to stay in the state
     kd.skip_f = kd.do_k1d;
-    PMX_TRY(launch_readout_kick_drift(X, P, F, mesh_nc(nw.f4, nc), md,
+    PMX_TRY(launch_readout_kick_drift(Xc, P, F, mesh_nc(nw.f4, nc), md,
                                       kd, s));
+    if (kd.do_k1d && kd.xout) Xc = kd.xout;
+  }
+  if (Xc != X) {
+    ProfScope prof_(kProfAdjUpdate, s);
+    PMX_HIP_CHECK(hipMemcpyAsync(X, Xc, sizeof(float) * BN3,
+                                 hipMemcpyDeviceToDevice, s));
   }
   return kOk;
 }

@@ -245,6 +245,7 @@ struct KickDrift {
   // particle grid (N = gx*gy*gz per batch member, C order) or 0: lets the
   // fused readout visit 4 x 64 (y, z) particle tiles, x fastest
   int gx, gy, gz;
+  float* xout;    // if set (with do_k1d): drifted X stored here, not in X
   float* xcopy;   // if set (with do_k1d): drifted X also stored here
 };
 // The three force meshes: separate (x, y, z), interleaved float4 (f4) or
