@@ -78,3 +78,22 @@ def test_scipy_helpers_match_oracle():
                              0.0, atol=1e-12)
   np.testing.assert_allclose(fint.simps(np.exp, 0.0, 1.0), np.e - 1,
                              rtol=1e-9)
+
+
+@pytest.mark.parametrize("stages", [np.linspace(0.1, 1.0, 41),
+                                    np.logspace(-2, 0, 17), [0.1, 0.5]])
+def test_step_coefficients_equal_scalar_factors(stages):
+  """The KDK coefficients of a whole run, evaluated on all steps at once,
+  against the scalar kick / drift factors (tfpm.py:273-314, 472-498) that
+  kick() and drift() use, step by step."""
+  from flowpm_amd import tfpm
+  c = Planck15()
+  ours = tfpm.step_coefficients(c, stages)
+  ref = []
+  for i in range(len(stages) - 1):
+    a0, a1 = float(stages[i]), float(stages[i + 1])
+    ah = (a0 * a1)**0.5
+    ref += [tfpm._kick_fac(c, a0, a0, ah), tfpm._drift_fac(c, a0, ah, a1),
+            tfpm._kick_fac(c, ah, a1, a1)]
+  np.testing.assert_allclose(ours, ref, rtol=1e-14, atol=0)
+  assert tfpm.step_coefficients(c, [0.3]) == []
