@@ -676,7 +676,9 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
   // the density buffer takes the potential once it is transformed
-  PMX_REQUIRE(grad_stencil_ok(p), "force: mesh sides must be multiples of 4");
+  PMX_REQUIRE(grad_stencil_ok(p),
+              "force: nz must be a multiple of 4, sides >= 4 (nz >= 8), "
+              "B * cells < 2^33");
   return force_mesh_f4(p, nw.rho, nw.w, nw.rho, nw.f4, sc, s, pgd, nc);
 }
 
@@ -1317,7 +1319,8 @@ int pmx_force_mesh_f4(const pmx_plan* plan, const float* rho, float* work,
               "pmx_force_mesh_f4: f4 must be 16-byte aligned");
   const Plan& p = P(plan);
   PMX_REQUIRE(grad_stencil_ok(p),
-              "pmx_force_mesh_f4: mesh sides must be multiples of 4");
+              "pmx_force_mesh_f4: nz must be a multiple of 4, sides >= 4 "
+              "(nz >= 8), B * cells < 2^33");
   return force_mesh_f4(p, rho, reinterpret_cast<float2*>(work), phi, f4,
                        rho_scale, S(stream), nullptr);
 }

@@ -220,7 +220,9 @@ bool grad_stencil_ok(const Plan& p) {
 
 int launch_grad_mesh(const Plan& p, const float* phi, float* fm, int nc,
                      hipStream_t s) {
-  PMX_REQUIRE(grad_stencil_ok(p), "gradient stencil: unsupported mesh");
+  PMX_REQUIRE(grad_stencil_ok(p),
+              "gradient stencil: nz % 4 != 0, a side below 4 (nz below 8) "
+              "or B * cells >= 2^33");
   PMX_REQUIRE(nc == 3 || nc == 4, "gradient stencil: 3 or 4 floats per cell");
   const GridDims g = grid_dims(p, 1);
   ProfScope prof_(kProfGrad, s);
@@ -236,7 +238,9 @@ int launch_grad_mesh(const Plan& p, const float* phi, float* fm, int nc,
 
 int launch_grad_adj(const Plan& p, const float* gx, const float* gy,
                     const float* gz, float* out, hipStream_t s) {
-  PMX_REQUIRE(grad_stencil_ok(p), "gradient stencil: unsupported mesh");
+  PMX_REQUIRE(grad_stencil_ok(p),
+              "gradient stencil: nz % 4 != 0, a side below 4 (nz below 8) "
+              "or B * cells >= 2^33");
   const GridDims g = grid_dims(p, 3);
   ProfScope prof_(kProfGradAdj, s);
   hipLaunchKernelGGL(k_grad_adj, dim3(nblocks(g)), dim3(256), 0, s, gx, gy,

@@ -125,7 +125,8 @@ int pmx_force_meshes(const pmx_plan* plan, const float* rho, float* work,
  * f4[cell] = (D_x phi, D_y phi, D_z phi, 0), D the periodic fourth-order
  * central difference 2/3 (x[n+1] - x[n-1]) - 1/12 (x[n+2] - x[n-2]) whose
  * Fourier multiplier is exactly gradient_kernel (kernels.py:51-78).
- * work: 1 spec buffer; nz % 4 == 0; f4 16-byte aligned. */
+ * work: 1 spec buffer; nz % 4 == 0, sides >= 4 (nz >= 8), B * cells
+ * < 2^33; f4 16-byte aligned. */
 int pmx_force_mesh_f4(const pmx_plan* plan, const float* rho, float* work,
                       float* phi, float* f4, float rho_scale, void* stream);
 /* Its adjoint: grho = sum_d T_d^T g_d, T_d^T = c2r(conj(lap i g_d) r2c(.)),
