@@ -32,8 +32,11 @@ Function map (reference file:line):
                                                 tfpm.py:222-270 / 369-422
 Pins: tests/test_oracle_pm.py (numpy.fft, brute-force DFT, brute-force CIC
 loops, paint/readout adjoint identity, mass and momentum conservation,
-plane-wave closed forms for lpt1 and lpt2_source, linear growth of a single
-small mode through nbody, autograd vs central finite differences).
+plane-wave closed forms for lpt1, lpt2_source, apply_longrange at split 0 and
+split != 0, apply_pgd and whitenoise_to_linear (rectangular box), the
+lpt_init time-derivative identities P = a^3 E dX/da, F = a^2 E dP/da and the
+closed-form 2LPT displacement, linear growth of a single small mode through
+nbody at pm_nc_factor 1 and 2, autograd vs central finite differences).
 """
 import numpy as np
 import torch
