@@ -1,5 +1,6 @@
 """Pins of oracle/spectrum.py (power spectrum, CIC compensation)."""
 import numpy as np
+import torch
 
 from oracle import spectrum as osp
 
@@ -77,3 +78,33 @@ def test_compensation_single_mode_and_constant():
   np.testing.assert_allclose(out, f * w, atol=1e-12)
   c = np.full((1,) + shape, 2.5)
   np.testing.assert_allclose(osp.compensate_cic(c), c, atol=1e-12)
+
+
+def test_torch_versions_equal_numpy_and_finite_differences():
+  rng = np.random.default_rng(5)
+  shape, L = (8, 8, 10), np.array([40.0, 40.0, 50.0])
+  x = rng.standard_normal((2,) + shape)
+  kmin, dk = 0.1, 0.3
+  _, P = osp.power_spectrum(x, kmin, dk, L)
+  xt = torch.tensor(x, requires_grad=True)
+  _, Pt = osp.power_spectrum_torch(xt, kmin, dk, L)
+  ok = np.isfinite(P)
+  np.testing.assert_allclose(Pt.detach().numpy()[ok], P[ok], rtol=1e-12)
+  np.testing.assert_allclose(osp.compensate_cic_torch(xt).detach().numpy(),
+                             osp.compensate_cic(x), rtol=1e-12, atol=1e-12)
+  # loss of the PGD fit (scripts/fit_pgd.py:61-70) on the compensated field
+  target = np.abs(P[:, ok[0]].mean(0)) + 1.0
+
+  def loss(t):
+    _, pk = osp.power_spectrum_torch(osp.compensate_cic_torch(t), kmin, dk, L)
+    pk = pk[:, torch.as_tensor(ok[0])].mean(0)
+    return ((1 - pk / torch.as_tensor(target))**2).sum()
+
+  loss(xt).backward()
+  g = xt.grad.numpy()
+  for idx in [(0, 1, 2, 3), (1, 7, 0, 9), (0, 4, 4, 0)]:
+    e = np.zeros_like(x)
+    e[idx] = 1e-6
+    fd = (float(loss(torch.tensor(x + e))) -
+          float(loss(torch.tensor(x - e)))) / 2e-6
+    assert abs(fd - g[idx]) < 1e-6 * max(1.0, abs(fd)), (idx, fd, g[idx])
