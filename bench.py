@@ -202,6 +202,8 @@ def main():
   if not args.no_batched:
     batched = batched_runs(fa, cosmo, dev)
 
+  copy_gbps = copy_peak(dev)
+
   cpu = None
   if not args.no_cpu_baseline:
     cpu = cpu_baseline(state0, n, cosmo)
@@ -240,6 +242,8 @@ def main():
           "unit": "GB/s",
           "frac": achieved / HBM_PEAK_GBS,
           "traffic": hbm_traffic(dom, n),
+          "copy_peak": copy_gbps,
+          "frac_of_copy": achieved / copy_gbps,
       },
       "step_alg_bytes": sum(
           (algorithmic_bytes(k, n, N) or 0) * v["launches"] / (K + 1)
@@ -360,13 +364,13 @@ def batched_runs(fa, cosmo, dev):
   mix = BatchedNbody(groups)
   t_mix = _time(mix.run, 3)
 
-  def mix_grad():
-    # forward and gradient of every group (taped adjoint per size class)
-    for g in groups:
-      x0 = g["state"].detach().requires_grad_(True)
-      out = fa.nbody(cosmo, x0, stages, g["nc"])
-      torch.autograd.grad(out, x0, grad_outputs=torch.ones_like(out))
-  t_mix_grad = _time(mix_grad, 2)
+  # forward and gradient of every class (taped run + reverse sweep per
+  # class) in one graph replay
+  mixg = BatchedNbody(groups, grad=True)
+  cts = [torch.ones_like(g["state"]) for g in groups]
+  mixg.run_grad(cts)
+  t_mix_grad = _time(mixg.run_grad, 2)
+  del mixg
   return {
       "sims64_batch": B,
       "sims64_per_s_batched": B / (t_b * 1e-3),
@@ -378,33 +382,57 @@ def batched_runs(fa, cosmo, dev):
       "sims64_batch_kernels": kern,
       "fft_tiles": {k: "%dx%d" % v for k, v in tiles.items()},
       "uneven_mix_fwd_grad_sims_per_s": 32 / (t_mix_grad * 1e-3),
-      "note": "10-step nbody per sim, ICs excluded; forward: graph replay; "
-              "fwd_grad: forward + taped adjoint per size class",
+      "note": "10-step nbody per sim, ICs excluded; forward: one graph "
+              "replay; fwd_grad: one graph replay of every class's taped "
+              "run and reverse sweep",
   }
 
 
-def cpu_baseline(state0, n, cosmo):
-  """One PM step (force + kick + drift) of the same workload with the CPU
-  oracle in float32 on all host threads."""
+def cpu_baseline(state0, n, cosmo, reps=5):
+  """PM steps (force, opening kick, drift, closing kick: one force
+  evaluation each) of the same workload with the CPU oracle in float32 on
+  all host threads: one warm-up step, then the median of ``reps`` steps."""
   from oracle import background as obg
   from oracle import pm as opm
   torch.set_num_threads(os.cpu_count() or 1)
   g = obg.GrowthTable(obg.planck15(), dtype=np.float64)
   st = state0.detach().cpu()
-  a0, a1 = 0.1, 0.1225
-  ah = (a0 * a1)**0.5
-  t0 = time.perf_counter()
-  st = opm.force(g, st, [n] * 3)
-  st = opm.kick(g, st, a0, a0, ah)
-  st = opm.drift(g, st, a0, ah, a1)
-  st = opm.force(g, st, [n] * 3)
-  st = opm.kick(g, st, ah, a1, a1)
-  dt = time.perf_counter() - t0
-  # the sample holds two force evaluations (jump start + the step's force)
-  return {"value": 1e3 * dt / 2, "unit": "ms/step",
+  a = np.linspace(0.1, 0.1 + 0.0225 * (reps + 1), reps + 2)
+  times = []
+  for i in range(reps + 1):
+    a0, a1 = a[i], a[i + 1]
+    ah = (a0 * a1)**0.5
+    t0 = time.perf_counter()
+    st = opm.force(g, st, [n] * 3)
+    st = opm.kick(g, st, a0, a0, ah)
+    st = opm.drift(g, st, a0, ah, a1)
+    st = opm.kick(g, st, ah, a1, a1)
+    times.append(time.perf_counter() - t0)
+  return {"value": 1e3 * float(np.median(times[1:])), "unit": "ms/step",
           "cores": torch.get_num_threads(), "kind": "port",
-          "sample": "1 KDK step (2 force evaluations) of the %d^3 workload "
-                    "with the float32 torch-CPU oracle" % n}
+          "sample": "%d^3 workload: 1 warm-up + median of %d PM steps (one "
+                    "force evaluation + kick + drift + kick each) with the "
+                    "float32 torch-CPU oracle" % (n, reps),
+          "step_ms_all": [1e3 * t for t in times]}
+
+
+def copy_peak(dev, mib=1024, reps=5):
+  """A plain device-to-device copy of ``mib`` MiB (torch copy_): read +
+  write bytes over the median HIP-event time, GB/s."""
+  src = torch.ones(mib << 18, device=dev)
+  dst = torch.empty_like(src)
+  dst.copy_(src)
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  ts = []
+  for _ in range(reps):
+    e0.record()
+    dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ts.append(e0.elapsed_time(e1))
+  del src, dst
+  return 2 * (mib << 20) / (float(np.median(ts)) * 1e-3) / 1e9
 
 
 if __name__ == "__main__":

@@ -45,6 +45,19 @@ _SIGS = {
     "pmx_fft_c2r_poisson_grad": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _vp]),
     "pmx_fft_c2r_longrange_grad": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f,
                                         _c.c_double, _vp]),
+    "pmx_fft_longrange_grad_adj": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f,
+                                        _c.c_double, _vp]),
+    "pmx_fft_c2r_pgd_grad": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _f,
+                                  _i, _vp]),
+    "pmx_fft_pgd_grad_adj": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _f,
+                                  _vp]),
+    "pmx_pgd": (_i, [_vp, _vp, _ll, _f, _f, _f, _f, _vp, _vp]),
+    "pmx_pgd_adj_work_floats": (_ll, [_vp, _ll]),
+    "pmx_pgd_adjoint": (_i, [_vp, _vp, _ll, _f, _f, _f, _f, _vp, _vp, _vp,
+                             _vp]),
+    "pmx_mesh_dot": (_i, [_vp, _vp, _ll, _vp, _vp]),
+    "pmx_power_spectrum_bins_adj": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i,
+                                         _vp, _vp]),
     "pmx_force_meshes": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _vp]),
     "pmx_force_meshes_adj": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _vp]),
     "pmx_force_mesh_f4": (_i, [_vp, _vp, _vp, _vp, _vp, _f, _vp]),
@@ -78,6 +91,10 @@ _SIGS = {
     "pmx_nbody_adj_work_floats": (_ll, [_vp, _ll, _i]),
     "pmx_nbody_adjoint": (_i, [_vp, _vp, _i, _i, _ll, _f, _f, _dp, _vp, _vp,
                                _vp]),
+    "pmx_nbody_run_last_segment": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _i,
+                                        _vp, _vp]),
+    "pmx_nbody_adjoint_last_taped": (_i, [_vp, _vp, _i, _i, _ll, _f, _f, _dp,
+                                          _vp, _vp, _vp]),
     "pmx_nbody_batched_step": (_i, [_i, _c.POINTER(_vp), _c.POINTER(_vp),
                                     _c.POINTER(_ll), _c.POINTER(_f),
                                     _c.POINTER(_f), _c.POINTER(_dp),

@@ -619,11 +619,14 @@ __global__ __launch_bounds__(T) void k_ypass(YJobs jobs, PlanDev p) {
 }
 
 // X pass: [forward FFT_x] -> multiply by Fourier terms -> [inverse FFT_x].
-template <int N, int TW, int T>
+// FAN = false: one input and one output (the time loop's potential), the
+// multiply in place on one LDS tile, which lets twice the workgroups share a
+// CU; FAN = true: several inputs summed, or several outputs.
+template <int N, int TW, int T, bool FAN>
 __global__ __launch_bounds__(T) void k_xpass(XJob J, PlanDev p) {
   constexpr int P = Pitch<TW>::P;
   __shared__ float2 a[N * P];
-  __shared__ float2 w[N * P];
+  __shared__ float2 w[FAN ? N * P : 1];
   __shared__ float2 tw[N];
   load_twiddles<N, T>(tw, p.ax[0].tw);
   const Tile tl = make_tile<0, TW>(p, blockIdx.x);
@@ -638,7 +641,22 @@ __global__ __launch_bounds__(T) void k_xpass(XJob J, PlanDev p) {
       kz = min(kz0 + c, p.H);
     }
   };
-  if (J.nin == 1) {
+  if constexpr (!FAN) {
+    load_tile<N, TW, T>(a, J.in[0], tl);
+    __syncthreads();
+    if (J.do_fwd) fft_tile<N, TW, T, -1>(a, tw);
+    const Term tm = J.term[0][0];
+    for (int q = threadIdx.x; q < N * TW; q += T) {
+      const int i = q / TW, c = q % TW;
+      int ky, kz;
+      kyz(c, ky, kz);
+      a[i * P + c] = apply_term(a[i * P + c], tm, i, ky, kz, p);
+    }
+    __syncthreads();
+    if (J.do_inv) fft_tile<N, TW, T, +1>(a, tw);
+    store_tile<N, TW, T>(J.out[0], a, tl, 1.0f);
+    return;
+  } else if (J.nin == 1) {
     load_tile<N, TW, T>(a, J.in[0], tl);
     __syncthreads();
     if (J.do_fwd) fft_tile<N, TW, T, -1>(a, tw);
@@ -821,7 +839,12 @@ static int x_nt(const Plan& pl, const XJob& job, hipStream_t s) {
   const int nkt = (p.H + TW - 1) / TW;
   const int tiles = p.B * p.ny * nkt + p.B * ((p.ny + TW - 1) / TW);
   ProfScope prof_(kProfX, s);
-  hipLaunchKernelGGL((k_xpass<N, TW, T>), dim3(tiles), dim3(T), 0, s, job, p);
+  if (job.nin == 1 && job.nout == 1)
+    hipLaunchKernelGGL((k_xpass<N, TW, T, false>), dim3(tiles), dim3(T), 0, s,
+                       job, p);
+  else
+    hipLaunchKernelGGL((k_xpass<N, TW, T, true>), dim3(tiles), dim3(T), 0, s,
+                       job, p);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

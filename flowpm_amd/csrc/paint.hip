@@ -12,8 +12,9 @@
 //   dense box, converted back to float, to its own arena slot; the C
 //   channels are deposited one after the other through the same box.  The
 //   fixed-point scale is a power of two chosen per brick and channel from
-//   the sum of the channel's |weights| so no cell can overflow: one deposit
-//   is rounded to at most 2^-31 of that sum.
+//   the sum S of the channel's |weights| so no cell can overflow: the scale
+//   is 2^(30 - ex) with S = m 2^ex, m in [0.5, 1), so one deposit is rounded
+//   by at most half a unit, 2^(ex - 31) <= 2^-30 S.
 //   A brick whose box does not fit the LDS budget falls back to
 //   device-scope float atomics into the finished mesh (k_brick_spill, after
 //   phase 2).
@@ -28,19 +29,15 @@
 // memory transaction per deposit; here the mesh sees none in the common
 // case.  C = 1 is the density paint, C = 3 the weighted paint of the force
 // VJP (three mesh cotangents at once).
-#include <climits>
-
 #include "pmx_internal.h"
 
 namespace pmx {
-
-constexpr int kBoxFloats = 16384;  // LDS budget of one brick box (64 KB)
-constexpr int kTile = 16;          // gather tile side
 
 struct BrickArgs {
   float* mesh[3];        // C output meshes [B][nx][ny][nz] (overwritten)
   float* arena;          // box arena
   long long arena_cap;   // floats
+  long long slot;        // arena floats per brick
   int* meta;             // 8 ints per brick: lo[3], ext[3], off lo/hi
   int* ctr;              // [2] overflow count, [3] box radius (written by
                          // k_brick_reduce between the two phases)
@@ -72,11 +69,6 @@ __device__ __forceinline__ bool ok_pos(float x, float y, float z) {
   return fabsf(x) < 1.0e9f && fabsf(y) < 1.0e9f && fabsf(z) < 1.0e9f;
 }
 
-__device__ __forceinline__ int imod(int a, int n) {
-  int r = a % n;
-  return r < 0 ? r + n : r;
-}
-
 // deposit workgroup: 512 threads (8 waves: with two 64 KB boxes per CU that
 // is 16 waves to hide the loads and the LDS atomics), 4 particles per
 // thread; bricks hold <= 2048 particles
@@ -98,11 +90,10 @@ template <int C>
 __device__ __forceinline__ void load_brick(const BrickArgs& a, int bid, int b,
                                            int tid, BrickRegs<C>& r) {
   const BrickGrid& g = a.g;
-  const int kz = bid % g.nbz, ky = (bid / g.nbz) % g.nby,
-            kx = bid / (g.nbz * g.nby);
-  const int q0x = kx * g.ex, q0y = ky * g.ey, q0z = kz * g.ez;
-  const int cy = min(g.ey, g.gy - q0y), cz = min(g.ez, g.gz - q0z);
-  const int nloc = min(g.ex, g.gx - q0x) * cy * cz;
+  int q0[3], cnt[3];
+  brick_block(g, bid, q0, cnt);
+  const int cy = cnt[1], cz = cnt[2];
+  const int nloc = cnt[0] * cy * cz;
   const long long pb = (long long)b * a.d.N;
   long long pidx[kPPT];
 #pragma unroll
@@ -111,8 +102,8 @@ __device__ __forceinline__ void load_brick(const BrickArgs& a, int bid, int b,
     r.ok[u] = t < nloc;
     const int tt = r.ok[u] ? t : 0;
     const int lz = tt % cz, q = tt / cz, ly = q % cy, lx = q / cy;
-    pidx[u] =
-        pb + ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz + q0z + lz;
+    pidx[u] = pb + ((long long)(q0[0] + lx) * g.gy + (q0[1] + ly)) * g.gz +
+              q0[2] + lz;
     r.px[u] = a.X[3 * pidx[u]];
     r.py[u] = a.X[3 * pidx[u] + 1];
     r.pz[u] = a.X[3 * pidx[u] + 2];
@@ -142,11 +133,8 @@ __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
   BrickRegs<C> cur;
   load_brick<C>(a, bid, b, tid, cur);
   {
-    const int kz = bid % g.nbz, ky = (bid / g.nbz) % g.nby,
-              kx = bid / (g.nbz * g.nby);
-    const int q0x = kx * g.ex, q0y = ky * g.ey, q0z = kz * g.ez;
-    const int cx = min(g.ex, g.gx - q0x), cy = min(g.ey, g.gy - q0y),
-              cz = min(g.ez, g.gz - q0z);
+    int q0[3], cnt[3];
+    brick_block(g, bid, q0, cnt);
     // the brick's registers, modified in place
     float(&px)[kPPT] = cur.px;
     float(&py)[kPPT] = cur.py;
@@ -217,23 +205,7 @@ __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
         }
       }
       const int n[3] = {a.d.nx, a.d.ny, a.d.nz};
-      int ext[3] = {0, 0, 0};
-      bool any = lo[0] <= hi[0];
-      long long vol = 0;
-      bool small = true;
-      if (any) {
-        for (int d = 0; d < 3; ++d) {
-          // long long: positions may span up to 2e9 cells
-          const long long e = (long long)hi[d] - lo[d] + 2;
-          small = small && e <= n[d];
-          ext[d] = small ? (int)e : 0;
-        }
-        if (small) vol = (long long)ext[0] * ext[1] * ext[2];
-      }
-      int mode = 0;  // 0: nothing, 1: LDS box, 2: device atomics (spill)
-      const long long off =
-          ((long long)b * g.nbricks + bid) * (long long)(C * kBoxFloats);
-      int R = 0;
+      const long long off = ((long long)b * g.nbricks + bid) * a.slot;
       // fixed-point scale 2^e with (sum of |weights|) * 2^e < 2^30: no cell
       // total (nor a partial sum) can leave the int32 range
       bool wfin = true;
@@ -250,28 +222,20 @@ __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
         shs[c][0] = ldexpf(1.0f, e);
         shs[c][1] = ldexpf(1.0f, -e);
       }
-      if (any) {
-        // every brick owns a fixed arena slot: no shared allocator, and no
-        // shared counter either (k_brick_reduce takes the radius maximum)
-        mode = (small && vol <= kBoxFloats && wfin) ? 1 : 2;
-        if (mode == 1) {
-          // box radius around the brick's own mesh interval, for the gather
-          const int q0[3] = {q0x, q0y, q0z}, cc[3] = {cx, cy, cz};
-          for (int d = 0; d < 3; ++d) {
-            const int f = g.f;
-            R = max(R, q0[d] * f - lo[d]);
-            R = max(R, lo[d] + ext[d] - (q0[d] + cc[d]) * f);
-          }
-        }
-      }
+      // every brick owns a fixed arena slot: no shared allocator, and no
+      // shared counter either (k_brick_reduce takes the radius maximum)
+      const BrickBox bx = brick_box(lo, hi, n, q0, cnt, g.f, wfin, kBoxFloats);
+      const int mode = bx.mode;
+      const int* ext = bx.ext;
+      const long long vol = bx.vol;
       int* m = a.meta + 8LL * ((long long)b * g.nbricks + bid);
       m[0] = lo[0];
       m[1] = lo[1];
       m[2] = lo[2];
-      m[3] = mode == 1 ? ext[0] : 0;
-      m[4] = mode == 1 ? ext[1] : 0;
-      m[5] = mode == 1 ? ext[2] : 0;
-      m[6] = R;
+      m[3] = ext[0];
+      m[4] = ext[1];
+      m[5] = ext[2];
+      m[6] = bx.R;
       m[7] = mode == 2 ? 1 : 0;
       sh[0] = mode;
       sh[1] = lo[0];
@@ -291,38 +255,40 @@ __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
     const long long off = (long long)sh[8] | ((long long)sh[9] << 31);
     // mode 0: no particles; mode 2: k_brick_spill adds it after the gather
     if (mode != 1) return;
-    {
-      constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};
-      constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};
-      constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+    constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};
+    constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};
+    constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+    // clears the box and deposits channel c of the brick's particles
+    auto deposit = [&](int c, float fxs) {
+      for (int i = tid; i < vol; i += kDT) box[i] = 0;
+      __syncthreads();
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const float fxs = shs[c][0], fxinv = shs[c][1];
-        for (int i = tid; i < vol; i += kDT) box[i] = 0;
-        __syncthreads();
+      for (int u = 0; u < kPPT; ++u) {
+        if (!ok[u]) continue;
+        const float x = px[u], y = py[u], z = pz[u];
+        const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+        const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
+        const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
+        const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
+        const int bx = (int)flx - lo0, by = (int)fly - lo1, bz = (int)flz - lo2;
+        const float wq = wc[u][c] * fxs;
 #pragma unroll
-        for (int u = 0; u < kPPT; ++u) {
-          if (!ok[u]) continue;
-          const float x = px[u], y = py[u], z = pz[u];
-          const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
-          const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
-          const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
-          const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
-          const int bx = (int)flx - lo0, by = (int)fly - lo1, bz = (int)flz - lo2;
-          const float wq = wc[u][c] * fxs;
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
-                             (OZ[k] ? w1z : w0z);
-            const int li = ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
-            atomicAdd(&box[li], __float2int_rn(wq * kw));
-          }
+        for (int k = 0; k < 8; ++k) {
+          const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
+                           (OZ[k] ? w1z : w0z);
+          const int li = ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
+          atomicAdd(&box[li], __float2int_rn(wq * kw));
         }
-        __syncthreads();
-        float* dst = a.arena + off + (long long)c * vol;
-        for (int i = tid; i < vol; i += kDT) dst[i] = (float)box[i] * fxinv;
-        if (c + 1 < C) __syncthreads();  // the box is cleared for the next one
       }
+      __syncthreads();
+    };
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      deposit(c, shs[c][0]);
+      const float fxinv = shs[c][1];
+      float* dst = a.arena + off + (long long)c * vol;
+      for (int i = tid; i < vol; i += kDT) dst[i] = (float)box[i] * fxinv;
+      if (c + 1 < C) __syncthreads();  // the box is cleared for the next one
     }
   }
 }
@@ -359,55 +325,6 @@ __global__ __launch_bounds__(1024) void k_brick_reduce(const int* __restrict__ m
   }
 }
 
-// overlap of the wrapped box interval [lo, lo+ext) (ext <= n) with the tile
-// interval [t0, t0+T): up to two pieces (start in tile coords, box offset,
-// length)
-struct Pieces {
-  int n;
-  int ts[2], bo[2], len[2];
-};
-__device__ __forceinline__ Pieces overlap(int lo, int ext, int t0, int T,
-                                          int n) {
-  Pieces P;
-  P.n = 0;
-  const int a = imod(lo, n);
-  // piece 1: wrapped [a, min(a+ext, n)), box offset 0
-  // piece 2: wrapped [0, a+ext-n),       box offset n-a
-  const int s[2] = {a, 0};
-  const int e[2] = {min(a + ext, n), max(a + ext - n, 0)};
-  const int o[2] = {0, n - a};
-  for (int k = 0; k < 2; ++k) {
-    const int b0 = max(s[k], t0), b1 = min(e[k], t0 + T);
-    if (b1 > b0) {
-      P.ts[P.n] = b0 - t0;
-      P.bo[P.n] = o[k] + (b0 - s[k]);
-      P.len[P.n] = b1 - b0;
-      ++P.n;
-    }
-  }
-  return P;
-}
-
-// does brick k (mesh interval [k*E, k*E + cnt*f) widened by R) touch the
-// tile [t0, t0+T) modulo n?
-__device__ __forceinline__ bool brick_hits(int k, int e, int gdim, int f,
-                                           int R, int t0, int T, int n) {
-  const int cnt = min(e, gdim - k * e);
-  if ((long long)cnt * f + 2LL * R >= n) return true;
-  const int a0 = k * e * f - R;
-  const int len = cnt * f + 2 * R;
-  const int s = imod(a0, n);
-  const int s1 = s + len;
-  return (s < t0 + T && s1 > t0) || (s < t0 + T + n && s1 > t0 + n);
-}
-
-// does the wrapped box [am, am + ext) (0 <= am < n, ext <= n) meet the tile
-// [t0, t0 + T) (t0 + T <= n)?
-__device__ __forceinline__ bool box_meets(int am, int ext, int t0, int T,
-                                          int n) {
-  return (am < t0 + T && am + ext > t0) || (am + ext - n > t0);
-}
-
 template <int C>
 __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
   constexpr int PER = kTile;  // owned cells per thread: (k, iy, iz), k < 16
@@ -415,6 +332,7 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
   __shared__ int ncand[3];
   __shared__ int lst[256][8];  // overlapping bricks of one chunk
   __shared__ int wcnt[4];
+  __shared__ short wl[4][256];  // per wave: the entries that meet its rows
   const BrickGrid& g = a.g;
   const MeshDims& d = a.d;
   const int tid = threadIdx.x;
@@ -502,19 +420,62 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
     }
     __syncthreads();
     // 2. every thread adds the bricks, in list order, to its own cells
-    if (yz_ok) {
+    if constexpr (C == 1) {
+      // each wave keeps the entries whose box meets its 4 tile rows (a
+      // wave-uniform test), then takes two bricks per round with the loads
+      // of both in flight together; the sums keep the list order
+      const int wy = 4 * wv;
+      const int wrows = max(0, min(4, Ty - wy));
+      int cnt = 0;
+      if (wrows > 0) {
+        for (int j0 = 0; j0 < nl; j0 += 64) {
+          const int j = j0 + ln;
+          const bool hit =
+              j < nl && box_meets(lst[j][2], lst[j][5], t0y + wy, wrows, d.ny);
+          const unsigned long long mk = __ballot(hit);
+          if (hit) wl[wv][cnt + __popcll(mk & ((1ULL << ln) - 1ULL))] = (short)j;
+          cnt += __popcll(mk);
+        }
+      }
+      __syncthreads();
+      // the 16 loads of entry j for this lane's cells (out-of-box loads read
+      // box element 0 and are discarded)
+      auto gather16 = [&](int j, bool live, float* v, bool* in) {
+        const int e0 = lst[j][4], e1 = lst[j][5], e2 = lst[j][6];
+        const int oy = wrap_off(cy, lst[j][2], d.ny);
+        const int oz = wrap_off(cz, lst[j][3], d.nz);
+        const bool lane = live && yz_ok && oy < e1 && oz < e2;
+        const float* src =
+            a.arena + ((long long)b * g.nbricks + lst[j][0]) * a.slot;
+        const int a0 = lst[j][1];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+          const int ox = wrap_off(t0x + k, a0, d.nx);
+          in[k] = lane && k < Tx && ox < e0;
+          v[k] = src[in[k] ? (ox * e1 + oy) * e2 + oz : 0];
+        }
+      };
+      for (int q = 0; q < cnt; q += 2) {
+        const bool two = q + 1 < cnt;
+        float vA[PER], vB[PER];
+        bool inA[PER], inB[PER];
+        gather16(wl[wv][q], true, vA, inA);
+        gather16(wl[wv][two ? q + 1 : q], two, vB, inB);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) acc[0][k] += inA[k] ? vA[k] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) acc[0][k] += inB[k] ? vB[k] : 0.0f;
+      }
+    } else if (yz_ok) {
       for (int j = 0; j < nl; ++j) {
         const int e1 = lst[j][5], e2 = lst[j][6];
-        int oy = cy - lst[j][2];
-        if (oy < 0) oy += d.ny;
-        int oz = cz - lst[j][3];
-        if (oz < 0) oz += d.nz;
+        const int oy = wrap_off(cy, lst[j][2], d.ny);
+        const int oz = wrap_off(cz, lst[j][3], d.nz);
         if (oy >= e1 || oz >= e2) continue;
         const int e0 = lst[j][4];
         const int vol = e0 * e1 * e2;
         const float* src =
-            a.arena + ((long long)b * g.nbricks + lst[j][0]) *
-                          (long long)(C * kBoxFloats);
+            a.arena + ((long long)b * g.nbricks + lst[j][0]) * a.slot;
         const int a0 = lst[j][1];
         // branch-free: every load is issued (out-of-box ones read box
         // element 0) before the first is consumed, so the 16 x C loads of
@@ -523,8 +484,7 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
         bool in[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-          int ox = t0x + k - a0;
-          if (ox < 0) ox += d.nx;
+          const int ox = wrap_off(t0x + k, a0, d.nx);
           in[k] = k < Tx && ox < e0;
           const int bi = in[k] ? (ox * e1 + oy) * e2 + oz : 0;
 #pragma unroll
@@ -565,17 +525,16 @@ __global__ __launch_bounds__(256) void k_brick_spill(BrickArgs a) {
   for (long long q = blockIdx.x; q < nq; q += gridDim.x) {
     if (a.meta[8 * q + 7] == 0) continue;
     const int b = (int)(q / g.nbricks), bid = (int)(q % g.nbricks);
-    const int kz = bid % g.nbz, ky = (bid / g.nbz) % g.nby,
-              kx = bid / (g.nbz * g.nby);
-    const int q0x = kx * g.ex, q0y = ky * g.ey, q0z = kz * g.ez;
-    const int cx = min(g.ex, g.gx - q0x), cy = min(g.ey, g.gy - q0y),
-              cz = min(g.ez, g.gz - q0z);
-    const int nloc = cx * cy * cz;
+    int q0[3], cnt[3];
+    brick_block(g, bid, q0, cnt);
+    const int cy = cnt[1], cz = cnt[2];
+    const int nloc = cnt[0] * cy * cz;
     for (int t = threadIdx.x; t < nloc; t += blockDim.x) {
       const int lz = t % cz, r = t / cz, ly = r % cy, lx = r / cy;
       const long long pi = (long long)b * d.N +
-                           ((long long)(q0x + lx) * g.gy + (q0y + ly)) * g.gz +
-                           q0z + lz;
+                           ((long long)(q0[0] + lx) * g.gy + (q0[1] + ly)) *
+                               g.gz +
+                           q0[2] + lz;
       const float x = a.X[3 * pi] * a.ps, y = a.X[3 * pi + 1] * a.ps,
                   z = a.X[3 * pi + 2] * a.ps;
       if (!ok_pos(x, y, z)) continue;
@@ -602,25 +561,6 @@ __global__ __launch_bounds__(256) void k_brick_spill(BrickArgs a) {
   }
 }
 
-BrickGrid make_brick_grid(int gx, int gy, int gz, int f, int C) {
-  BrickGrid g;
-  g.gx = gx;
-  g.gy = gy;
-  g.gz = gz;
-  g.f = f;
-  // mesh extent of a brick ~ 8 x 16 x 16 cells
-  (void)C;
-  const int bx = 8, by = 16, bz = 16;
-  g.ex = std::max(1, std::min(gx, bx / f));
-  g.ey = std::max(1, std::min(gy, by / f));
-  g.ez = std::max(1, std::min(gz, bz / f));
-  g.nbx = (gx + g.ex - 1) / g.ex;
-  g.nby = (gy + g.ey - 1) / g.ey;
-  g.nbz = (gz + g.ez - 1) / g.ez;
-  g.nbricks = g.nbx * g.nby * g.nbz;
-  return g;
-}
-
 bool brick_paint_ok(const MeshDims& d, const BrickGrid& g) {
   return g.ex * g.ey * g.ez <= kDT * kPPT && g.gx * g.f == d.nx &&
          g.gy * g.f == d.ny && g.gz * g.f == d.nz &&
@@ -642,8 +582,9 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
   if ((long long)d.B * d.N == 0) return kOk;
   BrickArgs a{};
   for (int c = 0; c < C; ++c) a.mesh[c] = mesh[c];
+  a.slot = (long long)C * kBoxFloats;
   a.arena = work;
-  a.arena_cap = (long long)d.B * g.nbricks * C * kBoxFloats;
+  a.arena_cap = (long long)d.B * g.nbricks * a.slot;
   a.meta = reinterpret_cast<int*>(work + a.arena_cap);
   a.ctr = a.meta + 8LL * d.B * g.nbricks;
   a.X = X;

@@ -356,9 +356,12 @@ static int force_meshes_adj(const Plan& p, const float* gx, const float* gy,
 
 // Adjoint of poisson_grad(spec, fwd=0): gdk = sum_d conj(lap i g_d) FFT3(g_d)
 // * scale (fully transformed half spectrum).
+// With pgd / lr_r2 the multiplier carries the same extra real factor as
+// the forward (PGD kernel, long-range filter).
 static int poisson_grad_adj(const Plan& p, const float* gx, const float* gy,
                             const float* gz, float2* w, float2* gdk,
-                            float scale, hipStream_t s) {
+                            float scale, hipStream_t s,
+                            const PgdTerm* pgd = nullptr, double lr_r2 = 0.0) {
   float2* ws[3] = {w, w + p.d.S, w + 2 * p.d.S};
   const float* in[3] = {gx, gy, gz};
   PMX_TRY(r2c_zy(p, in, ws, 3, 1.0f, s));
@@ -367,7 +370,9 @@ static int poisson_grad_adj(const Plan& p, const float* gx, const float* gy,
   xj.nout = 1;
   for (int d = 0; d < 3; ++d) {
     xj.in[d] = ws[d];
-    xj.term[d][0] = term(-scale, 1, d == 0, d == 1, d == 2);
+    xj.term[d][0] =
+        with_pgd(term(-scale, 1, d == 0, d == 1, d == 2), pgd);
+    xj.term[d][0].lr_r2 = lr_r2;
   }
   xj.out[0] = gdk;
   xj.do_fwd = 1;
@@ -618,7 +623,7 @@ static long long brick_floats(const Plan& p) {
     const MeshDims d = paint_dims(p, N);
     for (int C = 1; C <= 3; C += 2) {
       const BrickGrid g =
-          make_brick_grid(p.d.nx / f, p.d.ny / f, p.d.nz / f, f, C);
+          make_brick_grid(p.d.nx / f, p.d.ny / f, p.d.nz / f, f);
       m = std::max(m, brick_work_floats(d, g, C));
     }
   }
@@ -650,7 +655,7 @@ static int paint_meshes(const Plan& p, float* const* mesh, int C,
   if ((float)fi == f && fi >= 1 && p.d.nx % fi == 0 && p.d.ny % fi == 0 &&
       p.d.nz % fi == 0) {
     const BrickGrid g =
-        make_brick_grid(p.d.nx / fi, p.d.ny / fi, p.d.nz / fi, fi, C);
+        make_brick_grid(p.d.nx / fi, p.d.ny / fi, p.d.nz / fi, fi);
     if (brick_paint_ok(md, g) && brick_work_floats(md, g, C) <= brick_floats(p))
       return launch_brick_paint(mesh, C, X, w, wstride, wscale, f, md, g,
                                 brick, s);
@@ -839,8 +844,10 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
   const Mesh3 m = m3(f4);
   if (!fm) PMX_TRY(force_at(p, X, N, f, nw, s, pgd));
   float* gms[3] = {gm, gm + M, gm + 2 * M};
-  const bool grid = paint_meshes(p, gms, 3, X, gc, 3, fac / f, N, f,
-                                 nw.brick, s) == kOk;
+  const int rc3 =
+      paint_meshes(p, gms, 3, X, gc, 3, fac / f, N, f, nw.brick, s);
+  if (rc3 != kOk && rc3 != kUnsupported) return rc3;
+  const bool grid = rc3 == kOk;
   const double inv_nbar = (double)Mb / (double)N;
   const float sc = (float)(inv_nbar / (double)Mb);
   if (grid) {
@@ -919,6 +926,22 @@ static long long pgd_tape_floats(const Plan& p, long long N, int nsteps) {
 __global__ void k_zero_dbl(double* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
+}
+
+// out[0] += sum_i a[i] b[i] in float64: block partials, one atomic each
+__global__ __launch_bounds__(256) void k_dot(const float* __restrict__ a,
+                                             const float* __restrict__ b,
+                                             long long n, double* out) {
+  __shared__ double part[4];
+  double acc = 0.0;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    acc += (double)a[i] * (double)b[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, (part[0] + part[1]) + (part[2] + part[3]));
 }
 
 
@@ -1000,38 +1023,54 @@ static int adjoint_open(const Plan& p, long long N, int nsteps,
   return kOk;
 }
 
+// The recomputation tape of a checkpointed reverse sweep (inside its
+// workspace): X and the force meshes of the steps of one segment, and the
+// jump-start force.
+static Tape segment_tape(const Plan& p, long long N, int every, float* work,
+                         int b0) {
+  const long long M = p.d.rows * p.d.nz;
+  const long long BN3 = (long long)p.d.B * N * 3;
+  float* gm = work + nbody_work_floats(p);
+  float* st = gm + 3 * M;            // scratch state (3, B, N, 3)
+  Tape tp;
+  tp.xs = st + 3 * BN3;              // every * BN3
+  tp.fs = tp.xs + (long long)every * BN3;  // every * 4M
+  tp.fj = tp.fs + (long long)every * 4 * M;
+  tp.first = b0;
+  return tp;
+}
+
+// last_taped: the forward run already recorded the last segment's tape in
+// work (nbody_run_last_segment), so that segment is not recomputed.
 static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
                          int nsteps, long long N, float f, float fac,
                          const double* coef, float* gstate, float* work,
-                         hipStream_t s) {
+                         hipStream_t s, bool last_taped = false) {
   PMX_REQUIRE(every >= 1, "nbody_adjoint: every must be >= 1");
   const long long M = p.d.rows * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
   NbodyWork nw = nbody_work(p, work);
   float* gm = work + nbody_work_floats(p);
   float* st = gm + 3 * M;            // scratch state (3, B, N, 3)
-  float* xslots = st + 3 * BN3;      // every * BN3
-  float* fslots = xslots + (long long)every * BN3;  // (every + 1) * 4M
-  float* fjump = fslots + (long long)every * 4 * M;
   const int nseg = (nsteps + every - 1) / every;
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int b0 = sg * every, b1 = std::min(nsteps, b0 + every);
-    {
-      ProfScope prof_(kProfAdjUpdate, s);
-      PMX_HIP_CHECK(hipMemcpyAsync(st, ckpt + (long long)sg * 3 * BN3,
-                                   sizeof(float) * 3 * BN3,
-                                   hipMemcpyDeviceToDevice, s));
+    const Tape tp = segment_tape(p, N, every, work, b0);
+    if (!(last_taped && sg == nseg - 1)) {
+      {
+        ProfScope prof_(kProfAdjUpdate, s);
+        PMX_HIP_CHECK(hipMemcpyAsync(st, ckpt + (long long)sg * 3 * BN3,
+                                     sizeof(float) * 3 * BN3,
+                                     hipMemcpyDeviceToDevice, s));
+      }
+      // recompute the segment, keeping X and the force meshes of every step
+      PMX_TRY(nbody_run(p, st, N, f, fac, coef, b0, b1, sg == 0, work, s,
+                        &tp));
     }
-    // recompute the segment, keeping X and the force meshes of every step
-    Tape tp;
-    tp.fj = fjump;
-    tp.xs = xslots;
-    tp.fs = fslots;
-    tp.first = b0;
-    PMX_TRY(nbody_run(p, st, N, f, fac, coef, b0, b1, sg == 0, work, s, &tp));
     PMX_TRY(reverse_steps(p, tp, b0, b1, N, f, fac, coef, gstate, nw, gm, s));
   }
+  float* fjump = segment_tape(p, N, every, work, 0).fj;
   // jump-start force at the initial positions; the initial F is discarded
   AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
   particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
@@ -1300,6 +1339,136 @@ int pmx_fft_c2r_longrange_grad(const pmx_plan* plan, const float* dk,
   return poisson_grad(p, reinterpret_cast<const float2*>(dk), 0, w,
                       w + p.d.S, w + 2 * p.d.S, fx, fy, fz, scale, S(stream),
                       nullptr, r_split * r_split);
+}
+
+int pmx_fft_longrange_grad_adj(const pmx_plan* plan, const float* gfx,
+                               const float* gfy, const float* gfz,
+                               float* work, float* gdk, float scale,
+                               double r_split, void* stream) {
+  PMX_REQUIRE(plan && gfx && gfy && gfz && work && gdk,
+              "pmx_fft_longrange_grad_adj: null pointer");
+  return poisson_grad_adj(P(plan), gfx, gfy, gfz,
+                          reinterpret_cast<float2*>(work),
+                          reinterpret_cast<float2*>(gdk), scale, S(stream),
+                          nullptr, r_split * r_split);
+}
+
+int pmx_fft_c2r_pgd_grad(const pmx_plan* plan, const float* dk, float* work,
+                         float* fx, float* fy, float* fz, float scale,
+                         float kl, float ks, int deriv, void* stream) {
+  PMX_REQUIRE(plan && dk && work && fx && fy && fz,
+              "pmx_fft_c2r_pgd_grad: null pointer");
+  PMX_REQUIRE(kl != 0.0f && ks != 0.0f,
+              "pmx_fft_c2r_pgd_grad: kl and ks must be nonzero");
+  PMX_REQUIRE(deriv >= 0 && deriv <= 2,
+              "pmx_fft_c2r_pgd_grad: deriv must be 0, 1 or 2");
+  const Plan& p = P(plan);
+  float2* w = reinterpret_cast<float2*>(work);
+  const PgdTerm pt{kl, ks, 1 + deriv};
+  return poisson_grad(p, reinterpret_cast<const float2*>(dk), 0, w,
+                      w + p.d.S, w + 2 * p.d.S, fx, fy, fz, scale, S(stream),
+                      &pt);
+}
+
+int pmx_fft_pgd_grad_adj(const pmx_plan* plan, const float* gfx,
+                         const float* gfy, const float* gfz, float* work,
+                         float* gdk, float scale, float kl, float ks,
+                         void* stream) {
+  PMX_REQUIRE(plan && gfx && gfy && gfz && work && gdk,
+              "pmx_fft_pgd_grad_adj: null pointer");
+  PMX_REQUIRE(kl != 0.0f && ks != 0.0f,
+              "pmx_fft_pgd_grad_adj: kl and ks must be nonzero");
+  const PgdTerm pt{kl, ks, 1};
+  return poisson_grad_adj(P(plan), gfx, gfy, gfz,
+                          reinterpret_cast<float2*>(work),
+                          reinterpret_cast<float2*>(gdk), scale, S(stream),
+                          &pt);
+}
+
+int pmx_mesh_dot(const float* a, const float* b, long long n, double* out,
+                 void* stream) {
+  PMX_REQUIRE(a && b && out, "pmx_mesh_dot: null pointer");
+  PMX_REQUIRE(n >= 0, "pmx_mesh_dot: n < 0");
+  hipStream_t s = S(stream);
+  hipLaunchKernelGGL(k_zero_dbl, dim3(1), dim3(64), 0, s, out, 1);
+  PMX_LAUNCH_CHECK();
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_dot, dim3(std::max(1u, std::min(gridfor(n), 2048u))),
+                     dim3(256), 0, s, a, b, n, out);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int pmx_pgd(const pmx_plan* plan, float* state, long long N, float f,
+            float alpha, float kl, float ks, float* work, void* stream) {
+  PMX_REQUIRE(plan && state && work, "pmx_pgd: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_pgd: need at least one particle");
+  PMX_REQUIRE(kl != 0.0f && ks != 0.0f, "pmx_pgd: kl and ks must be nonzero");
+  PMX_REQUIRE(f >= 1.0f, "pmx_pgd: pm_nc_factor must be >= 1");
+  const Plan& p = P(plan);
+  hipStream_t s = S(stream);
+  const NbodyWork nw = nbody_work(p, work);
+  const PgdTerm pt{kl, ks, 1};
+  MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
+  PMX_TRY(force_at(p, state, N, f, nw, s, &pt));
+  return launch_pgd_shift(state, m3(nw.f4), md, f, alpha / f, s);
+}
+
+long long pmx_pgd_adj_work_floats(const pmx_plan* plan, long long N) {
+  if (!plan || N < 1) return -1;
+  const Plan& p = P(plan);
+  return nbody_work_floats(p) + 7 * p.d.rows * p.d.nz;
+}
+
+int pmx_pgd_adjoint(const pmx_plan* plan, const float* X, long long N,
+                    float f, float alpha, float kl, float ks, float* gstate,
+                    double* gpgd, float* work, void* stream) {
+  PMX_REQUIRE(plan && X && gstate && work, "pmx_pgd_adjoint: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_pgd_adjoint: need at least one particle");
+  PMX_REQUIRE(kl != 0.0f && ks != 0.0f,
+              "pmx_pgd_adjoint: kl and ks must be nonzero");
+  PMX_REQUIRE(f >= 1.0f, "pmx_pgd_adjoint: pm_nc_factor must be >= 1");
+  const Plan& p = P(plan);
+  hipStream_t s = S(stream);
+  int gx, gy, gz;
+  particle_grid(p, N, f, &gx, &gy, &gz);
+  PMX_REQUIRE(gx > 0, "pmx_pgd_adjoint: the particles must form the "
+                      "Lagrangian grid of the mesh / pm_nc_factor");
+  const long long M = p.d.rows * p.d.nz;
+  const long long BN3 = 3LL * p.d.B * N;
+  const NbodyWork nw = nbody_work(p, work);
+  float* gm = work + nbody_work_floats(p);  // 3 cotangent meshes
+  float* fpm = gm + 3 * M;                  // PGD meshes at X (float4)
+  NbodyWork nwp = nw;
+  nwp.f4 = fpm;
+  const PgdTerm pt{kl, ks, 1};
+  PMX_TRY(force_at(p, X, N, f, nwp, s, &pt));
+  float* gX = gstate;
+  float* gP = gstate + BN3;
+  float* gF = gstate + 2 * BN3;
+  if (gpgd) {
+    hipLaunchKernelGGL(k_zero_dbl, dim3(1), dim3(64), 0, s, gpgd, 3);
+    PMX_LAUNCH_CHECK();
+    // before gX is updated: gX is the cotangent of the displacement
+    PMX_TRY(pgd_param_grads(p, X, gX, fpm, alpha, kl, ks, N, f, nw, gpgd, s));
+  }
+  AdjUpdate u{};
+  u.mode = 0;
+  u.gx = gx;
+  u.gy = gy;
+  u.gz = gz;
+  return force_vjp(p, X, gF, gX, gP, N, f, alpha, nw, gm, fpm, u, s, &pt, gX);
+}
+
+int pmx_power_spectrum_bins_adj(const pmx_plan* plan, float* spec,
+                                const double* kx, const double* ky,
+                                const double* kz, const double* edges,
+                                int nedges, const double* gbin,
+                                void* stream) {
+  PMX_REQUIRE(plan && spec && kx && ky && kz && edges && gbin,
+              "pmx_power_spectrum_bins_adj: null pointer");
+  return launch_power_bins_adj(P(plan), reinterpret_cast<float2*>(spec), kx,
+                               ky, kz, edges, nedges, gbin, S(stream));
 }
 
 int pmx_force_meshes(const pmx_plan* plan, const float* rho, float* work,
@@ -1712,6 +1881,34 @@ int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
               "pmx_nbody_adjoint: bad steps");
   return nbody_adjoint(P(plan), ckpt, every, nsteps, N, f, fac, coef, gstate,
                        work, S(stream));
+}
+
+int pmx_nbody_run_last_segment(const pmx_plan* plan, float* state,
+                               long long N, float f, float fac,
+                               const double* coef, int nsteps, int every,
+                               float* work, void* stream) {
+  PMX_REQUIRE(plan && state && work && (coef || nsteps == 0),
+              "pmx_nbody_run_last_segment: null pointer");
+  PMX_REQUIRE(N >= 1 && nsteps >= 1 && every >= 1,
+              "pmx_nbody_run_last_segment: need particles, steps and "
+              "every >= 1");
+  const int nseg = (nsteps + every - 1) / every;
+  const int b0 = (nseg - 1) * every;
+  const Tape tp = segment_tape(P(plan), N, every, work, b0);
+  return nbody_run(P(plan), state, N, f, fac, coef, b0, nsteps, nseg == 1,
+                   work, S(stream), &tp);
+}
+
+int pmx_nbody_adjoint_last_taped(const pmx_plan* plan, const float* ckpt,
+                                 int every, int nsteps, long long N, float f,
+                                 float fac, const double* coef,
+                                 float* gstate, float* work, void* stream) {
+  PMX_REQUIRE(plan && ckpt && gstate && work,
+              "pmx_nbody_adjoint_last_taped: null pointer");
+  PMX_REQUIRE(nsteps >= 1 && coef,
+              "pmx_nbody_adjoint_last_taped: bad steps");
+  return nbody_adjoint(P(plan), ckpt, every, nsteps, N, f, fac, coef, gstate,
+                       work, S(stream), true);
 }
 
 }  // extern "C"

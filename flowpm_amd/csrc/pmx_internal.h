@@ -16,6 +16,8 @@
 #include <cstdint>
 #include <string>
 
+#include "brick_index.h"
+
 namespace pmx {
 
 // ------------------------------------------------------------ errors ------
@@ -284,19 +286,16 @@ constexpr int kMaxPkBins = 8192;
 int launch_power_bins(const Plan& p, const float2* spec, const double* kx,
                       const double* ky, const double* kz, const double* edges,
                       int nedges, double* psum, hipStream_t s, int count = 0);
+// spec *= g[b][bin] x the column weight (adjoint of launch_power_bins)
+int launch_power_bins_adj(const Plan& p, float2* spec, const double* kx,
+                          const double* ky, const double* kz,
+                          const double* edges, int nedges, const double* g,
+                          hipStream_t s);
 
 // Lagrangian-brick paint (paint.hip).
-struct BrickGrid {
-  int gx, gy, gz;     // particle grid (N = gx*gy*gz per batch member)
-  int ex, ey, ez;     // brick size in particles
-  int nbx, nby, nbz;  // bricks per dimension
-  int nbricks;
-  int f;              // mesh cells per particle spacing (pm_nc_factor)
-};
-BrickGrid make_brick_grid(int gx, int gy, int gz, int f, int C);
 bool brick_paint_ok(const MeshDims& d, const BrickGrid& g);
 long long brick_work_floats(const MeshDims& d, const BrickGrid& g, int C);
-// mesh[c] (zeroed by the caller) += sum_p w[p*wstride+c]*wscale * CIC(X*ps)
+// mesh[c] = sum_p w[p*wstride+c]*wscale * CIC(X*ps) (every cell written)
 int launch_brick_paint(float* const* mesh, int C, const float* X,
                        const float* w, int wstride, float wscale, float ps,
                        const MeshDims& d, const BrickGrid& g, float* work,

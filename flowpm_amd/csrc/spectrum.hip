@@ -131,4 +131,80 @@ int launch_power_bins(const Plan& p, const float2* spec, const double* kx,
   return kOk;
 }
 
+// Adjoint of the binning (the gradient of pk.py:40-103 w.r.t. the field):
+// for bin sums P_b = sum_{modes in b} W |F|^2, d L / d x =
+// 2 Re IDFT_unnorm(sum_k g_b(k) W_k F_k) over the full spectrum.  W is not
+// symmetric under k -> -k, but the real part keeps only the symmetric part
+// (W_k + W_-k) / 2, which on the half spectrum is the binning weight of the
+// column (as in k_pk_bin) halved where the column stands for two full
+// modes.  So spec[mode] *= g[b][bin] * Wh(j) * (0 < j < nz/2 ? 1/2 : 1) in
+// place; a c2r with scale 2 then gives the field gradient.  One thread per
+// mode (coalesced), bins by the same float64 digitize as k_pk_bin.
+struct PkAdjArgs {
+  float2* spec;
+  const double* kx;
+  const double* ky;
+  const double* kz;
+  const double* edges;
+  int nedges;
+  const double* g;  // [B][nedges + 1]
+  PlanDev d;
+};
+
+__global__ __launch_bounds__(256) void k_pk_bin_adj(PkAdjArgs a) {
+#pragma clang fp contract(off)
+  const PlanDev& d = a.d;
+  const int b = blockIdx.y;
+  const long long rows = (long long)d.nx * d.ny;
+  const int H = d.H;
+  const long long nmain = rows * H;
+  const double* g = a.g + (long long)b * (a.nedges + 1);
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       e < nmain + rows; e += (long long)gridDim.x * blockDim.x) {
+    long long r;
+    int j;
+    float2* dst;
+    if (e < nmain) {
+      r = e / H;
+      j = (int)(e % H);
+      dst = a.spec + ((long long)b * rows + r) * H + j;
+    } else {
+      r = e - nmain;
+      j = H;
+      dst = a.spec + d.nyq0 + (long long)b * rows + r;
+    }
+    const int x = (int)(r / d.ny), y = (int)(r % d.ny);
+    const double kk =
+        (a.kx[x] * a.kx[x] + a.ky[y] * a.ky[y]) + a.kz[j] * a.kz[j];
+    const int bin = digitize(sqrt(kk), a.edges, a.nedges);
+    double w = (double)pk_w(j, d.nz);
+    if (j > 0 && j < H) w = 0.5 * (w + (double)pk_w(d.nz - j, d.nz));
+    const float m = (float)(g[bin] * w);
+    const float2 v = *dst;
+    *dst = make_float2(v.x * m, v.y * m);
+  }
+}
+
+int launch_power_bins_adj(const Plan& p, float2* spec, const double* kx,
+                          const double* ky, const double* kz,
+                          const double* edges, int nedges, const double* g,
+                          hipStream_t s) {
+  PMX_REQUIRE(nedges >= 1 && nedges + 1 <= kMaxPkBins,
+              "power spectrum: 1 <= number of edges < 8192");
+  PkAdjArgs a;
+  a.spec = spec;
+  a.kx = kx;
+  a.ky = ky;
+  a.kz = kz;
+  a.edges = edges;
+  a.nedges = nedges;
+  a.g = g;
+  a.d = p.d;
+  const long long n = (long long)p.d.nx * p.d.ny * (p.d.H + 1);
+  const unsigned gx = (unsigned)std::min<long long>((n + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_pk_bin_adj, dim3(gx, p.d.B), dim3(256), 0, s, a);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
 }  // namespace pmx

@@ -9,16 +9,16 @@ accounted on the half spectrum); the weighted mode counts come from the
 same binning pass over W alone (``pmx_power_spectrum_counts``, exact
 integers in float64, equal to the reference's bincount) instead of a
 full-mesh NumPy digitize; the normalisation follows the reference's float32
-steps.  Not differentiable (the reference's bincount
-path is used as a diagnostic / loss on fixed bins; this one returns a
-detached tensor).
+steps.  Differentiable w.r.t. the field through the hand-written adjoint
+of the binning (``pmx_power_spectrum_bins_adj``), as the PGD fit needs
+(scripts/fit_pgd.py:44-70).
 """
 import numpy as np
 import torch
 
 from . import _lib
 from . import _plans
-from .utils import _f32, _mesh_dims, fwd_real
+from .utils import _f32, _mesh_dims, fwd_real, inv_real
 
 __all__ = ["power_spectrum"]
 
@@ -55,11 +55,47 @@ def _initialize_pk(shape, boxsize, kmin, dk):
   return k, kedges, Nsum
 
 
+class _PowerBins(torch.autograd.Function):
+  """psum[b][i] = sum over bin i of W |FFT3(field_b)|^2 (float64,
+  pmx_power_spectrum_bins).  Backward: for a bin cotangent g, the field
+  cotangent is 2 Re IDFT_unnorm(g_bin(k) W_k F_k) (the derivative of the
+  bincount is a gather, of |F|^2 w.r.t. the real field 2 Re conj(F) dF):
+  pmx_power_spectrum_bins_adj weights the kept spectrum, a c2r with scale
+  2 transforms it back."""
+
+  @staticmethod
+  def forward(ctx, field, plan, tabs):
+    kx, ky, kz, edges = tabs
+    spec = fwd_real(plan, _f32(field), 1.0)  # unnormalised, as fft3d
+    psum = torch.empty((plan.B, edges.numel() + 1), dtype=torch.float64,
+                       device=field.device)
+    _lib.call("pmx_power_spectrum_bins", plan.handle, _lib.ptr(spec),
+              _lib.ptr(kx), _lib.ptr(ky), _lib.ptr(kz), _lib.ptr(edges),
+              edges.numel(), _lib.ptr(psum), _lib.stream_of(field))
+    ctx.plan, ctx.tabs = plan, tabs
+    if field.requires_grad:
+      ctx.save_for_backward(spec)
+    return psum
+
+  @staticmethod
+  def backward(ctx, g):
+    plan = ctx.plan
+    kx, ky, kz, edges = ctx.tabs
+    (spec,) = ctx.saved_tensors
+    sp = spec.clone()
+    g = g.detach().to(torch.float64).contiguous()
+    _lib.call("pmx_power_spectrum_bins_adj", plan.handle, _lib.ptr(sp),
+              _lib.ptr(kx), _lib.ptr(ky), _lib.ptr(kz), _lib.ptr(edges),
+              edges.numel(), _lib.ptr(g), _lib.stream_of(sp))
+    return inv_real(plan, sp, 2.0), None, None
+
+
 def power_spectrum(field, kmin=5, dk=0.5, boxsize=False):
   """Binned power spectrum of a real (B, nx, ny, nz) field.
 
   Returns (kbins, power): kbins the NumPy bin centres, power a float32
-  (B, len(kbins)) tensor on the field's device (pk.py:40-103)."""
+  (B, len(kbins)) tensor on the field's device (pk.py:40-103);
+  differentiable w.r.t. ``field`` (the loss of scripts/fit_pgd.py:44-70)."""
   _lib.require_cuda(field)
   B, nx, ny, nz = _mesh_dims(field)
   if boxsize is False or boxsize is None:
@@ -72,18 +108,14 @@ def power_spectrum(field, kmin=5, dk=0.5, boxsize=False):
     return kedges[:0].astype(np.float64), torch.zeros(
         (B, 0), dtype=torch.float32, device=dev)
   plan = _plans.get(dev, B, nx, ny, nz)
-  spec = fwd_real(plan, _f32(field), 1.0)  # unnormalised, as fft3d
   tab = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64),
                                   device=dev)
-  kx, ky, kz, edges = tab(k[0]), tab(k[1]), tab(k[2]), tab(kedges)
-  psum = torch.empty((B, len(kedges) + 1), dtype=torch.float64, device=dev)
-  _lib.call("pmx_power_spectrum_bins", plan.handle, _lib.ptr(spec),
-            _lib.ptr(kx), _lib.ptr(ky), _lib.ptr(kz), _lib.ptr(edges),
-            len(kedges), _lib.ptr(psum), _lib.stream_of(field))
+  tabs = (tab(k[0]), tab(k[1]), tab(k[2]), tab(kedges))
+  psum = _PowerBins.apply(field, plan, tabs)
   nsum = torch.empty(len(kedges) + 1, dtype=torch.float64, device=dev)
-  _lib.call("pmx_power_spectrum_counts", plan.handle, _lib.ptr(kx),
-            _lib.ptr(ky), _lib.ptr(kz), _lib.ptr(edges), len(kedges),
-            _lib.ptr(nsum), _lib.stream_of(field))
+  _lib.call("pmx_power_spectrum_counts", plan.handle, _lib.ptr(tabs[0]),
+            _lib.ptr(tabs[1]), _lib.ptr(tabs[2]), _lib.ptr(tabs[3]),
+            len(kedges), _lib.ptr(nsum), _lib.stream_of(field))
   # float32 steps of pk.py:82-103: Psum (complex64) / Nsum (complex64),
   # [1:-1], * boxsize.prod(), / float32(M)**2
   power = psum.to(torch.float32) / nsum.to(torch.float32)

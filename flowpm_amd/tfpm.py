@@ -171,12 +171,113 @@ def _poisson_grad_meshes(plan, spec, split=0.0):
   return out
 
 
-def lpt1(dlin_k, pos, kvec=None, name="LTP1"):
-  """First-order LPT displacement read out at ``pos`` (tfpm.py:66-95)."""
-  B, nx, ny, nz = [int(s) for s in dlin_k.shape]
-  plan = _plans.get(dlin_k.device, B, nx, ny, nz)
-  meshes = _poisson_grad_meshes(plan, _full_to_real_half(plan, dlin_k))
+class _PoissonGradMeshes(torch.autograd.Function):
+  """(3, B, nx, ny, nz) meshes c2r3d(dk * lap [* exp(-k^2 split^2)] * i g_d)
+  of a full complex spectrum dk (tfpm.py:66-95, 183-219).  The gradient of
+  a real loss w.r.t. dk (torch's convention dL/dRe + i dL/dIm) is
+  sum_d conj(K_d) FFT3(g_d), K_d the mode's multiplier, a Hermitian
+  spectrum (pmx_fft_poisson_grad_adj / pmx_fft_longrange_grad_adj)."""
+
+  @staticmethod
+  def forward(ctx, dk, split):
+    _lib.require_cuda(dk)
+    B, nx, ny, nz = [int(s) for s in dk.shape]
+    plan = _plans.get(dk.device, B, nx, ny, nz)
+    ctx.plan, ctx.split = plan, float(split)
+    return _poisson_grad_meshes(plan, _full_to_real_half(plan, dk),
+                                split=float(split))
+
+  @staticmethod
+  def backward(ctx, g):
+    plan = ctx.plan
+    g = _f32(g)
+    gdk = plan.spec()
+    work = plan.spec(3)
+    st = _lib.stream_of(g)
+    if ctx.split == 0:
+      _lib.call("pmx_fft_poisson_grad_adj", plan.handle, _lib.ptr(g[0]),
+                _lib.ptr(g[1]), _lib.ptr(g[2]), _lib.ptr(work), _lib.ptr(gdk),
+                1.0, st)
+    else:
+      _lib.call("pmx_fft_longrange_grad_adj", plan.handle, _lib.ptr(g[0]),
+                _lib.ptr(g[1]), _lib.ptr(g[2]), _lib.ptr(work), _lib.ptr(gdk),
+                1.0, ctx.split, st)
+    return _half_to_full(plan, gdk), None
+
+
+def _dot(a, b):
+  """sum(a * b) of two float32 device tensors, in float64 on the GPU."""
+  out = torch.empty(1, dtype=torch.float64, device=a.device)
+  _lib.call("pmx_mesh_dot", _lib.ptr(a), _lib.ptr(b), a.numel(),
+            _lib.ptr(out), _lib.stream_of(a))
+  return out[0]
+
+
+class _PgdMeshes(torch.autograd.Function):
+  """The three meshes of apply_pgd (tfpm.py:222-270): c2r3d(dk * lap *
+  PGD_kernel(kl, ks) * i g_d).  Backward: the conjugate multiplier for dk
+  (pmx_fft_pgd_grad_adj) and, for kl and ks, the contraction of the mesh
+  cotangents with the meshes of dPGD/dkl and dPGD/dks."""
+
+  @staticmethod
+  def forward(ctx, dk, kl, ks):
+    _lib.require_cuda(dk)
+    B, nx, ny, nz = [int(s) for s in dk.shape]
+    plan = _plans.get(dk.device, B, nx, ny, nz)
+    klf, ksf = float(kl), float(ks)
+    if klf == 0 or ksf == 0:
+      raise ValueError("PGD scales kl and ks must be nonzero")
+    spec = _full_to_real_half(plan, dk)
+    out = _pgd_meshes(plan, spec, klf, ksf, 0)
+    ctx.plan, ctx.kl, ctx.ks = plan, klf, ksf
+    ctx.save_for_backward(spec)
+    ctx.ptype = [(p.dtype, p.device) if isinstance(p, torch.Tensor) else None
+                 for p in (kl, ks)]
+    return out
+
+  @staticmethod
+  def backward(ctx, g):
+    plan = ctx.plan
+    (spec,) = ctx.saved_tensors
+    g = _f32(g)
+    gdk = None
+    if ctx.needs_input_grad[0]:
+      sp = plan.spec()
+      work = plan.spec(3)
+      _lib.call("pmx_fft_pgd_grad_adj", plan.handle, _lib.ptr(g[0]),
+                _lib.ptr(g[1]), _lib.ptr(g[2]), _lib.ptr(work), _lib.ptr(sp),
+                1.0, ctx.kl, ctx.ks, _lib.stream_of(g))
+      gdk = _half_to_full(plan, sp)
+    gp = []
+    for q in (1, 2):
+      if ctx.needs_input_grad[q] and ctx.ptype[q - 1] is not None:
+        dt, dev = ctx.ptype[q - 1]
+        v = _dot(g, _pgd_meshes(plan, spec, ctx.kl, ctx.ks, q))
+        gp.append(v.to(dtype=dt, device=dev))
+      else:
+        gp.append(None)
+    return gdk, gp[0], gp[1]
+
+
+def _pgd_meshes(plan, spec, kl, ks, deriv):
+  out = torch.empty((3, plan.B, plan.nx, plan.ny, plan.nz),
+                    dtype=torch.float32, device=spec.device)
+  work = plan.spec(3)
+  _lib.call("pmx_fft_c2r_pgd_grad", plan.handle, _lib.ptr(spec),
+            _lib.ptr(work), _lib.ptr(out[0]), _lib.ptr(out[1]),
+            _lib.ptr(out[2]), 1.0, float(kl), float(ks), int(deriv),
+            _lib.stream_of(spec))
+  return out
+
+
+def _readout3(meshes, pos):
   return torch.stack([cic_readout(meshes[d], pos) for d in range(3)], dim=2)
+
+
+def lpt1(dlin_k, pos, kvec=None, name="LTP1"):
+  """First-order LPT displacement read out at ``pos`` (tfpm.py:66-95);
+  differentiable w.r.t. ``dlin_k`` and ``pos``."""
+  return _readout3(_PoissonGradMeshes.apply(dlin_k, 0.0), pos)
 
 
 def _lpt2_src(plan, spec):
@@ -188,12 +289,64 @@ def _lpt2_src(plan, spec):
   return src, work
 
 
+# exponents of (i g_x, i g_y, i g_z) of the six phi_ij transforms (xx, yy,
+# zz, then xy, xz, yz), the order pmx_lpt2_source_adj returns them in
+_PHI_EXPS = (((0, 1, 2), (2, 0, 0, 0, 2, 0, 0, 0, 2)),
+             ((3, 4, 5), (1, 1, 0, 1, 0, 1, 0, 1, 1)))
+
+
+def _phi_adjoint(plan, gphi, work, out, accumulate):
+  """out (+)= sum_ij IFFT3(lap (i g_i)(i g_j) FFT3(gphi_ij)) (normalised
+  inverse): the adjoint of the six phi_ij = c2r(dk lap (i g_i)(i g_j))
+  maps, in real space (the multipliers are real and even)."""
+  st = _lib.stream_of(gphi)
+  for k, (idx, exps) in enumerate(_PHI_EXPS):
+    ins = (_lib._vp * 3)(*[gphi[i].data_ptr() for i in idx])
+    e = (_lib._i * 9)(*exps)
+    sc = (_lib._f * 3)(1.0, 1.0, 1.0)
+    _lib.call("pmx_fourier_apply", plan.handle, 3, ins, e, sc, 1,
+              1.0 / plan.M, _lib.ptr(work), _lib.ptr(out),
+              1 if (accumulate or k > 0) else 0, st)
+
+
+class _Lpt2Source(torch.autograd.Function):
+  """lpt2_source (tfpm.py:98-145) with its hand-written adjoint: r2c3d's
+  conjugate transform, the pointwise adjoint of the source combination
+  (pmx_lpt2_source_adj), then the six phi_ij multipliers."""
+
+  @staticmethod
+  def forward(ctx, dk):
+    _lib.require_cuda(dk)
+    B, nx, ny, nz = [int(s) for s in dk.shape]
+    plan = _plans.get(dk.device, B, nx, ny, nz)
+    src, work2 = _lpt2_src(plan, _full_to_real_half(plan, dk))
+    ctx.plan = plan
+    ctx.save_for_backward(work2)
+    return _half_to_full(plan, src)
+
+  @staticmethod
+  def backward(ctx, g):
+    plan = ctx.plan
+    (work2,) = ctx.saved_tensors
+    M = plan.M
+    # cotangent of the real source: the adjoint of r2c3d(., norm = M)
+    gsrc = inv_real(plan, _full_to_real_half(plan, g), 1.0 / M)
+    dev = gsrc.device
+    gphi = torch.empty((6, plan.B, plan.nx, plan.ny, plan.nz),
+                       dtype=torch.float32, device=dev)
+    st = _lib.stream_of(gsrc)
+    _lib.call("pmx_lpt2_source_adj", plan.handle, _lib.ptr(work2),
+              _lib.ptr(gsrc), _lib.ptr(gphi), st)
+    # sum_ij K_ij FFT3(gphi_ij) as a spectrum: the real-space sum, then r2c
+    G = torch.empty_like(gsrc)
+    _phi_adjoint(plan, gphi, plan.spec(3), G, 0)
+    return _half_to_full(plan, fwd_real(plan, G, 1.0))
+
+
 def lpt2_source(dlin_k, kvec=None, name="LPT2Source"):
-  """Second-order LPT source term, full complex spectrum (tfpm.py:98-145)."""
-  B, nx, ny, nz = [int(s) for s in dlin_k.shape]
-  plan = _plans.get(dlin_k.device, B, nx, ny, nz)
-  src, _ = _lpt2_src(plan, _full_to_real_half(plan, dlin_k))
-  return _half_to_full(plan, src)
+  """Second-order LPT source term, full complex spectrum (tfpm.py:98-145);
+  differentiable w.r.t. ``dlin_k``."""
+  return _Lpt2Source.apply(dlin_k)
 
 
 def lpt_coefficients(cosmo, a, order):
@@ -261,14 +414,8 @@ class _LptInit(torch.autograd.Function):
       gphi = torch.empty((6, B, nx, ny, nz), dtype=torch.float32, device=dev)
       _lib.call("pmx_lpt2_source_adj", plan.handle, _lib.ptr(work2),
                 _lib.ptr(gsrc), _lib.ptr(gphi), st)
-      # phi_ij^T = c2r(lap (i g_i)(i g_j) r2c(.)/M): diagonal, then off-diag
-      for idx, exps in (((0, 1, 2), (2, 0, 0, 0, 2, 0, 0, 0, 2)),
-                        ((3, 4, 5), (1, 1, 0, 1, 0, 1, 0, 1, 1))):
-        ins = (_lib._vp * 3)(*[gphi[i].data_ptr() for i in idx])
-        e = (_lib._i * 9)(*exps)
-        sc = (_lib._f * 3)(1.0, 1.0, 1.0)
-        _lib.call("pmx_fourier_apply", plan.handle, 3, ins, e, sc, 1,
-                  1.0 / M, _lib.ptr(work), _lib.ptr(glin), 1, st)
+      # phi_ij^T = c2r(lap (i g_i)(i g_j) r2c(.)/M), added to glin
+      _phi_adjoint(plan, gphi, work, glin, 1)
     return glin, None, None
 
 
@@ -286,13 +433,72 @@ def apply_longrange(x, delta_k, split=0, factor=1, kvec=None,
                     name="ApplyLongrange"):
   """Long-range force at positions x from delta_k (tfpm.py:183-219); with
   split != 0 the Laplacian is filtered by exp(-k^2 split^2)
-  (kernels.py:81-102)."""
-  B, nx, ny, nz = [int(s) for s in delta_k.shape]
-  plan = _plans.get(delta_k.device, B, nx, ny, nz)
-  meshes = _poisson_grad_meshes(plan, _full_to_real_half(plan, delta_k),
-                                split=float(split))
-  f = torch.stack([cic_readout(meshes[d], x) for d in range(3)], dim=2)
-  return f * factor
+  (kernels.py:81-102).  Differentiable w.r.t. ``x`` and ``delta_k``."""
+  return _readout3(_PoissonGradMeshes.apply(delta_k, float(split)), x) * factor
+
+
+def apply_pgd(x, delta_k, alpha, kl, ks, kvec=None, name="ApplyPGD"):
+  """PGD displacement alpha * readout(c2r(delta_k lap PGD_kernel(kl, ks)
+  i g_d), x) (tfpm.py:222-270, kernels.py:105-134); differentiable w.r.t.
+  ``x``, ``delta_k`` and (as tensors) ``alpha``, ``kl``, ``ks``."""
+  return _readout3(_PgdMeshes.apply(delta_k, kl, ks), x) * alpha
+
+
+class _Pgd(torch.autograd.Function):
+  """One PGD correction of the state (tfpm.py:369-422), pmx_pgd forward and
+  pmx_pgd_adjoint backward (state cotangent and d/d(alpha, kl, ks))."""
+
+  @staticmethod
+  def forward(ctx, state, pgdparams, plan, N, f):
+    st = _f32(state).clone()
+    a, kl, ks = [float(v) for v in _pgd_row(pgdparams)]
+    X = st[0].clone()
+    _lib.call("pmx_pgd", plan.handle, _lib.ptr(st), N, f, a, kl, ks,
+              _lib.ptr(_work(plan)), _lib.stream_of(st))
+    ctx.save_for_backward(X)
+    ctx.args = (plan, N, f, a, kl, ks)
+    ctx.pmeta = (pgdparams.dtype, pgdparams.device, tuple(pgdparams.shape)) \
+        if isinstance(pgdparams, torch.Tensor) else None
+    return st
+
+  @staticmethod
+  def backward(ctx, g):
+    (X,) = ctx.saved_tensors
+    plan, N, f, a, kl, ks = ctx.args
+    gs = _f32(g).clone()
+    want = ctx.pmeta is not None and ctx.needs_input_grad[1]
+    gp = torch.empty(3, dtype=torch.float64, device=gs.device) \
+        if want else None
+    work = torch.empty(_lib.lib().pmx_pgd_adj_work_floats(plan.handle, N),
+                       dtype=torch.float32, device=gs.device)
+    _lib.call("pmx_pgd_adjoint", plan.handle, _lib.ptr(X), N, f, a, kl, ks,
+              _lib.ptr(gs), _lib.ptr(gp), _lib.ptr(work), _lib.stream_of(gs))
+    gpgd = None
+    if want:
+      dt, dev, shape = ctx.pmeta
+      gpgd = gp.reshape(shape).to(dtype=dt, device=dev)
+    return gs, gpgd, None, None, None
+
+
+def _pgd_row(pgdparams):
+  if isinstance(pgdparams, torch.Tensor):
+    pgdparams = pgdparams.detach().cpu().numpy()
+  arr = np.asarray(pgdparams, dtype=np.float32).reshape(-1)
+  if arr.size != 3:
+    raise ValueError("pgdparams must hold (alpha, kl, ks)")
+  if arr[1] == 0 or arr[2] == 0:
+    raise ValueError("PGD scales kl and ks must be nonzero")
+  return arr
+
+
+def pgd(state, pgdparams, nc, pm_nc_factor=1, name="PGD", **kwargs):
+  """One PGD correction X += apply_pgd(f X, delta_k(X), alpha, kl, ks) / f
+  (tfpm.py:369-422); differentiable w.r.t. ``state`` and (as a tensor)
+  ``pgdparams`` = (alpha, kl, ks).  The gradient needs the particles on the
+  Lagrangian grid of nc (the nbody state)."""
+  _lib.require_cuda(state)
+  plan, B, N, f, _ = _force_setup(None, state, nc, pm_nc_factor)
+  return _Pgd.apply(state, pgdparams, plan, N, f)
 
 
 def _kick_fac(cosmo, ai, ac, af):
@@ -312,24 +518,44 @@ def _state_dims(state):
   return int(state.shape[1]), int(state.shape[2])
 
 
-def _kick_drift(state, k2=0.0, k1=0.0, d=0.0, flags=0):
-  _lib.require_cuda(state)
-  B, N = _state_dims(state)
-  st = _f32(state).clone()
-  _lib.call("pmx_readout_kick_drift", _lib.ptr(st[0]), _lib.ptr(st[1]),
-            _lib.ptr(st[2]), None, None, None, B, N, 1, 1, 1, 1.0, 0.0,
-            float(k2), float(k1), float(d), flags | 4, _lib.stream_of(st))
-  return st
+def _axpy(st, dst, src, c):
+  """st[dst] += c * st[src] on a (3, B, N, 3) tensor, in place (the
+  kick-only path of pmx_readout_kick_drift)."""
+  B, N = int(st.shape[1]), int(st.shape[2])
+  _lib.call("pmx_readout_kick_drift", _lib.ptr(st[0]), _lib.ptr(st[dst]),
+            _lib.ptr(st[src]), None, None, None, B, N, 1, 1, 1, 1.0, 0.0,
+            float(c), 0.0, 0.0, 1 | 4, _lib.stream_of(st))
+
+
+class _Axpy(torch.autograd.Function):
+  """kick (P += c F, dst 1, src 2) and drift (X += c P, dst 0, src 1):
+  linear, so the cotangent update is g[src] += c g[dst]."""
+
+  @staticmethod
+  def forward(ctx, state, dst, src, c):
+    _lib.require_cuda(state)
+    _state_dims(state)
+    st = _f32(state).clone()
+    _axpy(st, dst, src, c)
+    ctx.args = (dst, src, c)
+    return st
+
+  @staticmethod
+  def backward(ctx, g):
+    dst, src, c = ctx.args
+    gs = _f32(g).clone()
+    _axpy(gs, src, dst, c)
+    return gs, None, None, None
 
 
 def kick(cosmo, state, ai, ac, af, dtype=torch.float32, name="Kick", **kw):
-  """P += fac F (tfpm.py:273-292)."""
-  return _kick_drift(state, k2=_kick_fac(cosmo, ai, ac, af), flags=1)
+  """P += fac F (tfpm.py:273-292); differentiable w.r.t. ``state``."""
+  return _Axpy.apply(state, 1, 2, _kick_fac(cosmo, ai, ac, af))
 
 
 def drift(cosmo, state, ai, ac, af, dtype=torch.float32, name="Drift", **kw):
-  """X += fac P (tfpm.py:295-314)."""
-  return _kick_drift(state, d=_drift_fac(cosmo, ai, ac, af), flags=2)
+  """X += fac P (tfpm.py:295-314); differentiable w.r.t. ``state``."""
+  return _Axpy.apply(state, 0, 1, _drift_fac(cosmo, ai, ac, af))
 
 
 def _force_setup(cosmo, state, nc, pm_nc_factor):
@@ -340,7 +566,7 @@ def _force_setup(cosmo, state, nc, pm_nc_factor):
                      (N, nc[0] * nc[1] * nc[2]))
   f = int(pm_nc_factor)
   plan = _plans.get(state.device, B, nc[0] * f, nc[1] * f, nc[2] * f)
-  fac = float(np.float32(1.5 * cosmo.Omega_m))
+  fac = float(np.float32(1.5 * cosmo.Omega_m)) if cosmo is not None else 0.0
   return plan, B, N, float(f), fac
 
 
@@ -349,11 +575,47 @@ def _work(plan):
                      dtype=torch.float32, device=plan.device)
 
 
+class _Force(torch.autograd.Function):
+  """F = force(X) (tfpm.py:317-366) as a taped run of zero steps: the
+  forward keeps X and the force mesh, the backward is the jump-start force
+  VJP of the nbody reverse sweep (gX += J^T gF, gF = 0).  The reverse
+  sweep rewrites only the fourth (density cotangent) component of the
+  taped force mesh, so the tape also serves a second backward pass
+  (retain_graph=True); it is released with the graph otherwise."""
+
+  @staticmethod
+  def forward(ctx, state, plan, N, f, fac):
+    st = _f32(state).clone()
+    tape = torch.empty(_lib.lib().pmx_nbody_tape_floats(plan.handle, N, 0),
+                       dtype=torch.float32, device=st.device)
+    _lib.call("pmx_nbody_run_taped", plan.handle, _lib.ptr(st), N, f, fac,
+              None, 0, _lib.ptr(tape), _lib.ptr(_work(plan)),
+              _lib.stream_of(st))
+    ctx.save_for_backward(tape)
+    ctx.args = (plan, N, f, fac)
+    return st
+
+  @staticmethod
+  def backward(ctx, g):
+    plan, N, f, fac = ctx.args
+    (tape,) = ctx.saved_tensors
+    gs = _f32(g).clone()
+    work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
+                                                            0),
+                       dtype=torch.float32, device=gs.device)
+    _lib.call("pmx_nbody_adjoint_taped", plan.handle, _lib.ptr(tape), 0, N,
+              f, fac, None, _lib.ptr(gs), _lib.ptr(work), _lib.stream_of(gs))
+    return gs, None, None, None, None
+
+
 def force(cosmo, state, nc, pm_nc_factor=1, dtype=torch.float32, name="Force",
           **kw):
-  """Replaces F by the PM force at X (tfpm.py:317-366)."""
+  """Replaces F by the PM force at X (tfpm.py:317-366); differentiable
+  w.r.t. ``state``."""
   _lib.require_cuda(state)
   plan, B, N, f, fac = _force_setup(cosmo, state, nc, pm_nc_factor)
+  if torch.is_grad_enabled() and state.requires_grad:
+    return _Force.apply(state, plan, N, f, fac)
   st = _f32(state).clone()
   _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, None, 0, 0,
             1, _lib.ptr(_work(plan)), _lib.stream_of(st))
@@ -402,18 +664,31 @@ class _Nbody(torch.autograd.Function):
                          dtype=torch.float32, device=st.device)
       _lib.call("pmx_nbody_run_taped", plan.handle, _lib.ptr(st), N, f, fac,
                 cptr, nsteps, _lib.ptr(tape), _lib.ptr(work), stream)
-      ctx.tape = tape
+      # released with the graph; the reverse sweep rewrites only the fourth
+      # (density cotangent) component of its force meshes, so a second
+      # backward pass (retain_graph=True) reads an intact tape
+      ctx.save_for_backward(tape)
       ctx.args = (plan, N, f, fac, coef, every)
       return st
     nseg = max(1, (nsteps + every - 1) // every)
     ckpt = torch.empty((nseg, 3, B, N, 3), dtype=torch.float32,
                        device=st.device)
+    # the reverse sweep's workspace holds the recomputation tape of one
+    # segment; the last segment is run into it here, so the backward pass
+    # recomputes the other segments only
+    k = min(every, max(nsteps, 1))
+    adj = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N, k),
+                      dtype=torch.float32, device=st.device)
     for s in range(nseg):
       ckpt[s].copy_(st)
       b0, b1 = s * every, min(nsteps, (s + 1) * every)
-      _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
-                b0, b1, 1 if s == 0 else 0, _lib.ptr(work), stream)
-    ctx.save_for_backward(ckpt)
+      if s < nseg - 1 or nsteps == 0:
+        _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac,
+                  cptr, b0, b1, 1 if s == 0 else 0, _lib.ptr(work), stream)
+      else:
+        _lib.call("pmx_nbody_run_last_segment", plan.handle, _lib.ptr(st), N,
+                  f, fac, cptr, nsteps, k, _lib.ptr(adj), stream)
+    ctx.save_for_backward(ckpt, adj)
     ctx.args = (plan, N, f, fac, coef, every)
     return st
 
@@ -423,23 +698,28 @@ class _Nbody(torch.autograd.Function):
     nsteps = len(coef) // 3
     gs = _f32(g).clone()
     if every == 0:
-      tape = ctx.tape
+      (tape,) = ctx.saved_tensors
       work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
                                                               0),
                          dtype=torch.float32, device=gs.device)
       _lib.call("pmx_nbody_adjoint_taped", plan.handle, _lib.ptr(tape),
                 nsteps, N, f, fac, _lib.dbl(coef), _lib.ptr(gs),
                 _lib.ptr(work), _lib.stream_of(gs))
-      del ctx.tape
       return gs, None, None, None, None, None, None, None
-    (ckpt,) = ctx.saved_tensors
+    ckpt, adj = ctx.saved_tensors
     k = min(every, max(nsteps, 1))
-    work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
-                                                            k),
-                       dtype=torch.float32, device=gs.device)
-    _lib.call("pmx_nbody_adjoint", plan.handle, _lib.ptr(ckpt), k, nsteps,
-              N, f, fac, _lib.dbl(coef), _lib.ptr(gs), _lib.ptr(work),
-              _lib.stream_of(gs))
+    if nsteps > 0 and not getattr(ctx, "tape_used", False):
+      _lib.call("pmx_nbody_adjoint_last_taped", plan.handle, _lib.ptr(ckpt),
+                k, nsteps, N, f, fac, _lib.dbl(coef), _lib.ptr(gs),
+                _lib.ptr(adj), _lib.stream_of(gs))
+    else:
+      # a second backward pass (retain_graph): the sweep has since recomputed
+      # the earlier segments into the same workspace, so every segment is
+      # recomputed
+      _lib.call("pmx_nbody_adjoint", plan.handle, _lib.ptr(ckpt), k, nsteps,
+                N, f, fac, _lib.dbl(coef), _lib.ptr(gs), _lib.ptr(adj),
+                _lib.stream_of(gs))
+    ctx.tape_used = True
     return gs, None, None, None, None, None, None, None
 
 
@@ -484,7 +764,8 @@ class _NbodyPgd(torch.autograd.Function):
     _lib.call("pmx_nbody_run_pgd_taped", plan.handle, _lib.ptr(st), N, f,
               fac, _lib.dbl(coef), pgd, nsteps, _lib.ptr(tape),
               _lib.ptr(work), _lib.stream_of(st))
-    ctx.tape, ctx.pgd = tape, pgd
+    ctx.save_for_backward(tape)
+    ctx.pgd = pgd
     ctx.args = (plan, N, f, fac, coef)
     ctx.pgd_is_tensor = isinstance(pgdparams, torch.Tensor)
     ctx.pgd_meta = (pgdparams.dtype, pgdparams.device,
@@ -502,10 +783,10 @@ class _NbodyPgd(torch.autograd.Function):
     want = ctx.pgd_is_tensor and ctx.needs_input_grad[1]
     gp = torch.empty(max(3 * nsteps, 1), dtype=torch.float64,
                      device=gs.device) if want else None
-    _lib.call("pmx_nbody_adjoint_pgd_taped", plan.handle, _lib.ptr(ctx.tape),
+    (tape,) = ctx.saved_tensors
+    _lib.call("pmx_nbody_adjoint_pgd_taped", plan.handle, _lib.ptr(tape),
               nsteps, N, f, fac, _lib.dbl(coef), ctx.pgd, _lib.ptr(gs),
               _lib.ptr(gp), _lib.ptr(work), _lib.stream_of(gs))
-    del ctx.tape
     gpgd = None
     if want:
       dt, dev, shape = ctx.pgd_meta
@@ -532,10 +813,12 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
   pgd = None
   if pgdparams is not None:
     pgd = _pgd_table(pgdparams, nsteps)
-    if torch.is_grad_enabled() and (
-        state.requires_grad or getattr(pgdparams, "requires_grad", False)):
-      return _NbodyPgd.apply(state, pgdparams, plan, N, f, fac, coef)
+  grad = torch.is_grad_enabled() and (
+      state.requires_grad or getattr(pgdparams, "requires_grad", False))
   if return_intermediate_states:
+    if grad:
+      return _nbody_states_differentiable(state, stages, plan, N, f, fac,
+                                          coef, pgdparams, checkpoint_every)
     st = _f32(state).clone()
     work = _work(plan)
     cptr = _lib.dbl(coef)
@@ -546,6 +829,8 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
       out.append((stages[i + 1], st.clone()))
     return out
   if pgd is not None:
+    if grad:
+      return _NbodyPgd.apply(state, pgdparams, plan, N, f, fac, coef)
     st = _f32(state).clone()
     _run(plan, st, N, f, fac, _lib.dbl(coef), pgd, 0, nsteps, 1, _work(plan))
     return st
@@ -554,3 +839,37 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
   every = _resolve_every(plan, N, len(coef) // 3, every,
                          state.device) if need_grad else 1
   return _Nbody.apply(state, plan, N, f, fac, coef, every, need_grad)
+
+
+def _nbody_states_differentiable(state, stages, plan, N, f, fac, coef,
+                                 pgdparams, checkpoint_every):
+  """[(a_i, state_i)] with gradients: one differentiable nbody run per step
+  (row i of the run's coefficient table).  Every run starts with the
+  jump-start force at its input positions, which recomputes the force the
+  previous step stored (the same kernels on the same positions), so each
+  intermediate state equals the single-run one and the composition is the
+  same function of the initial state (its F is discarded either way,
+  tfpm.py:470)."""
+  out = []
+  st = state
+  for i in range(len(stages) - 1):
+    c = coef[3 * i:3 * i + 3]
+    if pgdparams is not None:
+      row = pgdparams[i:i + 1] if isinstance(pgdparams, torch.Tensor) else \
+          np.asarray(pgdparams, dtype=np.float32).reshape(-1, 3)[i:i + 1]
+      if torch.is_grad_enabled() and (st.requires_grad or
+                                      getattr(row, "requires_grad", False)):
+        st = _NbodyPgd.apply(st, row, plan, N, f, fac, c)
+      else:
+        s2 = _f32(st).clone()
+        _run(plan, s2, N, f, fac, _lib.dbl(c), _pgd_table(row, 1), 0, 1, 1,
+             _work(plan))
+        st = s2
+    else:
+      need = torch.is_grad_enabled() and st.requires_grad
+      every = checkpoint_every if checkpoint_every is not None else \
+          _CKPT_EVERY[0]
+      every = _resolve_every(plan, N, 1, every, st.device) if need else 1
+      st = _Nbody.apply(st, plan, N, f, fac, c, every, need)
+    out.append((stages[i + 1], st))
+  return out

@@ -113,6 +113,52 @@ int pmx_fft_c2r_poisson_grad(const pmx_plan* plan, const float* dk,
 int pmx_fft_c2r_longrange_grad(const pmx_plan* plan, const float* dk,
                                float* work, float* fx, float* fy, float* fz,
                                float scale, double r_split, void* stream);
+/* Adjoint of pmx_fft_c2r_longrange_grad (the gradient of
+ * apply_longrange(split = r_split), tfpm.py:183-219, w.r.t. delta_k):
+ * gdk = sum_d conj(lap exp(-k^2 r_split^2) i g_d) FFT3(g_d) * scale, half
+ * spectrum.  work: 3 spec buffers. */
+int pmx_fft_longrange_grad_adj(const pmx_plan* plan, const float* gfx,
+                               const float* gfy, const float* gfz,
+                               float* work, float* gdk, float scale,
+                               double r_split, void* stream);
+
+/* The meshes of apply_pgd (tfpm.py:222-270): out_d = c2r3d(dk *
+ * laplace_kernel * PGD_kernel(kl, ks) * gradient_kernel(d) * scale)
+ * (kernels.py:105-134, PGD = exp(-kl^2/k^2) exp(-k^4/ks^4) in float32).
+ * deriv = 1 / 2 replaces PGD_kernel by its derivative w.r.t. kl / ks (the
+ * parameter gradients).  dk: half spectrum (not modified); work: 3 spec. */
+int pmx_fft_c2r_pgd_grad(const pmx_plan* plan, const float* dk, float* work,
+                         float* fx, float* fy, float* fz, float scale,
+                         float kl, float ks, int deriv, void* stream);
+/* Its adjoint w.r.t. dk (deriv = 0): gdk = sum_d conj(lap PGD i g_d)
+ * FFT3(g_d) * scale.  work: 3 spec. */
+int pmx_fft_pgd_grad_adj(const pmx_plan* plan, const float* gfx,
+                         const float* gfy, const float* gfz, float* work,
+                         float* gdk, float scale, float kl, float ks,
+                         void* stream);
+
+/* One PGD correction of a state, tfpm.pgd (tfpm.py:369-422): X += alpha /
+ * f * readout(c2r(delta_k lap PGD_kernel(kl, ks) i g_d), f X), delta_k =
+ * r2c3d(paint(f X) / nbar).  state (3,B,N,3) in place (P, F untouched); the
+ * plan is the force mesh (nc * f).  work: pmx_nbody_work_floats floats. */
+int pmx_pgd(const pmx_plan* plan, float* state, long long N, float f,
+            float alpha, float kl, float ks, float* work, void* stream);
+/* Its reverse: gstate (3,B,N,3) holds the cotangent of the corrected state
+ * on entry and of the input state on exit (gX += J^T gX; gP, gF unchanged);
+ * X the input positions; gpgd (device float64[3], may be NULL) receives
+ * d loss / d (alpha, kl, ks) (overwritten).  The particles must form the
+ * Lagrangian grid of the mesh / f.  work: pmx_pgd_adj_work_floats. */
+long long pmx_pgd_adj_work_floats(const pmx_plan* plan, long long N);
+int pmx_pgd_adjoint(const pmx_plan* plan, const float* X, long long N,
+                    float f, float alpha, float kl, float ks, float* gstate,
+                    double* gpgd, float* work, void* stream);
+
+/* out[0] = sum_i a[i] b[i] in float64 (device double, overwritten): the
+ * contractions of the parameter gradients (no reference counterpart; TF
+ * forms them inside its autodiff). */
+int pmx_mesh_dot(const float* a, const float* b, long long n, double* out,
+                 void* stream);
+
 /* Fused density -> force meshes: r2c3d(rho * rho_scale) followed by the
  * Poisson x gradient inverse above, with the forward x transform, the
  * multiply and the inverse x transform in one pass (tfpm.py:344-354). */
@@ -148,9 +194,16 @@ int pmx_lpt2_source(const pmx_plan* plan, const float* dk, float* work,
 long long pmx_lpt2_work_floats(const pmx_plan* plan);
 
 /* ---- lpt_init assembly: tfpm.py:148-180 ----
- * state[0] = Q + c[0] dx1 + c[3] dx2, state[1] = c[1] dx1 + c[4] dx2,
- * state[2] = c[2] dx1 + c[5] dx2 (dx2 may be NULL), dx meshes at the
- * particle grid (B, nx, ny, nz) with Q = np.indices order. */
+ * dx1, dx2: the unscaled displacement meshes (the lpt1 transforms of the
+ * linear field and of the 2LPT source) at the particle grid (B, nx, ny, nz),
+ * x / y / z meshes one after the other (dx2 may be NULL); the growth
+ * factors are applied here, with DX1 = c[0] dx1 and DX2 = c[3] dx2:
+ *   state[0] = Q + DX1 + DX2,
+ *   state[1] = c[1] DX1 + c[4] DX2   (= c[1] c[0] dx1 + c[4] c[3] dx2),
+ *   state[2] = c[2] DX1 + c[5] DX2,
+ * Q in np.indices order; coef6 = (D1, a^2 f1 E, a^2 E gf / D1, D2,
+ * a^2 f2 E, a^2 E gf2 / D2) as tfpm.py:168-177 forms them.  A caller must
+ * not pre-multiply dx1 / dx2 by D1 / D2. */
 int pmx_lpt_assemble(float* state, const float* dx1, const float* dx2,
                      const double* coef6, int B, int nx, int ny, int nz,
                      void* stream);
@@ -209,6 +262,19 @@ int pmx_power_spectrum_counts(const pmx_plan* plan, const double* kx,
                               const double* ky, const double* kz,
                               const double* edges, int nedges, double* nsum,
                               void* stream);
+
+/* Adjoint of the binning (the gradient of power_spectrum, pk.py:40-103,
+ * w.r.t. the field): spec (the unnormalised half spectrum that
+ * pmx_power_spectrum_bins read) is multiplied in place by gbin[b][bin] times
+ * the column's weight (W(j) + W(nz - j)) / 2, or W(j) on the kz = 0 and
+ * nz/2 planes; pmx_fft_c2r of the result with scale 2 (twice the
+ * unnormalised inverse) is the field cotangent when gbin[b][i] =
+ * d loss / d psum[b][i]. */
+int pmx_power_spectrum_bins_adj(const pmx_plan* plan, float* spec,
+                                const double* kx, const double* ky,
+                                const double* kz, const double* edges,
+                                int nedges, const double* gbin,
+                                void* stream);
 
 /* The linear-field amplitude of a box with equal sides (tfpm.py:56-62 with
  * |k| = 2 pi sqrt(mx^2 + my^2 + mz^2) / L): amp[mode] = table[mx^2 + my^2 +
@@ -270,6 +336,24 @@ int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
                       int nsteps, long long N, float f, float fac,
                       const double* coef, float* gstate, float* work,
                       void* stream);
+
+/* The checkpointed reverse sweep without recomputing the last segment:
+ * pmx_nbody_run_last_segment runs the last segment of a run of nsteps
+ * steps in segments of `every` (steps [(nseg - 1) every, nsteps), with the
+ * jump-start force when it is the only segment) on state, recording its
+ * positions and force meshes in work (pmx_nbody_adj_work_floats(plan, N,
+ * every) floats) exactly where pmx_nbody_adjoint's recomputation would;
+ * pmx_nbody_adjoint_last_taped is pmx_nbody_adjoint on that same work,
+ * which then recomputes segments 0 .. nseg - 2 only.  (No reference
+ * counterpart: TF keeps every forward intermediate, tfpm.py:425-503.) */
+int pmx_nbody_run_last_segment(const pmx_plan* plan, float* state,
+                               long long N, float f, float fac,
+                               const double* coef, int nsteps, int every,
+                               float* work, void* stream);
+int pmx_nbody_adjoint_last_taped(const pmx_plan* plan, const float* ckpt,
+                                 int every, int nsteps, long long N, float f,
+                                 float fac, const double* coef,
+                                 float* gstate, float* work, void* stream);
 
 /* Several independent groups of simulations of uneven sizes (one plan per
  * size class; the reference batches only equal sizes, tfpm.py:15-63):
