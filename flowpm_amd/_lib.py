@@ -80,6 +80,8 @@ _SIGS = {
                                     _i, _i, _f, _f, _f, _f, _f, _i, _vp]),
     "pmx_nbody_work_floats": (_ll, [_vp]),
     "pmx_nbody_run": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _i, _i, _vp, _vp]),
+    "pmx_nbody_run_noalloc": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _i, _i,
+                                   _vp]),
     "pmx_nbody_run_pgd": (_i, [_vp, _vp, _ll, _f, _f, _dp,
                                _c.POINTER(_f), _i, _i, _i, _vp, _vp]),
     "pmx_nbody_pgd_tape_floats": (_ll, [_vp, _ll, _i]),

@@ -1130,6 +1130,38 @@ struct pmx_plan {
 
 static inline const Plan& P(const pmx_plan* h) { return h->p; }
 
+// ------------------------------------------------------ static scratch --
+// Workspace of pmx_nbody_run_noalloc: a file-scope device array, so that
+// callers that cannot allocate (code under stream capture, serving loops)
+// can run the time loop.  Every device the library runs on holds its own
+// copy (64 MiB) until the process ends; calls on one device share it, so
+// calls that overlap on one device race.  In host code `scratch` names a
+// host-side stand-in: its device address comes from hipGetSymbolAddress,
+// once per device, and only that address is ever passed to a kernel.
+constexpr long long kScratchFloats = 16LL << 20;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's scratch address (looked up on its first use)
+static int scratch_ptr(float** out) {
+  int dev = -1;
+  PMX_HIP_CHECK(hipGetDevice(&dev));
+  PMX_REQUIRE(dev >= 0 && dev < kMaxDevices,
+              "static scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PMX_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PMX_REQUIRE(p != nullptr, "static scratch: no device address");
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
 extern "C" {
 
 int pmx_version(void) { return PMX_VERSION; }
@@ -1752,6 +1784,27 @@ int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
   PMX_REQUIRE(N >= 1, "pmx_nbody_run: need at least one particle");
   PMX_REQUIRE(b0 >= 0 && b1 >= b0 && (coef || b1 == b0),
               "pmx_nbody_run: bad step range");
+  return nbody_run(P(plan), state, N, f, fac, coef, b0, b1, jump, work,
+                   S(stream));
+}
+
+int pmx_nbody_run_noalloc(const pmx_plan* plan, float* state, long long N,
+                          float f, float fac, const double* coef, int b0,
+                          int b1, int jump, void* stream) {
+  PMX_REQUIRE(plan && state, "pmx_nbody_run_noalloc: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_nbody_run_noalloc: need at least one particle");
+  PMX_REQUIRE(b0 >= 0 && b1 >= b0 && (coef || b1 == b0),
+              "pmx_nbody_run_noalloc: bad step range");
+  const long long need = nbody_work_floats(P(plan));
+  if (need > kScratchFloats) {
+    set_error("pmx_nbody_run_noalloc: this plan needs " +
+              std::to_string(need) + " floats of scratch; the static scratch "
+              "holds " + std::to_string(kScratchFloats) +
+              " (use pmx_nbody_run with a workspace)");
+    return kBadArg;
+  }
+  float* work = nullptr;
+  PMX_TRY(scratch_ptr(&work));
   return nbody_run(P(plan), state, N, f, fac, coef, b0, b1, jump, work,
                    S(stream));
 }

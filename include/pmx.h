@@ -313,6 +313,20 @@ int pmx_nbody_run(const pmx_plan* plan, float* state, long long N, float f,
                   float fac, const double* coef, int step_begin, int step_end,
                   int jump_start, float* work, void* stream);
 
+/* pmx_nbody_run without a workspace argument.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP stream/graph capture, tight serving loops).  The workspace is a
+ * static device array of 16 Mi floats (64 MiB) that every device the
+ * library runs on holds until the process ends; plans whose
+ * pmx_nbody_work_floats() exceeds it are rejected (kBadArg, nothing
+ * enqueued).  The call allocates nothing (no hipMalloc / hipFree).  All
+ * calls on one device share the array: calls that overlap on one device
+ * (other streams, other threads) race and must be serialised by the caller. */
+int pmx_nbody_run_noalloc(const pmx_plan* plan, float* state, long long N,
+                          float f, float fac, const double* coef,
+                          int step_begin, int step_end, int jump_start,
+                          void* stream);
+
 /* pmx_nbody_run with the PGD correction (tfpm.py:369-422, :487-488): after
  * the drift of step i, X += alpha_i / f * readout(c2r(delta_k lap
  * PGD_kernel(kl_i, ks_i) grad_d), f X) with delta_k the density of the
