@@ -332,7 +332,6 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
   __shared__ int ncand[3];
   __shared__ int lst[256][8];  // overlapping bricks of one chunk
   __shared__ int wcnt[4];
-  __shared__ short wl[4][256];  // per wave: the entries that meet its rows
   const BrickGrid& g = a.g;
   const MeshDims& d = a.d;
   const int tid = threadIdx.x;
@@ -420,53 +419,7 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
     }
     __syncthreads();
     // 2. every thread adds the bricks, in list order, to its own cells
-    if constexpr (C == 1) {
-      // each wave keeps the entries whose box meets its 4 tile rows (a
-      // wave-uniform test), then takes two bricks per round with the loads
-      // of both in flight together; the sums keep the list order
-      const int wy = 4 * wv;
-      const int wrows = max(0, min(4, Ty - wy));
-      int cnt = 0;
-      if (wrows > 0) {
-        for (int j0 = 0; j0 < nl; j0 += 64) {
-          const int j = j0 + ln;
-          const bool hit =
-              j < nl && box_meets(lst[j][2], lst[j][5], t0y + wy, wrows, d.ny);
-          const unsigned long long mk = __ballot(hit);
-          if (hit) wl[wv][cnt + __popcll(mk & ((1ULL << ln) - 1ULL))] = (short)j;
-          cnt += __popcll(mk);
-        }
-      }
-      __syncthreads();
-      // the 16 loads of entry j for this lane's cells (out-of-box loads read
-      // box element 0 and are discarded)
-      auto gather16 = [&](int j, bool live, float* v, bool* in) {
-        const int e0 = lst[j][4], e1 = lst[j][5], e2 = lst[j][6];
-        const int oy = wrap_off(cy, lst[j][2], d.ny);
-        const int oz = wrap_off(cz, lst[j][3], d.nz);
-        const bool lane = live && yz_ok && oy < e1 && oz < e2;
-        const float* src =
-            a.arena + ((long long)b * g.nbricks + lst[j][0]) * a.slot;
-        const int a0 = lst[j][1];
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-          const int ox = wrap_off(t0x + k, a0, d.nx);
-          in[k] = lane && k < Tx && ox < e0;
-          v[k] = src[in[k] ? (ox * e1 + oy) * e2 + oz : 0];
-        }
-      };
-      for (int q = 0; q < cnt; q += 2) {
-        const bool two = q + 1 < cnt;
-        float vA[PER], vB[PER];
-        bool inA[PER], inB[PER];
-        gather16(wl[wv][q], true, vA, inA);
-        gather16(wl[wv][two ? q + 1 : q], two, vB, inB);
-#pragma unroll
-        for (int k = 0; k < PER; ++k) acc[0][k] += inA[k] ? vA[k] : 0.0f;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) acc[0][k] += inB[k] ? vB[k] : 0.0f;
-      }
-    } else if (yz_ok) {
+    if (yz_ok) {
       for (int j = 0; j < nl; ++j) {
         const int e1 = lst[j][5], e2 = lst[j][6];
         const int oy = wrap_off(cy, lst[j][2], d.ny);
