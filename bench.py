@@ -163,6 +163,19 @@ def main():
       v["hbm_bytes"] = h  # measured (committed counter summary)
       v["hbm_GBps"] = h / (v["avg_us"] * 1e-6) / 1e9
 
+  # the final paint of the run (cic_paint drop-in of one particle per cell:
+  # the brick paint on a workspace)
+  zeros = torch.zeros(1, n, n, n, device=dev)
+  fa.cic_paint(zeros, st[0])
+  torch.cuda.synchronize()
+  e0.record()
+  for _ in range(5):
+    rho_final = fa.cic_paint(zeros, st[0])
+  e1.record()
+  torch.cuda.synchronize()
+  final_paint_ms = e0.elapsed_time(e1) / 5
+  del rho_final, zeros
+
   # forward + adjoint (checkpoint every k, recompute in backward)
   fa_ms = fa8_ms = every_used = None
   adj_kernels = None
@@ -234,6 +247,7 @@ def main():
       "fwd_adjoint_ms_per_step": fa_ms,
       "adjoint_over_forward": (fa_ms / ms_step) if fa_ms else None,
       "fwd_adjoint_recompute8_ms_per_step": fa8_ms,
+      "final_cic_paint_ms": final_paint_ms,
       "roofline": {
           "bound": "hbm",
           "kernel": dom,

@@ -33,6 +33,9 @@ _SIGS = {
     "pmx_plan_spec_elems": (_ll, [_vp]),
     "pmx_plan_set_tiles": (_i, [_vp, _i, _i]),
     "pmx_cic_paint": (_i, [_vp, _vp, _vp, _i, _ll, _i, _i, _i, _f, _vp]),
+    "pmx_cic_paint_work_floats": (_ll, [_i, _ll, _i, _i, _i, _f]),
+    "pmx_cic_paint_ws": (_i, [_vp, _vp, _vp, _i, _ll, _i, _i, _i, _f, _vp,
+                              _ll, _vp]),
     "pmx_cic_readout": (_i, [_vp, _vp, _vp, _i, _ll, _i, _i, _i, _f, _vp]),
     "pmx_cic_paint_adj": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _ll, _i, _i, _i,
                                _f, _vp]),

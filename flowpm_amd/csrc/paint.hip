@@ -38,6 +38,7 @@ struct BrickArgs {
   float* arena;          // box arena
   long long arena_cap;   // floats
   long long slot;        // arena floats per brick
+  int accum;             // gather adds to the mesh instead of overwriting
   int* meta;             // 8 ints per brick: lo[3], ext[3], off lo/hi
   int* ctr;              // [2] overflow count, [3] box radius (written by
                          // k_brick_reduce between the two phases)
@@ -452,14 +453,16 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
     __syncthreads();  // lst is rewritten by the next chunk
   }
   if (!yz_ok) return;
-  // every cell of the mesh is written here (the mesh needs no clearing)
+  // every cell of the mesh is written here (the mesh needs no clearing);
+  // with accum the painted values are added to the mesh's own
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     if (k < Tx) {
       const long long cell =
           (((long long)b * d.nx + t0x + k) * d.ny + cy) * d.nz + cz;
 #pragma unroll
-      for (int c = 0; c < C; ++c) a.mesh[c][cell] = acc[c][k];
+      for (int c = 0; c < C; ++c)
+        a.mesh[c][cell] = a.accum ? a.mesh[c][cell] + acc[c][k] : acc[c][k];
     }
   }
 }
@@ -531,9 +534,10 @@ long long brick_work_floats(const MeshDims& d, const BrickGrid& g, int C) {
 int launch_brick_paint(float* const* mesh, int C, const float* X,
                        const float* w, int wstride, float wscale, float ps,
                        const MeshDims& d, const BrickGrid& g, float* work,
-                       hipStream_t s) {
+                       hipStream_t s, bool accumulate) {
   if ((long long)d.B * d.N == 0) return kOk;
   BrickArgs a{};
+  a.accum = accumulate ? 1 : 0;
   for (int c = 0; c < C; ++c) a.mesh[c] = mesh[c];
   a.slot = (long long)C * kBoxFloats;
   a.arena = work;

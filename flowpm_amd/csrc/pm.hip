@@ -1260,6 +1260,43 @@ int pmx_cic_paint(float* mesh, const float* pos, const float* weight, int B,
   return launch_paint(mesh, pos, weight, 1.0f, d, ps, 0, S(stream));
 }
 
+// Particles that number one per mesh cell are painted with the brick paint
+// (their grid order, when it is the Lagrangian one, keeps every brick's
+// box small; any order is painted correctly, scattered bricks spill to
+// atomics).
+static bool grid_paint(int B, long long N, int nx, int ny, int nz, float ps,
+                       BrickGrid* g) {
+  if (ps != 1.0f || N != (long long)nx * ny * nz) return false;
+  *g = make_brick_grid(nx, ny, nz, 1);
+  const MeshDims d{B, nx, ny, nz, N};
+  return brick_paint_ok(d, *g);
+}
+
+long long pmx_cic_paint_work_floats(int B, long long N, int nx, int ny,
+                                    int nz, float ps) {
+  BrickGrid g;
+  if (B < 1 || nx < 1 || ny < 1 || nz < 1 ||
+      !grid_paint(B, N, nx, ny, nz, ps, &g))
+    return 0;
+  return brick_work_floats(MeshDims{B, nx, ny, nz, N}, g, 1);
+}
+
+int pmx_cic_paint_ws(float* mesh, const float* pos, const float* weight,
+                     int B, long long N, int nx, int ny, int nz, float ps,
+                     float* work, long long work_floats, void* stream) {
+  CHECK_MESH(B, N, nx, ny, nz);
+  PMX_REQUIRE(mesh && (pos || N == 0), "pmx_cic_paint_ws: null pointer");
+  MeshDims d{B, nx, ny, nz, N};
+  BrickGrid g;
+  if (work && grid_paint(B, N, nx, ny, nz, ps, &g) &&
+      work_floats >= brick_work_floats(d, g, 1)) {
+    float* m[1] = {mesh};
+    return launch_brick_paint(m, 1, pos, weight, 1, 1.0f, ps, d, g, work,
+                              S(stream), true);
+  }
+  return launch_paint(mesh, pos, weight, 1.0f, d, ps, 0, S(stream));
+}
+
 int pmx_cic_readout(float* out, const float* mesh, const float* pos, int B,
                     long long N, int nx, int ny, int nz, float ps,
                     void* stream) {

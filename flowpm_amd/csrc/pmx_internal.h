@@ -295,11 +295,12 @@ int launch_power_bins_adj(const Plan& p, float2* spec, const double* kx,
 // Lagrangian-brick paint (paint.hip).
 bool brick_paint_ok(const MeshDims& d, const BrickGrid& g);
 long long brick_work_floats(const MeshDims& d, const BrickGrid& g, int C);
-// mesh[c] = sum_p w[p*wstride+c]*wscale * CIC(X*ps) (every cell written)
+// mesh[c] = sum_p w[p*wstride+c]*wscale * CIC(X*ps) (every cell written;
+// accumulate: mesh[c] += instead)
 int launch_brick_paint(float* const* mesh, int C, const float* X,
                        const float* w, int wstride, float wscale, float ps,
                        const MeshDims& d, const BrickGrid& g, float* work,
-                       hipStream_t s);
+                       hipStream_t s, bool accumulate = false);
 
 // Force VJP pieces.
 // KDK cotangent update applied after a force VJP (see k_force_vjp_finish)

@@ -52,8 +52,13 @@ class _CicPaint(torch.autograd.Function):
     N = pos.shape[1]
     w = None if weight is None else _f32(weight).reshape(B, N)
     out = _f32(mesh).clone()
-    _lib.call("pmx_cic_paint", _lib.ptr(out), _lib.ptr(pos), _lib.ptr(w), B,
-              N, nx, ny, nz, 1.0, _lib.stream_of(out))
+    # one particle per cell (the final paint of an nbody run): the time
+    # loop's atomic-free brick paint, on a workspace; else the scatter
+    nw = _lib.lib().pmx_cic_paint_work_floats(B, N, nx, ny, nz, 1.0)
+    work = torch.empty(nw, dtype=torch.float32, device=out.device) \
+        if nw > 0 else None
+    _lib.call("pmx_cic_paint_ws", _lib.ptr(out), _lib.ptr(pos), _lib.ptr(w),
+              B, N, nx, ny, nz, 1.0, _lib.ptr(work), nw, _lib.stream_of(out))
     ctx.save_for_backward(pos, w)
     ctx.dims = (B, N, nx, ny, nz)
     ctx.part_shape = part.shape

@@ -65,6 +65,18 @@ int pmx_plan_set_tiles(pmx_plan* plan, int tile_y, int tile_x);
 int pmx_cic_paint(float* mesh, const float* pos, const float* weight, int B,
                   long long N, int nx, int ny, int nz, float pos_scale,
                   void* stream);
+/* pmx_cic_paint with a workspace: particles that number one per mesh cell
+ * (N = nx*ny*nz, pos_scale 1: the final paint of an nbody run) go through
+ * the brick paint of the time loop, which writes the mesh without atomics
+ * (mesh += paint, as pmx_cic_paint); others through pmx_cic_paint's
+ * scatter.  work: pmx_cic_paint_work_floats() floats (0: not applicable;
+ * work may then be NULL). */
+long long pmx_cic_paint_work_floats(int B, long long N, int nx, int ny,
+                                    int nz, float pos_scale);
+int pmx_cic_paint_ws(float* mesh, const float* pos, const float* weight,
+                     int B, long long N, int nx, int ny, int nz,
+                     float pos_scale, float* work, long long work_floats,
+                     void* stream);
 /* out[b][p] = sum_corners mesh * w(pos * pos_scale) */
 int pmx_cic_readout(float* out, const float* mesh, const float* pos, int B,
                     long long N, int nx, int ny, int nz, float pos_scale,

@@ -642,3 +642,28 @@ def test_linear_amplitude_table_matches_per_mode(gpu, nc, box):
   assert amp.shape == ref.shape
   err = np.abs(amp - ref32) / np.abs(ref32).max()
   assert err.max() < 1e-6, err.max()
+
+
+@pytest.mark.parametrize("n,B,permute", [(32, 1, False), (32, 2, True),
+                                         (64, 1, False)])
+def test_cic_paint_one_particle_per_cell_against_oracle(gpu, n, B, permute):
+  """cic_paint of one particle per cell goes through the brick paint
+  (mesh += paint, weights, wraps); a shuffled particle order is painted
+  correctly too (its bricks are scattered and spill to atomics).  The brick
+  paint's fixed point rounds each deposit by at most 2^-30 of its brick's
+  sum of |weights| (here <= 2048, i.e. <= 1.9e-6 per deposit, about eight
+  deposits per cell), hence the 1e-5 bar."""
+  st = _displaced_state(n, B, amp=1.2, seed=21)
+  X = st[0]
+  if permute:
+    X = X[:, torch.randperm(n**3, generator=torch.Generator().manual_seed(3))]
+  gen = torch.Generator().manual_seed(22)
+  w = torch.rand(B, n**3, generator=gen, dtype=torch.float64)
+  mesh0 = torch.randn(B, n, n, n, generator=gen, dtype=torch.float64)
+  ref = opm.cic_paint(mesh0, X, w)
+  out = fa.cic_paint(mesh0.float().to(gpu), X.float().to(gpu),
+                     w.float().to(gpu))
+  close(out, ref, 1e-5)
+  ref1 = opm.cic_paint(torch.zeros_like(mesh0), X)
+  out1 = fa.cic_paint(torch.zeros(B, n, n, n, device=gpu), X.float().to(gpu))
+  close(out1, ref1, 1e-5)
