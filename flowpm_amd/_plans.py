@@ -9,8 +9,11 @@ from . import _lib
 _PLANS = {}
 _LOCK = threading.Lock()
 
-SUPPORTED_SIDES = (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512,
-                   768, 1024)
+# 2^a, 3 * 2^a, 5 * 2^a, 15 * 2^a and 25 * 2^a from 8 to 1024, multiples of
+# 4 (the radix 16/8/4/2/3/5 line transforms; the z stencil needs nz % 4 == 0)
+SUPPORTED_SIDES = (8, 12, 16, 20, 24, 32, 40, 48, 60, 64, 80, 96, 100, 120,
+                   128, 160, 192, 200, 240, 256, 320, 384, 400, 480, 512, 640,
+                   768, 800, 960, 1024)
 
 # tile widths chosen by flowpm_amd.autotune, keyed like the plans
 TILE_CHOICES = {}

@@ -7,7 +7,7 @@
 // field is Hermitian; every multiplied kernel preserves that symmetry).
 //
 // Structure: three line passes, each a workgroup-per-tile kernel that stages
-// a tile in LDS, runs a Stockham autosort FFT there (radix 8/4/2/3 stages,
+// a tile in LDS, runs a Stockham autosort FFT there (radix 16/8/4/2/3/5 stages,
 // register-staged, in place), and writes the tile back.
 //   Z pass  contiguous rows; two real rows are packed into one complex line.
 //   Y pass  column tiles: all ky for 16 consecutive kz (128-byte segments).
@@ -146,11 +146,41 @@ struct Dft<3, D> {
   }
 };
 
+template <int D>
+struct Dft<5, D> {
+  // X_m = sum_j a_j W^(jm), W = exp(D 2 pi i / 5).  With p_j = a_j + a_(5-j),
+  // m_j = a_j - a_(5-j) (j = 1, 2), c_j = cos(2 pi j / 5) and
+  // s_j = sin(2 pi j / 5):
+  //   X_1, X_4 = a_0 + c1 p_1 + c2 p_2 +- D i (s1 m_1 + s2 m_2)
+  //   X_2, X_3 = a_0 + c2 p_1 + c1 p_2 +- D i (s2 m_1 - s1 m_2)
+  static __device__ __forceinline__ void run(float2* a) {
+    constexpr float c1 = 0.30901699437494742410f;   // cos(2 pi / 5)
+    constexpr float c2 = -0.80901699437494742410f;  // cos(4 pi / 5)
+    constexpr float s1 = 0.95105651629515357212f;   // sin(2 pi / 5)
+    constexpr float s2 = 0.58778525229247312917f;   // sin(4 pi / 5)
+    const float2 p1 = cadd(a[1], a[4]), m1 = csub(a[1], a[4]);
+    const float2 p2 = cadd(a[2], a[3]), m2 = csub(a[2], a[3]);
+    const float2 r1 = make_float2(a[0].x + c1 * p1.x + c2 * p2.x,
+                                  a[0].y + c1 * p1.y + c2 * p2.y);
+    const float2 r2 = make_float2(a[0].x + c2 * p1.x + c1 * p2.x,
+                                  a[0].y + c2 * p1.y + c1 * p2.y);
+    const float2 q1 = mul_i<D>(make_float2(s1 * m1.x + s2 * m2.x,
+                                           s1 * m1.y + s2 * m2.y));
+    const float2 q2 = mul_i<D>(make_float2(s2 * m1.x - s1 * m2.x,
+                                           s2 * m1.y - s1 * m2.y));
+    a[0] = cadd(a[0], cadd(p1, p2));
+    a[1] = cadd(r1, q1);
+    a[4] = csub(r1, q1);
+    a[2] = cadd(r2, q2);
+    a[3] = csub(r2, q2);
+  }
+};
+
 // radix plan: 16 first (fewer LDS round trips: 256 = 16 x 16 is two
-// stages instead of three), then 8, 4, 2, 3
+// stages instead of three), then 8, 4, 2, 3, 5
 __host__ __device__ constexpr int pick_radix(int n) {
   return (n % 16 == 0) ? 16 : (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4
-       : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : n;
+       : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : (n % 5 == 0) ? 5 : n;
 }
 
 // LDS tile: element (line index i, column c) at s[i * P + c], P = TW + 1.
@@ -208,7 +238,8 @@ struct FftStages {
   static __device__ __forceinline__ void run(float2* s, const float2* tw) {
     if constexpr (NS < N) {
       constexpr int R = pick_radix(N / NS);
-      static_assert(R == 2 || R == 3 || R == 4 || R == 8 || R == 16,
+      static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 ||
+                        R == 16,
                     "unsupported radix");
       fft_stage<N, R, NS, TW, T, D>(s, tw);
       FftStages<N, NS * R, TW, T, D>::run(s, tw);
@@ -754,9 +785,12 @@ __global__ void k_full_to_spec(const float2* __restrict__ full,
 }
 
 // ------------------------------------------------------------ dispatch ---
-#define PMX_FFT_SIZES(X) \
-  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) \
-  X(384) X(512) X(768) X(1024)
+// 2^a, 3 * 2^a, 5 * 2^a, 15 * 2^a and 25 * 2^a up to 1024 (multiples of 4
+// from 8, as the time loop's z stencil needs)
+#define PMX_FFT_SIZES(X)                                                    \
+  X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(60) X(64) X(80) X(96)    \
+  X(100) X(120) X(128) X(160) X(192) X(200) X(240) X(256) X(320) X(384)     \
+  X(400) X(480) X(512) X(640) X(768) X(800) X(960) X(1024)
 
 bool fft_size_supported(int n) {
   switch (n) {
@@ -849,10 +883,11 @@ static int x_nt(const Plan& pl, const XJob& job, hipStream_t s) {
   return kOk;
 }
 // Column-tile width: 16 float2 (128-byte rows) or 8 (more workgroups per CU,
-// 64-byte rows); lines of 768 and more only fit LDS with 8.
+// 64-byte rows); lines longer than 512 only fit LDS with 8 (two tiles of
+// N x 17 float2 in the fan-out passes).
 template <int N>
 static int y_n(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
-  if constexpr (N >= 768) {
+  if constexpr (N > 512) {
     return y_nt<N, 8>(pl, jobs, inv, s);
   } else {
     return pl.tw_y == 8 ? y_nt<N, 8>(pl, jobs, inv, s)
@@ -861,7 +896,7 @@ static int y_n(const Plan& pl, const YJobs& jobs, int inv, hipStream_t s) {
 }
 template <int N>
 static int x_n(const Plan& pl, const XJob& job, hipStream_t s) {
-  if constexpr (N >= 768) {
+  if constexpr (N > 512) {
     return x_nt<N, 8>(pl, job, s);
   } else {
     return pl.tw_x == 8 ? x_nt<N, 8>(pl, job, s) : x_nt<N, 16>(pl, job, s);

@@ -38,8 +38,15 @@ def cosine(a, b):
 @pytest.mark.parametrize("shape", [(1, 16, 16, 16), (2, 32, 32, 32),
                                    (1, 16, 16, 48), (1, 96, 96, 96),
                                    (1, 24, 48, 12), (1, 128, 64, 256),
-                                   (1, 256, 256, 8), (2, 512, 16, 8)])
+                                   (1, 256, 256, 8), (2, 512, 16, 8),
+                                   (1, 80, 100, 60), (1, 20, 40, 160),
+                                   (2, 320, 120, 40), (1, 960, 8, 200),
+                                   (1, 800, 24, 240), (1, 400, 480, 20),
+                                   (1, 640, 12, 100)])
 def test_r2c_c2r_against_numpy(gpu, shape):
+  """r2c3d / c2r3d against numpy.fft on every radix (16, 8, 4, 2, 3, 5:
+  100 = 4 * 5 * 5, 960 = 16 * 4 * 3 * 5, 800 = 16 * 2 * 5 * 5) and both
+  column-tile widths (lines above 512 use 8-column tiles)."""
   rng = np.random.default_rng(sum(shape))
   x = rng.standard_normal(shape)
   M = np.prod(shape[1:])
@@ -257,12 +264,16 @@ def test_nbody_64_batch2_against_oracle(gpu):
 
 
 @pytest.mark.parametrize("nc,box", [([16, 16, 48], [50.0, 50.0, 150.0]),
-                                    ([32, 64, 128], [100.0, 200.0, 400.0])])
+                                    ([32, 64, 128], [100.0, 200.0, 400.0]),
+                                    ([20, 40, 60], [62.5, 125.0, 187.5]),
+                                    ([80, 100, 40], [160.0, 200.0, 80.0])])
 def test_nbody_rectangular_against_oracle(gpu, nc, box):
   """End-to-end nbody on rectangular meshes (tests/test_tfpm.py:186-212 of
   the reference: [nc, nc, 3 nc], box [bs, bs, 3 bs], 1LPT at 0.1, stages
   linspace(0.1, 1, 10)); the second shape runs the tiled particle passes
-  (32-row slabs) on a non-cubic grid."""
+  (32-row slabs) on a non-cubic grid, the last two sides with radix-5 line
+  transforms (20, 40, 60, 80, 100) and bricks cut short at the grid's
+  edge."""
   gen = torch.Generator().manual_seed(100)
   white = torch.randn([1] + nc, generator=gen, dtype=torch.float64) * (
       nc[0] * nc[1] * nc[2])**0.5
@@ -585,6 +596,7 @@ def test_gradient_64_20steps_against_oracle(gpu):
 
 @pytest.mark.parametrize("shape,tol", [((2, 32, 16, 24), 2e-6),
                                        ((1, 64, 64, 64), 2e-6),
+                                       ((1, 40, 100, 80), 2e-6),
                                        ((1, 8, 512, 256), 2e-5)])
 def test_force_mesh_f4_matches_fourier_gradient(gpu, shape, tol):
   """The time loop's force mesh (one potential, real-space fourth-order
