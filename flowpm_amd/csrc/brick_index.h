@@ -22,7 +22,7 @@ struct BrickGrid {
   int f;              // mesh cells per particle spacing (pm_nc_factor)
 };
 
-constexpr int kBoxFloats = 16384;  // LDS budget of one brick box (64 KB)
+constexpr int kBoxFloats = 16384;  // largest brick box: its arena slot (64 KB)
 constexpr int kTile = 16;          // gather tile side
 
 PMX_HD int imin(int a, int b) { return a < b ? a : b; }

@@ -6,7 +6,8 @@
 // phase 1 (k_brick_deposit): one workgroup per Lagrangian brick of
 //   particles (e.g. 8x16x16).  Particles that start close stay close, so
 //   the brick's deposits fit a small box.  The workgroup finds the box,
-//   accumulates the 8 deposits per particle in LDS as 32-bit fixed point
+//   accumulates the 8 deposits per particle in LDS (in x slabs when the box
+//   exceeds the 39.5 KB window) as 32-bit fixed point
 //   (integer LDS atomics run several times faster than float ones on
 //   gfx950, and integer sums do not depend on their order), and stores the
 //   dense box, converted back to float, to its own arena slot; the C
@@ -15,7 +16,7 @@
 //   the sum S of the channel's |weights| so no cell can overflow: the scale
 //   is 2^(30 - ex) with S = m 2^ex, m in [0.5, 1), so one deposit is rounded
 //   by at most half a unit, 2^(ex - 31) <= 2^-30 S.
-//   A brick whose box does not fit the LDS budget falls back to
+//   A brick whose box exceeds its 64 KB arena slot falls back to
 //   device-scope float atomics into the finished mesh (k_brick_spill, after
 //   phase 2).
 // phase 2 (k_brick_gather): one workgroup per 16^3 mesh tile evaluates the
@@ -70,10 +71,14 @@ __device__ __forceinline__ bool ok_pos(float x, float y, float z) {
   return fabsf(x) < 1.0e9f && fabsf(y) < 1.0e9f && fabsf(z) < 1.0e9f;
 }
 
-// deposit workgroup: 512 threads (8 waves: with two 64 KB boxes per CU that
-// is 16 waves to hide the loads and the LDS atomics), 4 particles per
-// thread; bricks hold <= 2048 particles
+// deposit workgroup: 512 threads (8 waves; four workgroups per CU give 32
+// waves to hide the loads and the LDS atomics), 4 particles per thread;
+// bricks hold <= 2048 particles.  A box (<= kBoxFloats, its arena slot) is
+// accumulated through an LDS window of kWinFloats ints: four windows and
+// the workgroups' other LDS fit a CU's 160 KB; a larger box is deposited
+// in slabs of whole x planes, one pass over the particles per slab.
 constexpr int kDT = 512;
+constexpr int kWinFloats = 10112;
 constexpr int kDW = kDT / 64;
 constexpr int kPPT = 2048 / kDT;
 
@@ -118,8 +123,10 @@ __device__ __forceinline__ void load_brick(const BrickArgs& a, int bid, int b,
 }
 
 template <int C>
-__global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
-  __shared__ int box[kBoxFloats];  // fixed-point accumulators
+// (launch bounds: at least 8 waves per SIMD, i.e. at most 64 VGPRs, so that
+// four 512-thread workgroups share a CU; their LDS windows fit its 160 KB)
+__global__ __launch_bounds__(kDT, 8) void k_brick_deposit(BrickArgs a) {
+  __shared__ int box[kWinFloats];  // fixed-point accumulators
   __shared__ int red[kDW][6];
   __shared__ float redw[kDW][C];
   __shared__ int sh[10];
@@ -252,17 +259,29 @@ __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
     __syncthreads();
     const int mode = sh[0];
     const int lo0 = sh[1], lo1 = sh[2], lo2 = sh[3];
-    const int e1 = sh[5], e2 = sh[6], vol = sh[7];
+    const int e0 = sh[4], e1 = sh[5], e2 = sh[6], vol = sh[7];
     const long long off = (long long)sh[8] | ((long long)sh[9] << 31);
     // mode 0: no particles; mode 2: k_brick_spill adds it after the gather
     if (mode != 1) return;
     constexpr int OX[8] = {0, 1, 0, 0, 1, 1, 0, 1};
     constexpr int OY[8] = {0, 0, 1, 0, 1, 0, 1, 1};
     constexpr int OZ[8] = {0, 0, 0, 1, 0, 1, 1, 1};
-    // clears the box and deposits channel c of the brick's particles
-    auto deposit = [&](int c, float fxs) {
-      for (int i = tid; i < vol; i += kDT) box[i] = 0;
+    // x planes of the box per LDS window: boxes larger than the window are
+    // deposited in several passes over x slabs
+    const int plane = e1 * e2;
+    const int hx = kWinFloats / plane;  // >= 1: plane <= vol / 2 <= 8192
+    const bool clip = hx < e0;
+    // clears the window and deposits channel c of the brick's particles
+    // whose corners fall in x planes [x0, x0 + nxw) of the box
+    auto deposit = [&](int c, float fxs, int x0, int nxw) {
+      const int wv = nxw * plane;
+      for (int i = tid; i < wv; i += kDT) box[i] = 0;
       __syncthreads();
+      // the positions are opaque here, so the corner weights are computed
+      // per pass instead of being kept live across the passes and channels
+#pragma unroll
+      for (int u = 0; u < kPPT; ++u)
+        asm volatile("" : "+v"(px[u]), "+v"(py[u]), "+v"(pz[u]));
 #pragma unroll
       for (int u = 0; u < kPPT; ++u) {
         if (!ok[u]) continue;
@@ -271,25 +290,42 @@ __global__ __launch_bounds__(kDT) void k_brick_deposit(BrickArgs a) {
         const float w0x = 1.0f - fabsf(x - flx), w1x = 1.0f - fabsf(x - (flx + 1.0f));
         const float w0y = 1.0f - fabsf(y - fly), w1y = 1.0f - fabsf(y - (fly + 1.0f));
         const float w0z = 1.0f - fabsf(z - flz), w1z = 1.0f - fabsf(z - (flz + 1.0f));
-        const int bx = (int)flx - lo0, by = (int)fly - lo1, bz = (int)flz - lo2;
+        const int bx = (int)flx - lo0 - x0, by = (int)fly - lo1, bz = (int)flz - lo2;
         const float wq = wc[u][c] * fxs;
+        if (!clip) {  // the whole box in one window: no x test
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
-                           (OZ[k] ? w1z : w0z);
-          const int li = ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
-          atomicAdd(&box[li], __float2int_rn(wq * kw));
+          for (int k = 0; k < 8; ++k) {
+            const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
+                             (OZ[k] ? w1z : w0z);
+            const int li =
+                ((bx + OX[k]) * e1 + (by + OY[k])) * e2 + bz + OZ[k];
+            atomicAdd(&box[li], __float2int_rn(wq * kw));
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const int xx = bx + OX[k];
+            if ((unsigned)xx >= (unsigned)nxw) continue;
+            const float kw = ((OX[k] ? w1x : w0x) * (OY[k] ? w1y : w0y)) *
+                             (OZ[k] ? w1z : w0z);
+            const int li = (xx * e1 + (by + OY[k])) * e2 + bz + OZ[k];
+            atomicAdd(&box[li], __float2int_rn(wq * kw));
+          }
         }
       }
       __syncthreads();
     };
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      deposit(c, shs[c][0]);
       const float fxinv = shs[c][1];
-      float* dst = a.arena + off + (long long)c * vol;
-      for (int i = tid; i < vol; i += kDT) dst[i] = (float)box[i] * fxinv;
-      if (c + 1 < C) __syncthreads();  // the box is cleared for the next one
+      for (int x0 = 0; x0 < e0; x0 += hx) {
+        const int nxw = min(hx, e0 - x0);
+        deposit(c, shs[c][0], x0, nxw);
+        float* dst = a.arena + off + (long long)c * vol + (long long)x0 * plane;
+        const int wv = nxw * plane;
+        for (int i = tid; i < wv; i += kDT) dst[i] = (float)box[i] * fxinv;
+        if (c + 1 < C || x0 + hx < e0) __syncthreads();  // window reused
+      }
     }
   }
 }
@@ -467,7 +503,7 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
   }
 }
 
-// Bricks whose box did not fit (box over the LDS budget, positions spread
+// Bricks whose box did not fit (box over its 64 KB arena slot, positions spread
 // over more than the mesh, non-finite weights) are added to the gathered
 // mesh with device-scope float atomics, one thread per particle.  Without
 // such bricks (k_brick_reduce counted none) every workgroup returns after
