@@ -16,6 +16,9 @@
 
 using namespace pmx;
 
+// accumulation window of the emulated deposit, in floats
+constexpr int kEmuWindow = 3200;
+
 struct Case {
   const char* name;
   int gx, gy, gz, f;
@@ -86,7 +89,7 @@ static int run(const Case& cs, int C) {
     std::vector<double> v[3];
   };
   std::vector<Box> boxes(g.nbricks);
-  int Rmax = 0, nspill = 0, nwrap = 0;
+  int Rmax = 0, nspill = 0, nwrap = 0, nslab = 0;  // nslab: boxes in slabs
   for (int bid = 0; bid < g.nbricks; ++bid) {
     int q0[3], cnt[3];
     brick_block(g, bid, q0, cnt);
@@ -125,17 +128,30 @@ static int run(const Case& cs, int C) {
           ++nwrap;
           break;
         }
-      // deposit boxes, one per channel
+      // deposit boxes, one per channel, through an accumulation window of
+      // whole x planes as k_brick_deposit does (a window much smaller than
+      // the kernel's, so that most boxes here take several slabs)
       bool bad = false;
+      const int plane = bx.ext[1] * bx.ext[2];
+      const int hx = imax(1, kEmuWindow / plane);
       for (int c = 0; c < C; ++c) {
         B.v[c].assign(bx.vol, 0.0);
-        for_particles([&](long long p) {
-          deposit(p, c, [&](int x, int y, int z, double v) {
-            const int o[3] = {x - lo[0], y - lo[1], z - lo[2]};
-            for (int d = 0; d < 3; ++d) bad = bad || o[d] < 0 || o[d] >= bx.ext[d];
-            if (!bad) B.v[c][((long long)o[0] * bx.ext[1] + o[1]) * bx.ext[2] + o[2]] += v;
+        for (int x0 = 0; x0 < bx.ext[0]; x0 += hx) {
+          const int nxw = imin(hx, bx.ext[0] - x0);
+          std::vector<double> win((size_t)nxw * plane, 0.0);
+          for_particles([&](long long p) {
+            deposit(p, c, [&](int x, int y, int z, double v) {
+              const int o[3] = {x - lo[0], y - lo[1], z - lo[2]};
+              for (int d = 0; d < 3; ++d) bad = bad || o[d] < 0 || o[d] >= bx.ext[d];
+              const int xx = o[0] - x0;
+              if (bad || xx < 0 || xx >= nxw) return;  // another slab's corner
+              win[((size_t)xx * bx.ext[1] + o[1]) * bx.ext[2] + o[2]] += v;
+            });
           });
-        });
+          for (size_t i = 0; i < win.size(); ++i)
+            B.v[c][(size_t)x0 * plane + i] = win[i];
+        }
+        if (c == 0 && hx < bx.ext[0]) ++nslab;  // a box of several slabs
       }
       if (bad) {
         printf("%s: brick %d deposit outside its box\n", cs.name, bid);
@@ -235,9 +251,9 @@ static int run(const Case& cs, int C) {
       mx = fmax(mx, fabs(direct[c][q]));
       tot += mesh[c][q];
     }
-  printf("%-24s C=%d mesh %dx%dx%d f=%d bricks %d (wrapping %d, spilled %d) R %d "
-         "max err %.2e sum %.6f\n", cs.name, C, n[0], n[1], n[2], cs.f,
-         g.nbricks, nwrap, nspill, Rmax, err, tot);
+  printf("%-24s C=%d mesh %dx%dx%d f=%d bricks %d (wrapping %d, spilled %d, "
+         "in slabs %d) R %d max err %.2e sum %.6f\n", cs.name, C, n[0], n[1], n[2],
+         cs.f, g.nbricks, nwrap, nspill, nslab, Rmax, err, tot);
   return err <= 1e-9 * fmax(1.0, mx) ? 0 : 1;
 }
 
