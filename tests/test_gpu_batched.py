@@ -31,7 +31,7 @@ def _rel(a, b):
 def test_batched_uneven_graph_replays(gpu):
   cosmo = fa.cosmology.Planck15()
   stages = np.linspace(0.1, 1.0, 6)
-  spec = [(16, 2), (32, 1), (24, 3)]
+  spec = [(16, 2), (32, 1), (24, 3), (20, 2), (40, 1)]  # radix-5 sides too
   groups = []
   ref = []
   for i, (n, B) in enumerate(spec):

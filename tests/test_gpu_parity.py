@@ -297,11 +297,12 @@ def test_nbody_rectangular_against_oracle(gpu, nc, box):
   assert ex < 1e-4 and ep < 1e-4 and er < 1e-4
 
 
-def test_nbody_pm_nc_factor2_against_oracle(gpu):
+@pytest.mark.parametrize("n,L", [(32, 100.0), (20, 62.5)])
+def test_nbody_pm_nc_factor2_against_oracle(gpu, n, L):
   """End-to-end nbody with the force mesh twice the particle grid
   (tests/test_tfpm.py:158-183 of the reference: pm_nc_factor = 2): 32^3
-  particles on a 64^3 force mesh."""
-  n, L = 32, 100.0
+  particles on a 64^3 force mesh, and 20^3 on a 40^3 mesh (radix-5 line
+  transforms, bricks of 4 particles along x)."""
   white, pk = _ics(n, L, 100)
   g = obg.GrowthTable(obg.planck15())
   stages = np.linspace(0.1, 1.0, 10)
@@ -657,7 +658,7 @@ def test_linear_amplitude_table_matches_per_mode(gpu, nc, box):
 
 
 @pytest.mark.parametrize("n,B,permute", [(32, 1, False), (32, 2, True),
-                                         (64, 1, False)])
+                                         (64, 1, False), (40, 2, False)])
 def test_cic_paint_one_particle_per_cell_against_oracle(gpu, n, B, permute):
   """cic_paint of one particle per cell goes through the brick paint
   (mesh += paint, weights, wraps); a shuffled particle order is painted
