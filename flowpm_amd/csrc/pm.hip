@@ -1313,6 +1313,7 @@ int pmx_cic_paint_adj(float* dpos, float* dweight, const float* gmesh,
                       void* stream) {
   CHECK_MESH(B, N, nx, ny, nz);
   PMX_REQUIRE(gmesh && (pos || N == 0), "pmx_cic_paint_adj: null pointer");
+  if (!dpos && !dweight) return kOk;  // no cotangent asked for
   MeshDims d{B, nx, ny, nz, N};
   return launch_paint_adj(dpos, dweight, gmesh, pos, weight, d, ps, 1.0f, 0,
                           S(stream));
@@ -1326,6 +1327,7 @@ int pmx_cic_readout_adj(float* gmesh, float* dpos, const float* gout,
   PMX_REQUIRE(gout && (pos || N == 0), "pmx_cic_readout_adj: null pointer");
   PMX_REQUIRE(dpos == nullptr || mesh != nullptr,
               "pmx_cic_readout_adj: position gradient needs the mesh");
+  if (!gmesh && !dpos) return kOk;  // no cotangent asked for
   MeshDims d{B, nx, ny, nz, N};
   return launch_readout_adj(gmesh, dpos, gout, mesh, pos, d, ps, 0,
                             S(stream));

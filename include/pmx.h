@@ -88,7 +88,7 @@ int pmx_cic_paint_adj(float* dpos, float* dweight, const float* gmesh,
                       long long N, int nx, int ny, int nz, float pos_scale,
                       void* stream);
 /* VJP of cic_readout: gmesh += paint(pos, gout) and dpos (either may be
- * NULL). */
+ * NULL; with both NULL the call does nothing). */
 int pmx_cic_readout_adj(float* gmesh, float* dpos, const float* gout,
                         const float* mesh, const float* pos, int B,
                         long long N, int nx, int ny, int nz, float pos_scale,
