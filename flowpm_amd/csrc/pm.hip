@@ -829,23 +829,51 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
 // pass (recomputed when null).  With pgd the same for the PGD displacement
 // S = (alpha / f) readout(PGD meshes): fac is alpha, gin (the cotangent of
 // S) is gX itself, and gF is only the target of the update u.
+// gF = scale * gP: the force cotangent an earlier update left unstored
+__global__ void k_scale_copy(float* __restrict__ dst,
+                             const float* __restrict__ src, float scale,
+                             long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = scale * src[i];
+}
+
+static int store_implicit_gf(float* gF, const float* gP, float* gf_scale,
+                             long long BN3, hipStream_t s) {
+  if (!gf_scale || *gf_scale == 0.0f) return kOk;
+  ProfScope prof_(kProfAdjUpdate, s);
+  hipLaunchKernelGGL(k_scale_copy, dim3(gridfor(BN3)), dim3(256), 0, s, gF,
+                     gP, *gf_scale, BN3);
+  PMX_LAUNCH_CHECK();
+  *gf_scale = 0.0f;
+  return kOk;
+}
+
+// gf_scale (optional, in / out): 0 when gF holds the force cotangent, else
+// the factor s with force cotangent s * gP (gF not stored).  On return it
+// says the same of the gF this update leaves: the grid path of the plain
+// (no PGD) force leaves it unstored.
 static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
                      float* gP, long long N, float f, float fac,
                      const NbodyWork& nw, float* gm, float* fm,
                      const AdjUpdate& u, hipStream_t s,
                      const PgdTerm* pgd = nullptr,
-                     const float* gin = nullptr) {
-  const float* gc = gin ? gin : gF;
+                     const float* gin = nullptr,
+                     float* gf_scale = nullptr) {
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
+  if (gin || pgd) PMX_TRY(store_implicit_gf(gF, gP, gf_scale, BN3, s));
+  const float gin_scale = gf_scale ? *gf_scale : 0.0f;
+  // the readout's cotangent: gin, gF, or gin_scale * gP
+  const float* gc = gin ? gin : (gin_scale != 0.0f ? gP : gF);
+  const float wsc = gin_scale != 0.0f ? gin_scale * (fac / f) : fac / f;
   MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
   float* f4 = fm ? fm : nw.f4;
   const Mesh3 m = m3(f4);
   if (!fm) PMX_TRY(force_at(p, X, N, f, nw, s, pgd));
   float* gms[3] = {gm, gm + M, gm + 2 * M};
-  const int rc3 =
-      paint_meshes(p, gms, 3, X, gc, 3, fac / f, N, f, nw.brick, s);
+  const int rc3 = paint_meshes(p, gms, 3, X, gc, 3, wsc, N, f, nw.brick, s);
   if (rc3 != kOk && rc3 != kUnsupported) return rc3;
   const bool grid = rc3 == kOk;
   const double inv_nbar = (double)Mb / (double)N;
@@ -857,10 +885,19 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
     // nw.rho is free here: it takes the potential's cotangent
     PMX_TRY(force_meshes_adj(p, gm, gm + M, gm + 2 * M, nw.w, nullptr, sc,
                              s, f4, pgd, nw.rho));
-    return launch_force_vjp_finish(gX, gP, gF, gc, X, m, nullptr, md, f,
-                                   fac / f, u, s);
+    AdjUpdate uu = u;
+    uu.gin_scale = gin ? 0.0f : gin_scale;
+    const bool upd = u.mode == 1 || u.mode == 2;
+    uu.implicit_gf = (gf_scale && !pgd && !gin && upd) ? 1 : 0;
+    PMX_TRY(launch_force_vjp_finish(gX, gP, gF, gc, X, m, nullptr, md, f,
+                                    fac / f, uu, s));
+    if (gf_scale)
+      *gf_scale = uu.implicit_gf ? (u.mode == 2 ? u.k1 + u.k2prev : u.k1)
+                                 : 0.0f;
+    return kOk;
   }
   PMX_REQUIRE(!pgd, "PGD adjoint: particles must form the Lagrangian grid");
+  PMX_TRY(store_implicit_gf(gF, gP, gf_scale, BN3, s));
   for (int c = 0; c < 3; ++c) PMX_TRY(launch_fill(gms[c], M, 0.0f, s));
   PMX_TRY(launch_force_vjp_particles(gm, gm + M, gm + 2 * M, gX, gF, X, m,
                                      md, f, fac / f, 1, s));
@@ -967,10 +1004,12 @@ static int pgd_param_grads(const Plan& p, const float* Xpre, const float* G,
 
 // pgd / gpgd (optional): the PGD parameters of every step and where their
 // gradients are accumulated (3 float64 per step).
+// gsc: the force cotangent's state across calls (see force_vjp's gf_scale)
 static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
                          long long N, float f, float fac, const double* coef,
                          float* gstate, const NbodyWork& nw, float* gm,
-                         hipStream_t s, const float* pgd = nullptr,
+                         hipStream_t s, float* gsc,
+                         const float* pgd = nullptr,
                          double* gpgd = nullptr) {
   const long long M = p.d.rows * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
@@ -987,9 +1026,10 @@ static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
     const long long slot = i - tp.first;
     if (!pgd) {
       PMX_TRY(force_vjp(p, tp.xs + slot * BN3, gF, gX, gP, N, f, fac, nw, gm,
-                        tp.fs + slot * 4 * M, u, s));
+                        tp.fs + slot * 4 * M, u, s, nullptr, nullptr, gsc));
       continue;
     }
+    PMX_TRY(store_implicit_gf(gF, gP, gsc, BN3, s));
     // force at the corrected positions (no update yet), then the PGD
     // displacement, whose cotangent is the position cotangent itself, then
     // the drift / kick update
@@ -1053,6 +1093,7 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
   float* gm = work + nbody_work_floats(p);
   float* st = gm + 3 * M;            // scratch state (3, B, N, 3)
   const int nseg = (nsteps + every - 1) / every;
+  float gsc = 0.0f;  // 0: gF stored; else the force cotangent is gsc * gP
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int b0 = sg * every, b1 = std::min(nsteps, b0 + every);
@@ -1068,14 +1109,16 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
       PMX_TRY(nbody_run(p, st, N, f, fac, coef, b0, b1, sg == 0, work, s,
                         &tp));
     }
-    PMX_TRY(reverse_steps(p, tp, b0, b1, N, f, fac, coef, gstate, nw, gm, s));
+    PMX_TRY(reverse_steps(p, tp, b0, b1, N, f, fac, coef, gstate, nw, gm, s,
+                          &gsc));
   }
   float* fjump = segment_tape(p, N, every, work, 0).fj;
   // jump-start force at the initial positions; the initial F is discarded
   AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
   particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
   return force_vjp(p, ckpt, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
-                   fac, nw, gm, nsteps > 0 ? fjump : nullptr, u0, s);
+                   fac, nw, gm, nsteps > 0 ? fjump : nullptr, u0, s, nullptr,
+                   nullptr, &gsc);
 }
 
 // Reverse sweep over a tape recorded by the forward run (no recomputation).
@@ -1087,13 +1130,14 @@ static int nbody_adjoint_taped(const Plan& p, float* tape, int nsteps,
   NbodyWork nw = nbody_work(p, work);
   float* gm = work + nbody_work_floats(p);
   const Tape tp = tape_of(p, tape, N, nsteps);
+  float gsc = 0.0f;  // 0: gF stored; else the force cotangent is gsc * gP
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
   PMX_TRY(reverse_steps(p, tp, 0, nsteps, N, f, fac, coef, gstate, nw, gm,
-                        s));
+                        s, &gsc));
   AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
   particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
   return force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
-                   fac, nw, gm, tp.fj, u0, s);
+                   fac, nw, gm, tp.fj, u0, s, nullptr, nullptr, &gsc);
 }
 
 static int nbody_adjoint_pgd_taped(const Plan& p, float* tape, int nsteps,
@@ -1110,9 +1154,10 @@ static int nbody_adjoint_pgd_taped(const Plan& p, float* tape, int nsteps,
                        0, s, gpgd, 3 * nsteps);
     PMX_LAUNCH_CHECK();
   }
+  float gsc = 0.0f;  // stays 0: every PGD step stores gF
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
   PMX_TRY(reverse_steps(p, tp, 0, nsteps, N, f, fac, coef, gstate, nw, gm, s,
-                        pgd, gpgd));
+                        &gsc, pgd, gpgd));
   AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
   particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
   return force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,

@@ -308,6 +308,12 @@ struct AdjUpdate {
   float d, k1, k2prev;
   int mode;  // 0 none, 1 update, 2 update + k2prev, 3 zero gF
   int gx, gy, gz;  // particle grid (tiled particle order) or 0
+  // gin_scale != 0: the readout cotangent is gin_scale * gP, the force
+  // cotangent an earlier update left unstored (gin is not read)
+  float gin_scale;
+  // 1: the update does not store gF = (k1 [+ k2prev]) gP; the next force
+  // VJP reads it as gin_scale * gP
+  int implicit_gf;
 };
 // m.f4 set: the density cotangent is m.f4[cell].w and rbar is unused.
 // gin: the cotangent of the readout (gF for the force; the position
