@@ -19,10 +19,11 @@
 //   A brick whose box exceeds its 64 KB arena slot falls back to
 //   device-scope float atomics into the finished mesh (k_brick_spill, after
 //   phase 2).
-// phase 2 (k_brick_gather): one workgroup per 16^3 mesh tile evaluates the
+// phase 2 (k_brick_gather): one workgroup per mesh tile (16^3; 4 x 16 x 16
+//   for the three channels) evaluates the
 //   candidate bricks in parallel, compacts those whose boxes meet the tile
 //   (in candidate order), and every thread adds them, in that order, to the
-//   16 cells it owns; the tile is written with plain stores, no barrier per
+//   cells of the x row it owns; the tile is written with plain stores, no barrier per
 //   brick, so the mesh is never cleared first.
 //
 // Device-scope float atomics cannot complete inside one XCD's L2 (the eight
@@ -79,6 +80,8 @@ __device__ __forceinline__ bool ok_pos(float x, float y, float z) {
 // in slabs of whole x planes, one pass over the particles per slab.
 constexpr int kDT = 512;
 constexpr int kWinFloats = 10112;
+// gather tile x lengths of the density and of the three-channel paint
+constexpr int kGatherTx1 = 16, kGatherTx3 = 4;
 constexpr int kDW = kDT / 64;
 constexpr int kPPT = 2048 / kDT;
 
@@ -362,9 +365,12 @@ __global__ __launch_bounds__(1024) void k_brick_reduce(const int* __restrict__ m
   }
 }
 
-template <int C>
+// TX: the tile's x length, the cells each thread owns (16 for the density;
+// 4 for the three channels, whose 3 x 16 accumulators would take 124 VGPRs
+// and half the waves: 281 us at 256^3 with 16, 249 with 8, 235 with 4)
+template <int C, int TX>
 __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
-  constexpr int PER = kTile;  // owned cells per thread: (k, iy, iz), k < 16
+  constexpr int PER = TX;  // owned cells per thread: (k, iy, iz), k < TX
   __shared__ short cand[3][1024];
   __shared__ int ncand[3];
   __shared__ int lst[256][8];  // overlapping bricks of one chunk
@@ -373,12 +379,12 @@ __global__ __launch_bounds__(256) void k_brick_gather(BrickArgs a) {
   const MeshDims& d = a.d;
   const int tid = threadIdx.x;
   const int b = blockIdx.y;
-  const int ntx = (d.nx + kTile - 1) / kTile, nty = (d.ny + kTile - 1) / kTile,
+  const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + kTile - 1) / kTile,
             ntz = (d.nz + kTile - 1) / kTile;
   const int tt = (int)xcd_block(blockIdx.x, gridDim.x);
   const int tz = tt % ntz, ty = (tt / ntz) % nty, tx = tt / (ntz * nty);
-  const int t0x = tx * kTile, t0y = ty * kTile, t0z = tz * kTile;
-  const int Tx = min(kTile, d.nx - t0x), Ty = min(kTile, d.ny - t0y),
+  const int t0x = tx * TX, t0y = ty * kTile, t0z = tz * kTile;
+  const int Tx = min(TX, d.nx - t0x), Ty = min(kTile, d.ny - t0y),
             Tz = min(kTile, d.nz - t0z);
   const int R = a.ctr[3];
   // candidate bricks per dimension (wave w builds list w)
@@ -601,13 +607,16 @@ int launch_brick_paint(float* const* mesh, int C, const float* X,
   PMX_LAUNCH_CHECK();
   {
     ProfScope prof_(C == 1 ? kProfPaintGather : kProfPaintWGather, s);
-    const int nt = ((d.nx + kTile - 1) / kTile) * ((d.ny + kTile - 1) / kTile) *
+    const int tx = C == 1 ? kGatherTx1 : kGatherTx3;
+    const int nt = ((d.nx + tx - 1) / tx) * ((d.ny + kTile - 1) / kTile) *
                    ((d.nz + kTile - 1) / kTile);
     dim3 grid(nt, d.B);
     if (C == 1)
-      hipLaunchKernelGGL(k_brick_gather<1>, grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_brick_gather<1, kGatherTx1>), grid, dim3(256), 0,
+                         s, a);
     else
-      hipLaunchKernelGGL(k_brick_gather<3>, grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_brick_gather<3, kGatherTx3>), grid, dim3(256), 0,
+                         s, a);
     PMX_LAUNCH_CHECK();
     const long long nq = (long long)d.B * g.nbricks;
     const dim3 sgrid((unsigned)std::min<long long>(nq, 256));

@@ -166,17 +166,19 @@ static int run(const Case& cs, int C) {
   std::vector<double> mesh[3];
   for (int c = 0; c < C; ++c) mesh[c].assign(M, 0.0);
   std::vector<int> owner_hits(M, 0);
-  const int nt[3] = {(n[0] + kTile - 1) / kTile, (n[1] + kTile - 1) / kTile,
-                     (n[2] + kTile - 1) / kTile};
+  // gather tiles: 16^3, 4 x 16 x 16 for three channels (as the kernel)
+  const int tl[3] = {C == 1 ? kTile : kTile / 4, kTile, kTile};
+  const int nt[3] = {(n[0] + tl[0] - 1) / tl[0], (n[1] + tl[1] - 1) / tl[1],
+                     (n[2] + tl[2] - 1) / tl[2]};
   const int e[3] = {g.ex, g.ey, g.ez}, gd[3] = {g.gx, g.gy, g.gz},
             nb[3] = {g.nbx, g.nby, g.nbz};
   long long used = 0;
   for (int tx = 0; tx < nt[0]; ++tx)
     for (int ty = 0; ty < nt[1]; ++ty)
       for (int tz = 0; tz < nt[2]; ++tz) {
-        const int t0[3] = {tx * kTile, ty * kTile, tz * kTile};
+        const int t0[3] = {tx * tl[0], ty * tl[1], tz * tl[2]};
         int T[3];
-        for (int d = 0; d < 3; ++d) T[d] = imin(kTile, n[d] - t0[d]);
+        for (int d = 0; d < 3; ++d) T[d] = imin(tl[d], n[d] - t0[d]);
         std::vector<int> cand[3];
         for (int d = 0; d < 3; ++d)
           for (int k = 0; k < nb[d]; ++k)
