@@ -346,6 +346,37 @@ def test_nbody_gradient_against_oracle(gpu):
     assert cs > 0.999
 
 
+def test_nbody_gradient_pm_nc_factor2_against_oracle(gpu):
+  """Full-chain gradient with the force mesh twice the particle grid (16^3
+  particles, 32^3 mesh), taped and with checkpoints every 2 steps, against
+  the float64 oracle's autograd: the reverse sweep's force VJPs at f = 2
+  (paint and readout at scaled positions, cotangent weights fac / f)."""
+  n, L = 16, 64.0
+  white, pk = _ics(n, L, 13)
+  tgt = torch.rand(1, n, n, n, dtype=torch.float64,
+                   generator=torch.Generator().manual_seed(14)) * 2
+  g = obg.GrowthTable(obg.planck15())
+  stages = np.linspace(0.1, 1.0, 6)
+  wo = white.clone().requires_grad_(True)
+  lin = opm.whitenoise_to_linear(wo, L, pk)
+  st = opm.nbody(g, opm.lpt_init(g, lin, 0.1, order=2), stages, [n] * 3,
+                 pm_nc_factor=2)
+  rho = opm.cic_paint(torch.zeros_like(lin), st[0])
+  ((rho - tgt)**2).sum().backward()
+  c = fa.cosmology.Planck15()
+  for every in (0, 2):
+    wg = white.float().to(gpu).requires_grad_(True)
+    ling = fa.whitenoise_to_linear(n, L, wg, pk)
+    stg = fa.nbody(c, fa.lpt_init(c, ling, 0.1, order=2), stages, n,
+                   pm_nc_factor=2, checkpoint_every=every)
+    rhog = fa.cic_paint(torch.zeros_like(ling), stg[0])
+    ((rhog - tgt.float().to(gpu))**2).sum().backward()
+    cs = cosine(wg.grad, wo.grad)
+    print("pm_nc_factor 2 gradient cosine (every=%s): %.7f, rel %.3g" %
+          (every, cs, rel_rms(wg.grad, wo.grad)))
+    assert cs > 0.999
+
+
 def test_nbody_gradient_64_tiled(gpu):
   """Full-chain gradient at 64^3 (the particle passes of the adjoint visit
   4 x 64 tiles here), taped adjoint, against the float64 oracle."""
