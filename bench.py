@@ -14,7 +14,12 @@ Reports (one JSON line):
                            the automatic trajectory policy (at 256^3 the
                            whole trajectory is taped: no recomputation);
                            fwd_adjoint_recompute8_ms_per_step: checkpoints
-                           every 8 steps, segments recomputed
+                           every 8 steps, segments recomputed from the
+                           kept potential of every force (gradient stencil,
+                           no paint or transforms);
+                           fwd_adjoint_recompute8_states_only_ms_per_step:
+                           the same keeping the states only (segments
+                           recomputed from scratch)
   roofline                 dominant kernel: algorithmic bytes per launch /
                            average launch time from HIP events recorded on
                            the launch stream during a forward run
@@ -177,7 +182,7 @@ def main():
   del rho_final, zeros
 
   # forward + adjoint (checkpoint every k, recompute in backward)
-  fa_ms = fa8_ms = every_used = None
+  fa_ms = fa8_ms = fa8s_ms = every_used = None
   adj_kernels = None
   if not args.no_adjoint:
     x0 = state0.clone().requires_grad_(True)
@@ -197,14 +202,24 @@ def main():
     assert torch.isfinite(gx).all()
     every_used = fa.tfpm._resolve_every(plan, N, K, args.every, dev)
 
-    # the recomputing adjoint (segments of 8 steps) for comparison
-    e0.record()
-    fin = fa.nbody(cosmo, x0, stages, n, checkpoint_every=8)
-    (gx8,) = torch.autograd.grad(fin, x0, grad_outputs=gseed)
-    e1.record()
-    torch.cuda.synchronize()
-    fa8_ms = e0.elapsed_time(e1) / K
-    del gx8, fin
+    # the recomputing adjoint (segments of 8 steps) for comparison: with
+    # the potentials of every force kept (the default when they fit), and
+    # with the states only
+    def ckpt8_ms(keep):
+      fa.tfpm.set_keep_potentials(keep)
+      try:
+        for rep in range(2):  # warm-up, then timed
+          e0.record()
+          fin = fa.nbody(cosmo, x0, stages, n, checkpoint_every=8)
+          (gx8,) = torch.autograd.grad(fin, x0, grad_outputs=gseed)
+          e1.record()
+          torch.cuda.synchronize()
+        assert torch.isfinite(gx8).all()
+        return e0.elapsed_time(e1) / K
+      finally:
+        fa.tfpm.set_keep_potentials(None)
+    fa8_ms = ckpt8_ms(True)
+    fa8s_ms = ckpt8_ms(False)
 
     def fwd_adj():
       fin = fa.nbody(cosmo, x0, stages, n, checkpoint_every=args.every)
@@ -247,6 +262,7 @@ def main():
       "fwd_adjoint_ms_per_step": fa_ms,
       "adjoint_over_forward": (fa_ms / ms_step) if fa_ms else None,
       "fwd_adjoint_recompute8_ms_per_step": fa8_ms,
+      "fwd_adjoint_recompute8_states_only_ms_per_step": fa8s_ms,
       "final_cic_paint_ms": final_paint_ms,
       "roofline": {
           "bound": "hbm",

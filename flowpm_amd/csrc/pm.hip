@@ -51,8 +51,9 @@ static int plan_create(Plan** out, int B, int nx, int ny, int nz) {
   PMX_REQUIRE(B >= 1, "plan_create: batch must be >= 1");
   PMX_REQUIRE(fft_size_supported(nx) && fft_size_supported(ny) &&
                   fft_size_supported(nz),
-              "plan_create: mesh sides must be in {8,12,16,24,32,48,64,96,"
-              "128,192,256,384,512,768,1024}");
+              "plan_create: mesh sides must be in {8,12,16,20,24,32,40,48,"
+              "60,64,80,96,100,120,128,160,192,200,240,256,320,384,400,480,"
+              "512,640,768,800,960,1024}");
   Plan* p = new Plan();
   PlanDev& d = p->d;
   d.B = B;
@@ -667,9 +668,11 @@ static int paint_meshes(const Plan& p, float* const* mesh, int C,
 
 // force meshes at X (positions * f), density scaled by 1/nbar and 1/M
 // (with pgd: the PGD meshes of the same density); nc floats per cell.
+// phi_out (optional): where the potential goes (else the density buffer).
 static int force_at(const Plan& p, const float* X, long long N, float f,
                     const NbodyWork& nw, hipStream_t s,
-                    const PgdTerm* pgd = nullptr, int nc = 4) {
+                    const PgdTerm* pgd = nullptr, int nc = 4,
+                    float* phi_out = nullptr) {
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   (void)M;
@@ -684,7 +687,8 @@ static int force_at(const Plan& p, const float* X, long long N, float f,
   PMX_REQUIRE(grad_stencil_ok(p),
               "force: nz must be a multiple of 4, sides >= 4 (nz >= 8), "
               "B * cells < 2^33");
-  return force_mesh_f4(p, nw.rho, nw.w, nw.rho, nw.f4, sc, s, pgd, nc);
+  return force_mesh_f4(p, nw.rho, nw.w, phi_out ? phi_out : nw.rho, nw.f4,
+                       sc, s, pgd, nc);
 }
 
 // Where a run leaves what the reverse sweep needs of each force evaluation
@@ -701,6 +705,14 @@ struct Tape {
   float* xp = nullptr;
   float* fp = nullptr;
   int first = 0;
+  // Potentials of the run's force evaluations, M floats per slot: slot 0
+  // the jump-start force, slot 1 + i the force of step i (absolute step
+  // numbers).  phi: the run stores them there; phi_in: the run takes its
+  // force meshes from them (the gradient stencil) instead of painting and
+  // transforming, as a checkpointed reverse sweep does when it recomputes
+  // a segment whose potentials the forward run kept.
+  float* phi = nullptr;
+  const float* phi_in = nullptr;
 };
 
 // pgd (optional): (alpha, kl, ks) of every step; after the drift of step
@@ -712,6 +724,8 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   PMX_REQUIRE(!(pgd && tp && (!tp->xp || !tp->fp)),
               "nbody: a PGD run keeps its pre-correction positions and PGD "
               "meshes in the tape");
+  PMX_REQUIRE(!(pgd && tp && (tp->phi || tp->phi_in)),
+              "nbody: kept potentials are not supported with PGD");
   const long long BN3 = (long long)p.d.B * N * 3;
   const long long M = p.d.rows * p.d.nz;
   const Tape tape = tp ? *tp : Tape{};
@@ -756,6 +770,17 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     kd.xcopy = pgd ? slot : nullptr;
     x_kept = slot != nullptr;
   };
+  // the force meshes of potential slot k (0: jump start, 1 + i: step i)
+  // at Xf: from the kept potential, or painted and transformed (keeping
+  // the potential when asked)
+  auto force_k = [&](int k, const float* Xf, const NbodyWork& nw,
+                     int nc) -> int {
+    if (tape.phi_in)
+      return launch_grad_mesh(p, tape.phi_in + (long long)k * M, nw.f4, nc,
+                              s);
+    return force_at(p, Xf, N, f, nw, s, nullptr, nc,
+                    tape.phi ? tape.phi + (long long)k * M : nullptr);
+  };
   // a force mesh the tape keeps is float4 (the reverse sweep stores the
   // density cotangent in its fourth slot); one only the readout reads is
   // packed, 12 bytes per cell
@@ -763,7 +788,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     PMX_TRY(keep_x(tape.xj, X));
     const NbodyWork nw = at(tape.fj);
     const int nc = tape.fj ? 4 : 3;
-    PMX_TRY(force_at(p, X, N, f, nw, s, nullptr, nc));
+    PMX_TRY(force_k(0, X, nw, nc));
     kd.do_k2 = 0;
     kd.do_k1d = (b0 < b1);
     if (kd.do_k1d) {
@@ -799,7 +824,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     }
     const NbodyWork nw = at(tape.fs ? tape.fs + slot * 4 * M : nullptr);
     const int nc = tape.fs ? 4 : 3;
-    PMX_TRY(force_at(p, Xc, N, f, nw, s, nullptr, nc));
+    PMX_TRY(force_k(1 + i, Xc, nw, nc));
     kd.do_k2 = 1;
     kd.k2 = (float)coef[3 * i + 2];
     kd.do_k1d = (i + 1 < b1);
@@ -1082,10 +1107,14 @@ static Tape segment_tape(const Plan& p, long long N, int every, float* work,
 
 // last_taped: the forward run already recorded the last segment's tape in
 // work (nbody_run_last_segment), so that segment is not recomputed.
+// phi (optional): the potentials of every force of the forward run (slot 0
+// the jump start, 1 + i step i); the recomputation then takes its force
+// meshes from them and neither paints nor transforms.
 static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
                          int nsteps, long long N, float f, float fac,
                          const double* coef, float* gstate, float* work,
-                         hipStream_t s, bool last_taped = false) {
+                         hipStream_t s, bool last_taped = false,
+                         const float* phi = nullptr) {
   PMX_REQUIRE(every >= 1, "nbody_adjoint: every must be >= 1");
   const long long M = p.d.rows * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
@@ -1097,7 +1126,8 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int b0 = sg * every, b1 = std::min(nsteps, b0 + every);
-    const Tape tp = segment_tape(p, N, every, work, b0);
+    Tape tp = segment_tape(p, N, every, work, b0);
+    tp.phi_in = phi;
     if (!(last_taped && sg == nseg - 1)) {
       {
         ProfScope prof_(kProfAdjUpdate, s);
@@ -2046,6 +2076,56 @@ int pmx_nbody_adjoint_last_taped(const pmx_plan* plan, const float* ckpt,
               "pmx_nbody_adjoint_last_taped: bad steps");
   return nbody_adjoint(P(plan), ckpt, every, nsteps, N, f, fac, coef, gstate,
                        work, S(stream), true);
+}
+
+long long pmx_nbody_phi_floats(const pmx_plan* plan, int nsteps) {
+  if (!plan || nsteps < 0) return -1;
+  const Plan& p = P(plan);
+  return (long long)(nsteps + 1) * p.d.rows * p.d.nz;
+}
+
+int pmx_nbody_run_phi(const pmx_plan* plan, float* state, long long N,
+                      float f, float fac, const double* coef, int b0, int b1,
+                      int jump, float* work, float* phi, void* stream) {
+  PMX_REQUIRE(plan && state && work && phi, "pmx_nbody_run_phi: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_nbody_run_phi: need at least one particle");
+  PMX_REQUIRE(b0 >= 0 && b1 >= b0 && (coef || b1 == b0),
+              "pmx_nbody_run_phi: bad step range");
+  Tape tp;
+  tp.phi = phi;
+  return nbody_run(P(plan), state, N, f, fac, coef, b0, b1, jump, work,
+                   S(stream), &tp);
+}
+
+int pmx_nbody_run_last_segment_phi(const pmx_plan* plan, float* state,
+                                   long long N, float f, float fac,
+                                   const double* coef, int nsteps, int every,
+                                   float* work, float* phi, void* stream) {
+  PMX_REQUIRE(plan && state && work && phi && (coef || nsteps == 0),
+              "pmx_nbody_run_last_segment_phi: null pointer");
+  PMX_REQUIRE(N >= 1 && nsteps >= 1 && every >= 1,
+              "pmx_nbody_run_last_segment_phi: need particles, steps and "
+              "every >= 1");
+  const int nseg = (nsteps + every - 1) / every;
+  const int b0 = (nseg - 1) * every;
+  Tape tp = segment_tape(P(plan), N, every, work, b0);
+  tp.phi = phi;
+  return nbody_run(P(plan), state, N, f, fac, coef, b0, nsteps, nseg == 1,
+                   work, S(stream), &tp);
+}
+
+int pmx_nbody_adjoint_phi(const pmx_plan* plan, const float* ckpt,
+                          const float* phi, int every, int nsteps,
+                          long long N, float f, float fac, const double* coef,
+                          float* gstate, float* work, int last_taped,
+                          void* stream) {
+  PMX_REQUIRE(plan && ckpt && phi && gstate && work,
+              "pmx_nbody_adjoint_phi: null pointer");
+  PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0) &&
+                  (!last_taped || nsteps >= 1),
+              "pmx_nbody_adjoint_phi: bad steps");
+  return nbody_adjoint(P(plan), ckpt, every, nsteps, N, f, fac, coef, gstate,
+                       work, S(stream), last_taped != 0, phi);
 }
 
 }  // extern "C"

@@ -31,7 +31,8 @@ from .utils import cic_readout, white_noise, _f32, _half_to_full, \
 
 __all__ = ["linear_field", "whitenoise_to_linear", "lpt1", "lpt2_source",
            "lpt_init", "apply_longrange", "kick", "drift", "force", "nbody",
-           "step_coefficients", "set_checkpoint_every"]
+           "step_coefficients", "set_checkpoint_every",
+           "set_keep_potentials"]
 
 _CKPT_EVERY = [None]
 # automatic policy: record the whole trajectory when the tape fits in this
@@ -50,6 +51,27 @@ def set_checkpoint_every(k):
   if k is not None and int(k) < 0:
     raise ValueError("checkpoint interval must be >= 0 (or None)")
   _CKPT_EVERY[0] = None if k is None else int(k)
+
+
+_KEEP_PHI = [None]
+
+
+def set_keep_potentials(keep):
+  """Whether checkpointed differentiable nbody runs (checkpoint interval
+  k >= 1) also keep the potential of every force evaluation (one float per
+  mesh cell and step), so that the backward pass recomputes each segment's
+  force meshes with the gradient stencil instead of painting and
+  transforming again (same gradients, about half the recomputation time):
+  True, False, or None (automatic: when they fit in half of the free device
+  memory)."""
+  _KEEP_PHI[0] = None if keep is None else bool(keep)
+
+
+def _keep_phi(plan, nsteps, device):
+  if _KEEP_PHI[0] is not None:
+    return _KEEP_PHI[0]
+  need = _lib.lib().pmx_nbody_phi_floats(plan.handle, nsteps) * 4
+  return need <= _TAPE_FRACTION * torch.cuda.mem_get_info(device)[0]
 
 
 def _resolve_every(plan, N, nsteps, every, device):
@@ -679,16 +701,31 @@ class _Nbody(torch.autograd.Function):
     k = min(every, max(nsteps, 1))
     adj = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N, k),
                       dtype=torch.float32, device=st.device)
+    # the potential of every force evaluation, when kept: the backward pass
+    # then recomputes a segment's force meshes with the gradient stencil
+    phi = None
+    if _keep_phi(plan, nsteps, st.device):
+      phi = torch.empty(_lib.lib().pmx_nbody_phi_floats(plan.handle, nsteps),
+                        dtype=torch.float32, device=st.device)
     for s in range(nseg):
       ckpt[s].copy_(st)
       b0, b1 = s * every, min(nsteps, (s + 1) * every)
       if s < nseg - 1 or nsteps == 0:
-        _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac,
-                  cptr, b0, b1, 1 if s == 0 else 0, _lib.ptr(work), stream)
-      else:
+        if phi is None:
+          _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac,
+                    cptr, b0, b1, 1 if s == 0 else 0, _lib.ptr(work), stream)
+        else:
+          _lib.call("pmx_nbody_run_phi", plan.handle, _lib.ptr(st), N, f,
+                    fac, cptr, b0, b1, 1 if s == 0 else 0, _lib.ptr(work),
+                    _lib.ptr(phi), stream)
+      elif phi is None:
         _lib.call("pmx_nbody_run_last_segment", plan.handle, _lib.ptr(st), N,
                   f, fac, cptr, nsteps, k, _lib.ptr(adj), stream)
-    ctx.save_for_backward(ckpt, adj)
+      else:
+        _lib.call("pmx_nbody_run_last_segment_phi", plan.handle,
+                  _lib.ptr(st), N, f, fac, cptr, nsteps, k, _lib.ptr(adj),
+                  _lib.ptr(phi), stream)
+    ctx.save_for_backward(ckpt, adj, phi)
     ctx.args = (plan, N, f, fac, coef, every)
     return st
 
@@ -706,9 +743,17 @@ class _Nbody(torch.autograd.Function):
                 nsteps, N, f, fac, _lib.dbl(coef), _lib.ptr(gs),
                 _lib.ptr(work), _lib.stream_of(gs))
       return gs, None, None, None, None, None, None, None
-    ckpt, adj = ctx.saved_tensors
+    ckpt, adj, phi = ctx.saved_tensors
     k = min(every, max(nsteps, 1))
-    if nsteps > 0 and not getattr(ctx, "tape_used", False):
+    if phi is not None:
+      # the last segment's tape is used once; a second backward pass
+      # (retain_graph) recomputes every segment from the potentials
+      last = nsteps > 0 and not getattr(ctx, "tape_used", False)
+      _lib.call("pmx_nbody_adjoint_phi", plan.handle, _lib.ptr(ckpt),
+                _lib.ptr(phi), k, nsteps, N, f, fac, _lib.dbl(coef),
+                _lib.ptr(gs), _lib.ptr(adj), 1 if last else 0,
+                _lib.stream_of(gs))
+    elif nsteps > 0 and not getattr(ctx, "tape_used", False):
       _lib.call("pmx_nbody_adjoint_last_taped", plan.handle, _lib.ptr(ckpt),
                 k, nsteps, N, f, fac, _lib.dbl(coef), _lib.ptr(gs),
                 _lib.ptr(adj), _lib.stream_of(gs))

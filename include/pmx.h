@@ -381,6 +381,33 @@ int pmx_nbody_adjoint_last_taped(const pmx_plan* plan, const float* ckpt,
                                  float fac, const double* coef,
                                  float* gstate, float* work, void* stream);
 
+/* Checkpointing that also keeps the potential of every force evaluation
+ * (no reference counterpart: TF keeps every forward intermediate,
+ * tfpm.py:425-503).  phi holds pmx_nbody_phi_floats(plan, nsteps) floats:
+ * slot 0 (B*nx*ny*nz floats) the jump-start force's potential, slot 1 + i
+ * that of step i.  pmx_nbody_run_phi and pmx_nbody_run_last_segment_phi
+ * are pmx_nbody_run and pmx_nbody_run_last_segment that also store the
+ * potentials of the forces they evaluate in their slots (same results);
+ * pmx_nbody_adjoint_phi is pmx_nbody_adjoint (last_taped = 0) or
+ * pmx_nbody_adjoint_last_taped (last_taped = 1) whose segment
+ * recomputation takes every force mesh from its kept potential (the
+ * fourth-order gradient) instead of painting and transforming: the same
+ * gradients for about half the recomputation time.  No PGD. */
+long long pmx_nbody_phi_floats(const pmx_plan* plan, int nsteps);
+int pmx_nbody_run_phi(const pmx_plan* plan, float* state, long long N,
+                      float f, float fac, const double* coef, int step_begin,
+                      int step_end, int jump_start, float* work, float* phi,
+                      void* stream);
+int pmx_nbody_run_last_segment_phi(const pmx_plan* plan, float* state,
+                                   long long N, float f, float fac,
+                                   const double* coef, int nsteps, int every,
+                                   float* work, float* phi, void* stream);
+int pmx_nbody_adjoint_phi(const pmx_plan* plan, const float* ckpt,
+                          const float* phi, int every, int nsteps,
+                          long long N, float f, float fac, const double* coef,
+                          float* gstate, float* work, int last_taped,
+                          void* stream);
+
 /* Several independent groups of simulations of uneven sizes (one plan per
  * size class; the reference batches only equal sizes, tfpm.py:15-63):
  * group g advances its state over steps [step_begin[g], step_end[g]) of

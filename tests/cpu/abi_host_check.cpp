@@ -86,6 +86,7 @@ int main() {
       pmx_nbody_adj_work_floats(nullptr, 8, 1) >= 0 ||
       pmx_nbody_tape_floats(nullptr, 8, 2) >= 0 ||
       pmx_nbody_pgd_tape_floats(nullptr, 8, 2) >= 0 ||
+      pmx_nbody_phi_floats(nullptr, 2) >= 0 ||
       pmx_pgd_adj_work_floats(nullptr, 8) >= 0) {
     printf("FAIL a size function accepted a null plan\n");
     ++g_bad;
@@ -186,6 +187,12 @@ int main() {
   FAILS(pmx_nbody_run_last_segment(np, fp, 8, 1.0f, 0.4f, dbuf, 2, 1, fp, nullptr));
   FAILS(pmx_nbody_adjoint_last_taped(np, cfp, 1, 2, 8, 1.0f, 0.4f, dbuf, fp, fp,
                                      nullptr));
+  FAILS(pmx_nbody_run_phi(np, fp, 8, 1.0f, 0.4f, dbuf, 0, 1, 1, fp, fp,
+                          nullptr));
+  FAILS(pmx_nbody_run_last_segment_phi(np, fp, 8, 1.0f, 0.4f, dbuf, 2, 1, fp,
+                                       fp, nullptr));
+  FAILS(pmx_nbody_adjoint_phi(np, cfp, cfp, 1, 2, 8, 1.0f, 0.4f, dbuf, fp, fp,
+                              1, nullptr));
   FAILS(pmx_nbody_run_taped(np, fp, 8, 1.0f, 0.4f, dbuf, 1, fp, fp, nullptr));
   FAILS(pmx_nbody_adjoint_taped(np, fp, 1, 8, 1.0f, 0.4f, dbuf, fp, fp, nullptr));
   FAILS(pmx_nbody_run_pgd_taped(np, fp, 8, 1.0f, 0.4f, dbuf, cfp, 1, fp, fp,

@@ -487,6 +487,41 @@ def test_taped_adjoint_matches_recompute(gpu):
   close(grads[0], grads[3], 1e-6)
 
 
+@pytest.mark.parametrize("every", [1, 3, 8])
+def test_checkpointed_adjoint_with_and_without_kept_potentials(gpu, every):
+  """A checkpointed reverse sweep recomputes its segments' force meshes from
+  the potentials the forward run kept (gradient stencil) or by painting and
+  transforming again: both give the taped sweep's gradient, and so does a
+  second backward pass (retain_graph), which recomputes every segment
+  (the last one too).  6 steps: every = 8 is one segment, 3 two, 1 six."""
+  n, L = 32, 96.0
+  white, pk = _ics(n, L, 23)
+  c = fa.cosmology.Planck15()
+  stages = np.linspace(0.1, 1.0, 7)
+  gseed = torch.randn(3, 1, n**3, 3, generator=torch.Generator().manual_seed(
+      24)).to(gpu)
+  lin = fa.whitenoise_to_linear(n, L, white.float().to(gpu), pk)
+  st0 = fa.lpt_init(c, lin, 0.1, order=2)
+  x0 = st0.clone().requires_grad_(True)
+  out0 = fa.nbody(c, x0, stages, n, checkpoint_every=0)
+  (g0,) = torch.autograd.grad(out0, x0, grad_outputs=gseed)
+  for keep in (True, False):
+    fa.tfpm.set_keep_potentials(keep)
+    try:
+      x0 = st0.clone().requires_grad_(True)
+      out = fa.nbody(c, x0, stages, n, checkpoint_every=every)
+      close(out, out0, 1e-6, "final state (keep=%s)" % keep)
+      (g1,) = torch.autograd.grad(out, x0, grad_outputs=gseed,
+                                  retain_graph=True)
+      (g2,) = torch.autograd.grad(out, x0, grad_outputs=gseed)
+    finally:
+      fa.tfpm.set_keep_potentials(None)
+    e1 = close(g1, g0, 1e-6, "gradient (keep=%s)" % keep)
+    e2 = close(g2, g0, 1e-6, "second gradient (keep=%s)" % keep)
+    print("every=%d keep=%s: rel RMS %.3g / %.3g against the tape" %
+          (every, keep, e1, e2))
+
+
 @pytest.mark.parametrize("f", [1, 2])
 def test_nbody_pgd_against_oracle(gpu, f):
   """nbody with the PGD correction (tfpm.py:369-422, :487-488): final
