@@ -163,7 +163,7 @@ def main():
     if b:
       v["alg_bytes"] = b
       v["GBps"] = b / (v["avg_us"] * 1e-6) / 1e9
-    h = hbm_traffic(name, n)
+    h = hbm_traffic(name, n, tiles)
     if h:
       v["hbm_bytes"] = h  # measured (committed counter summary)
       v["hbm_GBps"] = h / (v["avg_us"] * 1e-6) / 1e9
@@ -271,7 +271,7 @@ def main():
           "peak": HBM_PEAK_GBS,
           "unit": "GB/s",
           "frac": achieved / HBM_PEAK_GBS,
-          "traffic": hbm_traffic(dom, n),
+          "traffic": hbm_traffic(dom, n, tiles),
           "copy_peak": copy_gbps,
           "frac_of_copy": achieved / copy_gbps,
       },
@@ -288,24 +288,27 @@ def main():
 
 # rocprofv3 counter kernel names (regular expressions) of the profiler
 # kinds, for the kernels the forward step launches
+# ({ty} / {tx}: the column-tile widths of the y / x passes the run used)
 _KIND_KERNEL = {
     "readout_kick_drift": r"k_readout_kick_drift<", "paint": r"k_brick_deposit<1>",
-    "paint_gather": r"k_brick_gather<1>", "fft_z_r2c": r"k_zpass_r2c<",
-    "fft_y_fwd": r"k_ypass<\d+, 16, \d+, -1, false>",
-    "fft_x_fused": r"k_xpass<\d+, 16, ", "fft_y_inv": r"k_ypass<\d+, 16, \d+, 1, false>",
-    "fft_z_c2r": r"k_zpass_c2r<\d+, 16, ", "grad": r"k_grad_mesh<", "fill": r"k_fill",
+    "paint_gather": r"k_brick_gather<1, ", "fft_z_r2c": r"k_zpass_r2c<",
+    "fft_y_fwd": r"k_ypass<\d+, {ty}, \d+, -1, false>",
+    "fft_x_fused": r"k_xpass<\d+, {tx}, \d+, false>",
+    "fft_y_inv": r"k_ypass<\d+, {ty}, \d+, 1, false>",
+    "fft_z_c2r": r"k_zpass_c2r<\d+, 16, ", "grad": r"k_grad_mesh<3>", "fill": r"k_fill",
 }
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "pm_step_256_hbm_traffic.md")
 
 
-def hbm_traffic(kind, n):
+def hbm_traffic(kind, n, tiles=(16, 16)):
   """HBM bytes per launch of this kernel kind, from the committed rocprofv3
   FETCH_SIZE / WRITE_SIZE summary of the 256^3 workload (each counter scaled
-  by what it reads for a plain 1 GiB copy); None when not measured."""
+  by what it reads for a plain 1 GiB copy); None when not measured.
+  tiles: the (y, x) column-tile widths of the run."""
   import re
   if n != 256 or kind not in _KIND_KERNEL or not os.path.exists(TRAFFIC_TABLE):
     return None
-  pat = re.compile(_KIND_KERNEL[kind])
+  pat = re.compile(_KIND_KERNEL[kind].format(ty=tiles[0], tx=tiles[1]))
   for line in open(TRAFFIC_TABLE):
     if line.startswith("| `") and pat.search(line):
       cols = [c.strip() for c in line.strip().strip("|").split("|")]
