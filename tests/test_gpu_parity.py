@@ -65,6 +65,29 @@ def test_r2c_c2r_against_numpy(gpu, shape):
   close(r, torch.tensor(refr), 2e-6)
 
 
+@pytest.mark.parametrize("tiles", [(8, 8), (8, 16), (16, 8), (16, 16)])
+def test_fft_every_column_tile_width(gpu, tiles):
+  """The y and x passes on 8- and 16-column tiles (8-column tile rows are
+  64 bytes, half a cache line, and neighbouring tiles run on one XCD)
+  against numpy.fft, on a batch of two rectangular meshes with a Nyquist
+  plane that is not a multiple of the tile width."""
+  from flowpm_amd import _plans
+  shape = (2, 48, 40, 24)
+  rng = np.random.default_rng(5)
+  x = rng.standard_normal(shape)
+  plan = _plans.get(gpu, *shape)
+  fa._lib.call("pmx_plan_set_tiles", plan.handle, *tiles)
+  try:
+    c = fa.r2c3d(torch.tensor(x, dtype=torch.float32, device=gpu))
+    ref = np.fft.fftn(x, axes=(1, 2, 3)) / np.prod(shape[1:])
+    err = np.sqrt(np.mean(np.abs(c.cpu().numpy() - ref)**2) /
+                  np.mean(np.abs(ref)**2))
+    assert err < 2e-6, err
+    close(fa.c2r3d(c), torch.tensor(x), 2e-6)
+  finally:
+    fa._lib.call("pmx_plan_set_tiles", plan.handle, 16, 16)
+
+
 def test_cic_against_oracle(gpu):
   rng = np.random.default_rng(0)
   nc = [16, 16, 16]
