@@ -41,7 +41,8 @@ struct BrickArgs {
   long long arena_cap;   // floats
   long long slot;        // arena floats per brick
   int accum;             // gather adds to the mesh instead of overwriting
-  int* meta;             // 8 ints per brick: lo[3], ext[3], off lo/hi
+  int* meta;             // 8 ints per brick: lo[3], ext[3], R (how far the
+                         // box reaches past the brick), overflow flag
   int* ctr;              // [2] overflow count, [3] box radius (written by
                          // k_brick_reduce between the two phases)
   const float* X;        // positions [B][N][3]
@@ -52,10 +53,6 @@ struct BrickArgs {
   MeshDims d;            // mesh dims and N
   BrickGrid g;
 };
-
-__global__ void k_zero_ints(int* p, int n) {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
-}
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -340,9 +337,24 @@ __global__ __launch_bounds__(1024) void k_brick_reduce(const int* __restrict__ m
                                                        int* __restrict__ ctr) {
   __shared__ int smax[16], ssum[16];
   int r = 0, o = 0;
-  for (long long i = threadIdx.x; i < nb; i += 1024) {
-    r = max(r, meta[8 * i + 6]);
-    o += meta[8 * i + 7];
+  // (R, overflow) of 8 bricks per thread and round, all loads issued before
+  // the first is used (indices past the end reload the last brick and are
+  // dropped)
+  constexpr int U = 8;
+  for (long long i0 = threadIdx.x; i0 < nb; i0 += 1024LL * U) {
+    int vr[U], vo[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = min(i0 + 1024LL * u, nb - 1);
+      vr[u] = meta[8 * i + 6];
+      vo[u] = meta[8 * i + 7];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = i0 + 1024LL * u < nb;
+      r = max(r, in ? vr[u] : 0);
+      o += in ? vo[u] : 0;
+    }
   }
 #pragma unroll
   for (int k = 32; k > 0; k >>= 1) {
