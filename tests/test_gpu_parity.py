@@ -510,18 +510,19 @@ def test_taped_adjoint_matches_recompute(gpu):
   close(grads[0], grads[3], 1e-6)
 
 
-@pytest.mark.parametrize("every", [1, 3, 8])
-def test_checkpointed_adjoint_with_and_without_kept_potentials(gpu, every):
+@pytest.mark.parametrize("every,B", [(1, 1), (3, 1), (8, 1), (3, 2)])
+def test_checkpointed_adjoint_with_and_without_kept_potentials(gpu, every, B):
   """A checkpointed reverse sweep recomputes its segments' force meshes from
   the potentials the forward run kept (gradient stencil) or by painting and
   transforming again: both give the taped sweep's gradient, and so does a
   second backward pass (retain_graph), which recomputes every segment
-  (the last one too).  6 steps: every = 8 is one segment, 3 two, 1 six."""
+  (the last one too).  6 steps: every = 8 is one segment, 3 two, 1 six;
+  B = 2 keeps one potential per member in each slot."""
   n, L = 32, 96.0
-  white, pk = _ics(n, L, 23)
+  white, pk = _ics(n, L, 23, B=B)
   c = fa.cosmology.Planck15()
   stages = np.linspace(0.1, 1.0, 7)
-  gseed = torch.randn(3, 1, n**3, 3, generator=torch.Generator().manual_seed(
+  gseed = torch.randn(3, B, n**3, 3, generator=torch.Generator().manual_seed(
       24)).to(gpu)
   lin = fa.whitenoise_to_linear(n, L, white.float().to(gpu), pk)
   st0 = fa.lpt_init(c, lin, 0.1, order=2)
@@ -541,8 +542,8 @@ def test_checkpointed_adjoint_with_and_without_kept_potentials(gpu, every):
       fa.tfpm.set_keep_potentials(None)
     e1 = close(g1, g0, 1e-6, "gradient (keep=%s)" % keep)
     e2 = close(g2, g0, 1e-6, "second gradient (keep=%s)" % keep)
-    print("every=%d keep=%s: rel RMS %.3g / %.3g against the tape" %
-          (every, keep, e1, e2))
+    print("every=%d B=%d keep=%s: rel RMS %.3g / %.3g against the tape" %
+          (every, B, keep, e1, e2))
 
 
 @pytest.mark.parametrize("f", [1, 2])
