@@ -95,18 +95,18 @@ _SIGS = {
                                          _vp]),
     "pmx_nbody_adj_work_floats": (_ll, [_vp, _ll, _i]),
     "pmx_nbody_adjoint": (_i, [_vp, _vp, _i, _i, _ll, _f, _f, _dp, _vp, _vp,
-                               _vp]),
+                               _vp, _vp]),
     "pmx_nbody_run_last_segment": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _i,
                                         _vp, _vp]),
     "pmx_nbody_adjoint_last_taped": (_i, [_vp, _vp, _i, _i, _ll, _f, _f, _dp,
-                                          _vp, _vp, _vp]),
+                                          _vp, _vp, _vp, _vp]),
     "pmx_nbody_phi_floats": (_ll, [_vp, _i]),
     "pmx_nbody_run_phi": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _i, _i, _vp,
                                _vp, _vp]),
     "pmx_nbody_run_last_segment_phi": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i,
                                             _i, _vp, _vp, _vp]),
     "pmx_nbody_adjoint_phi": (_i, [_vp, _vp, _vp, _i, _i, _ll, _f, _f, _dp,
-                                   _vp, _vp, _i, _vp]),
+                                   _vp, _vp, _i, _vp, _vp]),
     "pmx_nbody_batched_step": (_i, [_i, _c.POINTER(_vp), _c.POINTER(_vp),
                                     _c.POINTER(_ll), _c.POINTER(_f),
                                     _c.POINTER(_f), _c.POINTER(_dp),
@@ -116,7 +116,7 @@ _SIGS = {
     "pmx_nbody_run_taped": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _vp, _vp,
                                  _vp]),
     "pmx_nbody_adjoint_taped": (_i, [_vp, _vp, _i, _ll, _f, _f, _dp, _vp,
-                                     _vp, _vp]),
+                                     _vp, _vp, _vp]),
 }
 
 _LIB = None

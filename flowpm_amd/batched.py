@@ -25,6 +25,10 @@ from . import tfpm
 class _Group:
 
   def __init__(self, cosmo, state, stages, nc, pm_nc_factor, grad=False):
+    if cosmo.grad_params():
+      raise NotImplementedError(
+          "BatchedNbody: gradients w.r.t. cosmological parameters are "
+          "available through flowpm_amd.nbody, not the batched runner")
     self.plan, self.B, self.N, self.f, self.fac = tfpm._force_setup(
         cosmo, state, nc, pm_nc_factor)
     self.state = tfpm._f32(state).clone()
@@ -65,7 +69,7 @@ class _Group:
     _lib.call("pmx_nbody_adjoint_taped", self.plan.handle,
               _lib.ptr(self.tape), self.nsteps, self.N, self.f, self.fac,
               self.cptr, _lib.ptr(self.gstate), _lib.ptr(self.adj_work),
-              stream)
+              None, stream)
 
 
 class BatchedNbody:

@@ -427,6 +427,112 @@ __global__ __launch_bounds__(256) void k_readout_dot(
     atomicAdd(out, scale * ((red[0] + red[1]) + (red[2] + red[3])));
 }
 
+// Scalar cotangents of the time loop's coefficients (the cosmology-parameter
+// gradients of nbody): for the force F = (fac / f) readout(m4, f X) of one
+// step, a = sum_p (gP - dstep gX) . F and, with Xprev, b = sum_p
+// gX . (X - Xprev) / dstep (the momentum of the drift that produced X from
+// Xprev).  gP and gX are the cotangents after that step's update (gP - dstep
+// gX restores the momentum cotangent before the drift's adjoint added
+// dstep gX).  One (a, b) pair of float64 partial sums per workgroup.
+__global__ __launch_bounds__(256) void k_coef_dot(
+    const float* __restrict__ X, const float* __restrict__ Xprev,
+    const float* __restrict__ gX, const float* __restrict__ gP,
+    const float4* __restrict__ m4, MeshDims d, float f, float fac,
+    float dstep, double* __restrict__ part) {
+  __shared__ double red[2][4];
+  const long long tot = (long long)d.B * d.N;
+  const long long blk = xcd_block(blockIdx.x, gridDim.x);
+  const long long p = blk * 256 + threadIdx.x;
+  double a = 0.0, bb = 0.0;
+  if (p < tot) {
+    const int b = d.B == 1 ? 0 : (int)(p / d.N);
+    const float x0 = X[3 * p], y0 = X[3 * p + 1], z0 = X[3 * p + 2];
+    const float x = x0 * f, y = y0 * f, z = z0 * f;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    if (pos_ok(x, y, z)) {
+      const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
+      float4 v[8];
+      PMX_CORNERS({ v[cc] = m4[cell]; })
+      PMX_CORNERS({
+        const float k = (wx * wy) * wz;
+        ax += v[cc].x * k;
+        ay += v[cc].y * k;
+        az += v[cc].z * k;
+      })
+    }
+    const double Fx = (double)((ax * fac) / f), Fy = (double)((ay * fac) / f),
+                 Fz = (double)((az * fac) / f);
+    const double gx = gX[3 * p], gy = gX[3 * p + 1], gz = gX[3 * p + 2];
+    const double ds = dstep;
+    a = ((double)gP[3 * p] - ds * gx) * Fx +
+        ((double)gP[3 * p + 1] - ds * gy) * Fy +
+        ((double)gP[3 * p + 2] - ds * gz) * Fz;
+    if (Xprev) {
+      bb = (gx * ((double)x0 - (double)Xprev[3 * p]) +
+            gy * ((double)y0 - (double)Xprev[3 * p + 1]) +
+            gz * ((double)z0 - (double)Xprev[3 * p + 2])) / ds;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    bb += __shfl_xor(bb, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = bb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blk] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    part[2 * blk + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+// out[ia] += sum of the a partials, out[ib] += sum of the b partials (ia or
+// ib < 0: not wanted); one workgroup, in a fixed order (deterministic).
+__global__ __launch_bounds__(1024) void k_coef_reduce(
+    const double* __restrict__ part, long long nb, double* __restrict__ out,
+    int ia, int ib) {
+  __shared__ double red[2][16];
+  double a = 0.0, bb = 0.0;
+  constexpr int U = 8;
+  for (long long i0 = threadIdx.x; i0 < nb; i0 += 1024LL * U) {
+    double va[U], vb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = min(i0 + 1024LL * u, nb - 1);
+      va[u] = part[2 * i];
+      vb[u] = part[2 * i + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = i0 + 1024LL * u < nb;
+      a += in ? va[u] : 0.0;
+      bb += in ? vb[u] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    bb += __shfl_xor(bb, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = bb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+    for (int w = 0; w < 16; ++w) {
+      sa += red[0][w];
+      sb += red[1][w];
+    }
+    if (ia >= 0) out[ia] += sa;
+    if (ib >= 0) out[ib] += sb;
+  }
+}
+
 // PGD displacement (tfpm.py:404-421): X += c * readout(m, X * f), with
 // c = alpha / pm_nc_factor and m the three PGD meshes.
 template <bool I4>
@@ -734,6 +840,23 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
                            : k_readout_kick_drift<kMeshSep, false>);
   hipLaunchKernelGGL(kern, dim3(nblk(tot, 256)), dim3(256), 0, s, X, P, F, m,
                      d, cc);
+  PMX_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_coef_dot(const float* X, const float* Xprev, const float* gX,
+                    const float* gP, const float4* m4, const MeshDims& d,
+                    float f, float fac, float dstep, double* part, double* out,
+                    int ia, int ib, hipStream_t s) {
+  const long long tot = (long long)d.B * d.N;
+  if (tot == 0) return kOk;
+  const long long nb = nblk(tot, 256);
+  ProfScope prof_(kProfAdjUpdate, s);
+  hipLaunchKernelGGL(k_coef_dot, dim3((unsigned)nb), dim3(256), 0, s, X,
+                     Xprev, gX, gP, m4, d, f, fac, dstep, part);
+  PMX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_coef_reduce, dim3(1), dim3(1024), 0, s, part, nb, out,
+                     ia, ib);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

@@ -1030,17 +1030,36 @@ static int pgd_param_grads(const Plan& p, const float* Xpre, const float* G,
 // pgd / gpgd (optional): the PGD parameters of every step and where their
 // gradients are accumulated (3 float64 per step).
 // gsc: the force cotangent's state across calls (see force_vjp's gf_scale)
+// Where the reverse sweep accumulates the scalar cotangents of the time
+// loop's coefficients (null: not wanted): for a run of nsteps steps,
+// out[i] = sum_p gP . F_i over the force F_i of step i (gP the momentum
+// cotangent at the kicks that use F_i), out[nsteps + i] = sum_p gX . P_i
+// over the momentum P_i of the drift of step i, and out[2 nsteps] the
+// first of these for the jump-start force.  xb0: the positions before the
+// drift of the sweep's first step.
+struct CoefGrad {
+  double* out = nullptr;
+  int nsteps = 0;
+  const float* xb0 = nullptr;
+};
+static double* coef_scratch(const NbodyWork& nw) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(nw.brick);
+  return reinterpret_cast<double*>((a + 7) & ~uintptr_t(7));
+}
+
 static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
                          long long N, float f, float fac, const double* coef,
                          float* gstate, const NbodyWork& nw, float* gm,
                          hipStream_t s, float* gsc,
                          const float* pgd = nullptr,
-                         double* gpgd = nullptr) {
+                         double* gpgd = nullptr,
+                         const CoefGrad& cg = CoefGrad{}) {
   const long long M = p.d.rows * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
   float* gX = gstate;
   float* gP = gstate + BN3;
   float* gF = gstate + 2 * BN3;
+  const MeshDims md{p.d.B, p.d.nx, p.d.ny, p.d.nz, N};
   for (int i = b1 - 1; i >= b0; --i) {
     AdjUpdate u{};
     u.d = (float)coef[3 * i + 1];
@@ -1052,6 +1071,13 @@ static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
     if (!pgd) {
       PMX_TRY(force_vjp(p, tp.xs + slot * BN3, gF, gX, gP, N, f, fac, nw, gm,
                         tp.fs + slot * 4 * M, u, s, nullptr, nullptr, gsc));
+      if (cg.out) {
+        const float* xprev = i > b0 ? tp.xs + (slot - 1) * BN3 : cg.xb0;
+        PMX_TRY(launch_coef_dot(
+            tp.xs + slot * BN3, xprev, gX, gP,
+            reinterpret_cast<const float4*>(tp.fs + slot * 4 * M), md, f, fac,
+            u.d, coef_scratch(nw), cg.out, i, cg.nsteps + i, s));
+      }
       continue;
     }
     PMX_TRY(store_implicit_gf(gF, gP, gsc, BN3, s));
@@ -1114,7 +1140,8 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
                          int nsteps, long long N, float f, float fac,
                          const double* coef, float* gstate, float* work,
                          hipStream_t s, bool last_taped = false,
-                         const float* phi = nullptr) {
+                         const float* phi = nullptr,
+                         double* gcoef = nullptr) {
   PMX_REQUIRE(every >= 1, "nbody_adjoint: every must be >= 1");
   const long long M = p.d.rows * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
@@ -1139,35 +1166,58 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
       PMX_TRY(nbody_run(p, st, N, f, fac, coef, b0, b1, sg == 0, work, s,
                         &tp));
     }
+    CoefGrad cg;
+    cg.out = gcoef;
+    cg.nsteps = nsteps;
+    cg.xb0 = ckpt + (long long)sg * 3 * BN3;  // X before the segment
     PMX_TRY(reverse_steps(p, tp, b0, b1, N, f, fac, coef, gstate, nw, gm, s,
-                          &gsc));
+                          &gsc, nullptr, nullptr, cg));
   }
   float* fjump = segment_tape(p, N, every, work, 0).fj;
   // jump-start force at the initial positions; the initial F is discarded
   AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
   particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
-  return force_vjp(p, ckpt, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
-                   fac, nw, gm, nsteps > 0 ? fjump : nullptr, u0, s, nullptr,
-                   nullptr, &gsc);
+  PMX_TRY(force_vjp(p, ckpt, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
+                    fac, nw, gm, nsteps > 0 ? fjump : nullptr, u0, s, nullptr,
+                    nullptr, &gsc));
+  if (gcoef && nsteps > 0)
+    PMX_TRY(launch_coef_dot(ckpt, nullptr, gstate, gstate + BN3,
+                            reinterpret_cast<const float4*>(fjump),
+                            MeshDims{p.d.B, p.d.nx, p.d.ny, p.d.nz, N}, f,
+                            fac, 0.0f, coef_scratch(nw), gcoef, 2 * nsteps,
+                            -1, s));
+  return kOk;
 }
 
 // Reverse sweep over a tape recorded by the forward run (no recomputation).
 static int nbody_adjoint_taped(const Plan& p, float* tape, int nsteps,
                                long long N, float f, float fac,
                                const double* coef, float* gstate,
-                               float* work, hipStream_t s) {
+                               float* work, hipStream_t s,
+                               double* gcoef = nullptr) {
   const long long BN3 = (long long)p.d.B * N * 3;
   NbodyWork nw = nbody_work(p, work);
   float* gm = work + nbody_work_floats(p);
   const Tape tp = tape_of(p, tape, N, nsteps);
   float gsc = 0.0f;  // 0: gF stored; else the force cotangent is gsc * gP
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
+  CoefGrad cg;
+  cg.out = gcoef;
+  cg.nsteps = nsteps;
+  cg.xb0 = tp.xj;
   PMX_TRY(reverse_steps(p, tp, 0, nsteps, N, f, fac, coef, gstate, nw, gm,
-                        s, &gsc));
+                        s, &gsc, nullptr, nullptr, cg));
   AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
   particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
-  return force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
-                   fac, nw, gm, tp.fj, u0, s, nullptr, nullptr, &gsc);
+  PMX_TRY(force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
+                    fac, nw, gm, tp.fj, u0, s, nullptr, nullptr, &gsc));
+  if (gcoef && nsteps > 0)
+    PMX_TRY(launch_coef_dot(tp.xj, nullptr, gstate, gstate + BN3,
+                            reinterpret_cast<const float4*>(tp.fj),
+                            MeshDims{p.d.B, p.d.nx, p.d.ny, p.d.nz, N}, f,
+                            fac, 0.0f, coef_scratch(nw), gcoef, 2 * nsteps,
+                            -1, s));
+  return kOk;
 }
 
 static int nbody_adjoint_pgd_taped(const Plan& p, float* tape, int nsteps,
@@ -2029,25 +2079,25 @@ int pmx_nbody_adjoint_pgd_taped(const pmx_plan* plan, float* tape,
 int pmx_nbody_adjoint_taped(const pmx_plan* plan, float* tape, int nsteps,
                             long long N, float f, float fac,
                             const double* coef, float* gstate, float* work,
-                            void* stream) {
+                            double* gcoef, void* stream) {
   PMX_REQUIRE(plan && tape && gstate && work,
               "pmx_nbody_adjoint_taped: null pointer");
   PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
               "pmx_nbody_adjoint_taped: bad steps");
   return nbody_adjoint_taped(P(plan), tape, nsteps, N, f, fac, coef, gstate,
-                             work, S(stream));
+                             work, S(stream), gcoef);
 }
 
 int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
                       int nsteps, long long N, float f, float fac,
                       const double* coef, float* gstate, float* work,
-                      void* stream) {
+                      double* gcoef, void* stream) {
   PMX_REQUIRE(plan && ckpt && gstate && work,
               "pmx_nbody_adjoint: null pointer");
   PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
               "pmx_nbody_adjoint: bad steps");
   return nbody_adjoint(P(plan), ckpt, every, nsteps, N, f, fac, coef, gstate,
-                       work, S(stream));
+                       work, S(stream), false, nullptr, gcoef);
 }
 
 int pmx_nbody_run_last_segment(const pmx_plan* plan, float* state,
@@ -2069,13 +2119,14 @@ int pmx_nbody_run_last_segment(const pmx_plan* plan, float* state,
 int pmx_nbody_adjoint_last_taped(const pmx_plan* plan, const float* ckpt,
                                  int every, int nsteps, long long N, float f,
                                  float fac, const double* coef,
-                                 float* gstate, float* work, void* stream) {
+                                 float* gstate, float* work, double* gcoef,
+                                 void* stream) {
   PMX_REQUIRE(plan && ckpt && gstate && work,
               "pmx_nbody_adjoint_last_taped: null pointer");
   PMX_REQUIRE(nsteps >= 1 && coef,
               "pmx_nbody_adjoint_last_taped: bad steps");
   return nbody_adjoint(P(plan), ckpt, every, nsteps, N, f, fac, coef, gstate,
-                       work, S(stream), true);
+                       work, S(stream), true, nullptr, gcoef);
 }
 
 long long pmx_nbody_phi_floats(const pmx_plan* plan, int nsteps) {
@@ -2118,14 +2169,14 @@ int pmx_nbody_adjoint_phi(const pmx_plan* plan, const float* ckpt,
                           const float* phi, int every, int nsteps,
                           long long N, float f, float fac, const double* coef,
                           float* gstate, float* work, int last_taped,
-                          void* stream) {
+                          double* gcoef, void* stream) {
   PMX_REQUIRE(plan && ckpt && phi && gstate && work,
               "pmx_nbody_adjoint_phi: null pointer");
   PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0) &&
                   (!last_taped || nsteps >= 1),
               "pmx_nbody_adjoint_phi: bad steps");
   return nbody_adjoint(P(plan), ckpt, every, nsteps, N, f, fac, coef, gstate,
-                       work, S(stream), last_taped != 0, phi);
+                       work, S(stream), last_taped != 0, phi, gcoef);
 }
 
 }  // extern "C"

@@ -26,6 +26,7 @@ import torch
 from . import _lib
 from . import _plans
 from . import tfbackground as bg
+from .cosmology import host_fn
 from .utils import cic_readout, white_noise, _f32, _half_to_full, \
     _full_to_real_half, fwd_real, inv_real
 
@@ -104,15 +105,23 @@ def _half_kmesh(nc, boxsize):
 
 class _WhiteToLinear(torch.autograd.Function):
   """linear = c2r(r2c(white) * amp); amp real and even, so the map is
-  symmetric and its adjoint is itself (tfpm.py:56-62)."""
+  symmetric and its adjoint is itself (tfpm.py:56-62).  The cotangent of
+  amp (when P(k) depends on cosmological parameters that require grad) is
+  Re(W conj(G)) / M per half-spectrum mode, W and G the transforms of the
+  white field and of the output cotangent, counted twice for the modes whose
+  conjugate partner the half spectrum leaves out (0 < kz < nz/2)."""
 
   @staticmethod
   def forward(ctx, white, amp):
     B, nx, ny, nz = [int(s) for s in white.shape]
     plan = _plans.get(white.device, B, nx, ny, nz)
     ctx.plan = plan
-    ctx.save_for_backward(amp)
-    return _WhiteToLinear.apply_op(plan, _f32(white), amp)
+    w = _f32(white)
+    if ctx.needs_input_grad[1]:
+      ctx.save_for_backward(amp, w)
+    else:
+      ctx.save_for_backward(amp)
+    return _WhiteToLinear.apply_op(plan, w, amp.detach())
 
   @staticmethod
   def apply_op(plan, x, amp):
@@ -123,8 +132,60 @@ class _WhiteToLinear(torch.autograd.Function):
 
   @staticmethod
   def backward(ctx, g):
-    (amp,) = ctx.saved_tensors
-    return _WhiteToLinear.apply_op(ctx.plan, _f32(g), amp), None
+    plan = ctx.plan
+    saved = ctx.saved_tensors
+    amp = saved[0].detach()
+    g = _f32(g)
+    gw = _WhiteToLinear.apply_op(plan, g, amp) if ctx.needs_input_grad[0] \
+        else None
+    gamp = None
+    if ctx.needs_input_grad[1]:
+      B, nx, ny, H = plan.B, plan.nx, plan.ny, plan.nz // 2
+      W = torch.view_as_complex(fwd_real(plan, saved[1], 1.0).view(-1, 2))
+      G = torch.view_as_complex(fwd_real(plan, g, 1.0).view(-1, 2))
+      prod = (W.real.double() * G.real.double() +
+              W.imag.double() * G.imag.double())
+      nm = B * nx * ny * H
+      main = prod[:nm].view(B, nx, ny, H).sum(0)
+      main[:, :, 1:] *= 2.0
+      nyq = prod[nm:].view(B, nx, ny).sum(0)
+      gamp = (torch.cat([main.reshape(-1), nyq.reshape(-1)]) /
+              plan.M).to(amp.dtype)
+    return gw, gamp
+
+
+class _AmpTable(torch.autograd.Function):
+  """The amplitude over the half spectrum of one mesh from its table per
+  s = mx^2 + my^2 + mz^2 (pmx_amp_from_table); the cotangent of a table
+  entry is the sum of its modes' cotangents."""
+
+  @staticmethod
+  def forward(ctx, tab, plan1, modes):
+    dev = plan1.device
+    tab32 = tab.detach().to(device=dev, dtype=torch.float32).contiguous()
+    amp = torch.empty(modes, dtype=torch.float32, device=dev)
+    _lib.call("pmx_amp_from_table", plan1.handle, _lib.ptr(tab32),
+              tab32.numel(), _lib.ptr(amp), _lib.stream_of(amp))
+    ctx.plan1, ctx.n, ctx.meta = plan1, tab.numel(), (tab.dtype, tab.device)
+    return amp
+
+  @staticmethod
+  def backward(ctx, gamp):
+    p = ctx.plan1
+    dev = gamp.device
+    H = p.nz // 2
+
+    def m2(n):
+      i = torch.arange(n, device=dev, dtype=torch.int64)
+      m = torch.where(i < (n + 1) // 2, i, i - n)
+      return m * m
+    sx, sy, sz = m2(p.nx), m2(p.ny), m2(p.nz)
+    sxy = sx[:, None] + sy[None, :]
+    idx = torch.cat([(sxy[:, :, None] + sz[None, None, :H]).reshape(-1),
+                     (sxy + sz[H]).reshape(-1)])
+    gt = torch.zeros(ctx.n, dtype=torch.float64, device=dev)
+    gt.index_add_(0, idx, gamp.double())
+    return gt.to(dtype=ctx.meta[0], device=ctx.meta[1]), None, None
 
 
 def whitenoise_to_linear(nc, boxsize, white, pk, kvec=None, name=None):
@@ -154,16 +215,28 @@ def _linear_amp(nc, boxsize, pk, device):
   if boxsize[0] == boxsize[1] == boxsize[2] and smax + 1 < modes:
     s = np.arange(smax + 1, dtype=np.float64)
     k = 2 * np.pi * np.sqrt(s) / boxsize[0]
-    tab = (np.asarray(pk(k), dtype=np.float64).reshape(-1) / V)**0.5
+    pkv = pk(k)
+    plan = _plans.get(device, 1, *nc)
+    if isinstance(pkv, torch.Tensor) and pkv.requires_grad:
+      # P(k) depends on parameters that require grad
+      tab = (pkv.to(torch.float64).reshape(-1) / V)**0.5
+      return _AmpTable.apply(tab, plan, modes)
+    tab = (np.asarray(_np(pkv), dtype=np.float64).reshape(-1) / V)**0.5
     tab = torch.as_tensor(tab.astype(np.float32), device=device)
     amp = torch.empty(modes, dtype=torch.float32, device=device)
-    plan = _plans.get(device, 1, *nc)
     _lib.call("pmx_amp_from_table", plan.handle, _lib.ptr(tab), smax + 1,
               _lib.ptr(amp), _lib.stream_of(amp))
     return amp
-  pkv = np.asarray(pk(_half_kmesh(nc, boxsize)), dtype=np.float64).reshape(-1)
-  amp = (pkv / V)**0.5
+  pkv = pk(_half_kmesh(nc, boxsize))
+  if isinstance(pkv, torch.Tensor) and pkv.requires_grad:
+    return ((pkv.to(torch.float64).reshape(-1) / V)**0.5).to(
+        device=device, dtype=torch.float32)
+  amp = (np.asarray(_np(pkv), dtype=np.float64).reshape(-1) / V)**0.5
   return torch.as_tensor(amp.astype(np.float32), device=device)
+
+
+def _np(v):
+  return v.detach().cpu().numpy() if isinstance(v, torch.Tensor) else v
 
 
 def linear_field(nc, boxsize, pk, kvec=None, batch_size=1, seed=None,
@@ -387,9 +460,13 @@ def lpt_coefficients(cosmo, a, order):
 
 
 class _LptInit(torch.autograd.Function):
+  """coef_t (optional): the six coefficients as a float64 tensor that
+  requires grad (a cosmology with parameters that require grad); their
+  cotangents are the dot products of the state cotangent with the
+  displacement meshes."""
 
   @staticmethod
-  def forward(ctx, linear, coef, order):
+  def forward(ctx, linear, coef, order, coef_t=None):
     B, nx, ny, nz = [int(s) for s in linear.shape]
     plan = _plans.get(linear.device, B, nx, ny, nz)
     M = plan.M
@@ -409,13 +486,38 @@ class _LptInit(torch.autograd.Function):
     ctx.plan, ctx.coef, ctx.order = plan, coef, order
     if order == 2:
       ctx.save_for_backward(work2)
+    ctx.dx = (dx1, dx2) if (len(ctx.needs_input_grad) > 3 and
+                            ctx.needs_input_grad[3]) else None
     return state
+
+  @staticmethod
+  def _coef_grad(ctx, g):
+    """d L / d coef from the state cotangent g (3, B, N, 3): the state is
+    X = Q + c0 dx1 + c3 dx2, P = c1 c0 dx1 + c4 c3 dx2,
+    F = c2 c0 dx1 + c5 c3 dx2 (pmx_lpt_assemble)."""
+    c = ctx.coef
+    B, M = ctx.plan.B, ctx.plan.M
+    gd = [g[i].permute(2, 0, 1).reshape(3, B, M).double() for i in range(3)]
+    out = [0.0] * 6
+    for j, dx in enumerate(ctx.dx):
+      if dx is None:
+        continue
+      d = dx.reshape(3, B, M).double()
+      dots = [float((gd[i] * d).sum()) for i in range(3)]
+      c0, c1, c2 = c[3 * j:3 * j + 3]
+      out[3 * j] = dots[0] + c1 * dots[1] + c2 * dots[2]
+      out[3 * j + 1] = c0 * dots[1]
+      out[3 * j + 2] = c0 * dots[2]
+    return torch.tensor(out, dtype=torch.float64)
 
   @staticmethod
   def backward(ctx, g):
     plan, coef, order = ctx.plan, ctx.coef, ctx.order
     B, nx, ny, nz, M = plan.B, plan.nx, plan.ny, plan.nz, plan.M
     g = _f32(g)
+    gcoef = _LptInit._coef_grad(ctx, g) if ctx.dx is not None else None
+    if not ctx.needs_input_grad[0]:
+      return None, None, None, gcoef
     dev = g.device
     st = _lib.stream_of(g)
     g1 = torch.empty((3, B, nx, ny, nz), dtype=torch.float32, device=dev)
@@ -438,16 +540,20 @@ class _LptInit(torch.autograd.Function):
                 _lib.ptr(gsrc), _lib.ptr(gphi), st)
       # phi_ij^T = c2r(lap (i g_i)(i g_j) r2c(.)/M), added to glin
       _phi_adjoint(plan, gphi, work, glin, 1)
-    return glin, None, None
+    return glin, None, None, gcoef
 
 
 def lpt_init(cosmo, linear, a, order=2, kvec=None, name="LPTInit"):
   """LPT initial state (3, batch, npart, 3) from a linear field
-  (tfpm.py:148-180); differentiable w.r.t. ``linear``."""
+  (tfpm.py:148-180); differentiable w.r.t. ``linear`` and w.r.t. the
+  cosmology's parameters given as tensors that require grad."""
   assert order in (1, 2)
   _lib.require_cuda(linear)
   coef = lpt_coefficients(cosmo, a, order)
-  return _LptInit.apply(linear, coef, order)
+  coef_t = None
+  if cosmo.grad_params() and torch.is_grad_enabled():
+    coef_t = host_fn(lambda c: lpt_coefficients(c, a, order), cosmo)
+  return _LptInit.apply(linear, coef, order, coef_t)
 
 
 # ------------------------------------------------------------------ nbody --
@@ -551,13 +657,16 @@ def _axpy(st, dst, src, c):
 
 class _Axpy(torch.autograd.Function):
   """kick (P += c F, dst 1, src 2) and drift (X += c P, dst 0, src 1):
-  linear, so the cotangent update is g[src] += c g[dst]."""
+  linear, so the cotangent update is g[src] += c g[dst]; with c_t (c as a
+  tensor that requires grad) also d L / d c = g[dst] . state[src]."""
 
   @staticmethod
-  def forward(ctx, state, dst, src, c):
+  def forward(ctx, state, dst, src, c, c_t=None):
     _lib.require_cuda(state)
     _state_dims(state)
     st = _f32(state).clone()
+    ctx.src_val = st[src].clone() if (len(ctx.needs_input_grad) > 4 and
+                                      ctx.needs_input_grad[4]) else None
     _axpy(st, dst, src, c)
     ctx.args = (dst, src, c)
     return st
@@ -565,19 +674,36 @@ class _Axpy(torch.autograd.Function):
   @staticmethod
   def backward(ctx, g):
     dst, src, c = ctx.args
-    gs = _f32(g).clone()
+    g = _f32(g)
+    gc = None
+    if ctx.src_val is not None:
+      gc = torch.tensor(float((g[dst].double() * ctx.src_val.double()).sum()),
+                        dtype=torch.float64)
+    gs = g.clone()
     _axpy(gs, src, dst, c)
-    return gs, None, None, None
+    return (gs, None, None, None, gc)[:len(ctx.needs_input_grad)]
+
+
+def _coef_tensor(cosmo, fn):
+  """fn(cosmo) as a tensor differentiable w.r.t. the cosmology's grad
+  parameters, or None when it has none."""
+  if cosmo is None or not cosmo.grad_params() or not torch.is_grad_enabled():
+    return None
+  return host_fn(fn, cosmo)
 
 
 def kick(cosmo, state, ai, ac, af, dtype=torch.float32, name="Kick", **kw):
-  """P += fac F (tfpm.py:273-292); differentiable w.r.t. ``state``."""
-  return _Axpy.apply(state, 1, 2, _kick_fac(cosmo, ai, ac, af))
+  """P += fac F (tfpm.py:273-292); differentiable w.r.t. ``state`` and the
+  cosmology's parameters given as tensors that require grad."""
+  c_t = _coef_tensor(cosmo, lambda c: _kick_fac(c, ai, ac, af))
+  return _Axpy.apply(state, 1, 2, _kick_fac(cosmo, ai, ac, af), c_t)
 
 
 def drift(cosmo, state, ai, ac, af, dtype=torch.float32, name="Drift", **kw):
-  """X += fac P (tfpm.py:295-314); differentiable w.r.t. ``state``."""
-  return _Axpy.apply(state, 0, 1, _drift_fac(cosmo, ai, ac, af))
+  """X += fac P (tfpm.py:295-314); differentiable w.r.t. ``state`` and the
+  cosmology's parameters given as tensors that require grad."""
+  c_t = _coef_tensor(cosmo, lambda c: _drift_fac(c, ai, ac, af))
+  return _Axpy.apply(state, 0, 1, _drift_fac(cosmo, ai, ac, af), c_t)
 
 
 def _force_setup(cosmo, state, nc, pm_nc_factor):
@@ -606,7 +732,7 @@ class _Force(torch.autograd.Function):
   (retain_graph=True); it is released with the graph otherwise."""
 
   @staticmethod
-  def forward(ctx, state, plan, N, f, fac):
+  def forward(ctx, state, plan, N, f, fac, fac_t=None):
     st = _f32(state).clone()
     tape = torch.empty(_lib.lib().pmx_nbody_tape_floats(plan.handle, N, 0),
                        dtype=torch.float32, device=st.device)
@@ -615,29 +741,41 @@ class _Force(torch.autograd.Function):
               _lib.stream_of(st))
     ctx.save_for_backward(tape)
     ctx.args = (plan, N, f, fac)
+    # F = fac G(X): d L / d fac = gF . F / fac
+    ctx.f_out = st[2].clone() if (len(ctx.needs_input_grad) > 5 and
+                                  ctx.needs_input_grad[5]) else None
     return st
 
   @staticmethod
   def backward(ctx, g):
     plan, N, f, fac = ctx.args
     (tape,) = ctx.saved_tensors
-    gs = _f32(g).clone()
+    g = _f32(g)
+    gfac = None
+    if ctx.f_out is not None:
+      gfac = torch.tensor(
+          float((g[2].double() * ctx.f_out.double()).sum()) / fac,
+          dtype=torch.float64)
+    gs = g.clone()
     work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
                                                             0),
                        dtype=torch.float32, device=gs.device)
     _lib.call("pmx_nbody_adjoint_taped", plan.handle, _lib.ptr(tape), 0, N,
-              f, fac, None, _lib.ptr(gs), _lib.ptr(work), _lib.stream_of(gs))
-    return gs, None, None, None, None
+              f, fac, None, _lib.ptr(gs), _lib.ptr(work), None,
+              _lib.stream_of(gs))
+    return (gs, None, None, None, None, gfac)[:len(ctx.needs_input_grad)]
 
 
 def force(cosmo, state, nc, pm_nc_factor=1, dtype=torch.float32, name="Force",
           **kw):
   """Replaces F by the PM force at X (tfpm.py:317-366); differentiable
-  w.r.t. ``state``."""
+  w.r.t. ``state`` and the cosmology's parameters given as tensors that
+  require grad (through the factor 1.5 Omega_m)."""
   _lib.require_cuda(state)
   plan, B, N, f, fac = _force_setup(cosmo, state, nc, pm_nc_factor)
-  if torch.is_grad_enabled() and state.requires_grad:
-    return _Force.apply(state, plan, N, f, fac)
+  fac_t = _coef_tensor(cosmo, lambda c: 1.5 * c.Omega_m)
+  if torch.is_grad_enabled() and (state.requires_grad or fac_t is not None):
+    return _Force.apply(state, plan, N, f, fac, fac_t)
   st = _f32(state).clone()
   _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, None, 0, 0,
             1, _lib.ptr(_work(plan)), _lib.stream_of(st))
@@ -666,10 +804,38 @@ def step_coefficients(cosmo, stages):
   return [float(v) for v in np.stack([k1, d, k2], axis=1).reshape(-1)]
 
 
+def _coef_cotangents(coef, fac, gc, gfin):
+  """Cotangents of (k1_i, d_i, k2_i per step, fac) from the reverse sweep's
+  scalar sums gc (pmx_nbody_adjoint's gcoef: a_i = gP . F_i, b_i = gX . P_i,
+  a_J = gP . F_jump) and gfin = gF_final . F_final.  F_i feeds the closing
+  kick of step i and the opening kick of step i + 1, the jump-start force
+  the opening kick of step 0; every force is fac times a function of the
+  positions, so d L / d fac = sum over forces of gF . F / fac."""
+  n = len(coef) // 3
+  a, b, aj = gc[:n], gc[n:2 * n], gc[2 * n]
+  out = np.zeros(3 * n + 1)
+  if n:
+    out[0] = aj
+    out[3:3 * n:3] = a[:n - 1]
+    out[1::3][:n] = b
+    out[2::3][:n] = a
+  k1 = np.asarray(coef[0::3], np.float64)
+  k2 = np.asarray(coef[2::3], np.float64)
+  wa = k2 + np.append(k1[1:], 0.0)  # kicks that use F_i
+  out[3 * n] = ((float(np.dot(wa, a)) + (k1[0] * aj if n else 0.0) + gfin) /
+                fac)
+  return torch.tensor(out, dtype=torch.float64)
+
+
 class _Nbody(torch.autograd.Function):
+  """cpar (optional): the step coefficients and the force factor,
+  (k1, d, k2) per step then fac, as a float64 tensor that requires grad (a
+  cosmology with parameters that require grad); the reverse sweep then
+  also accumulates their scalar cotangents (gcoef of pmx_nbody_adjoint)."""
 
   @staticmethod
-  def forward(ctx, state, plan, N, f, fac, coef, every, need_grad):
+  def forward(ctx, state, plan, N, f, fac, coef, every, need_grad,
+              cpar=None):
     B = plan.B
     nsteps = len(coef) // 3
     st = _f32(state).clone()
@@ -680,6 +846,18 @@ class _Nbody(torch.autograd.Function):
       _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
                 0, nsteps, 1, _lib.ptr(work), stream)
       return st
+    ctx.want_c = len(ctx.needs_input_grad) > 8 and ctx.needs_input_grad[8]
+    st = _Nbody._run_for_grad(ctx, st, plan, N, f, fac, coef, every, work,
+                              cptr, stream)
+    if ctx.want_c:
+      ctx.f_final = st[2].clone()
+    return st
+
+  @staticmethod
+  def _run_for_grad(ctx, st, plan, N, f, fac, coef, every, work, cptr,
+                    stream):
+    B = plan.B
+    nsteps = len(coef) // 3
     if every == 0:
       tape = torch.empty(_lib.lib().pmx_nbody_tape_floats(plan.handle, N,
                                                           nsteps),
@@ -733,7 +911,15 @@ class _Nbody(torch.autograd.Function):
   def backward(ctx, g):
     plan, N, f, fac, coef, every = ctx.args
     nsteps = len(coef) // 3
-    gs = _f32(g).clone()
+    g = _f32(g)
+    gs = g.clone()
+    gcoef = gfin = None
+    if ctx.want_c:
+      gcoef = torch.zeros(2 * nsteps + 1, dtype=torch.float64,
+                          device=gs.device)
+      # the final force's own cotangent
+      gfin = float((g[2].double() * ctx.f_final.double()).sum())
+    gptr = _lib.ptr(gcoef) if gcoef is not None else None
     if every == 0:
       (tape,) = ctx.saved_tensors
       work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
@@ -741,31 +927,35 @@ class _Nbody(torch.autograd.Function):
                          dtype=torch.float32, device=gs.device)
       _lib.call("pmx_nbody_adjoint_taped", plan.handle, _lib.ptr(tape),
                 nsteps, N, f, fac, _lib.dbl(coef), _lib.ptr(gs),
-                _lib.ptr(work), _lib.stream_of(gs))
-      return gs, None, None, None, None, None, None, None
-    ckpt, adj, phi = ctx.saved_tensors
-    k = min(every, max(nsteps, 1))
-    if phi is not None:
-      # the last segment's tape is used once; a second backward pass
-      # (retain_graph) recomputes every segment from the potentials
-      last = nsteps > 0 and not getattr(ctx, "tape_used", False)
-      _lib.call("pmx_nbody_adjoint_phi", plan.handle, _lib.ptr(ckpt),
-                _lib.ptr(phi), k, nsteps, N, f, fac, _lib.dbl(coef),
-                _lib.ptr(gs), _lib.ptr(adj), 1 if last else 0,
-                _lib.stream_of(gs))
-    elif nsteps > 0 and not getattr(ctx, "tape_used", False):
-      _lib.call("pmx_nbody_adjoint_last_taped", plan.handle, _lib.ptr(ckpt),
-                k, nsteps, N, f, fac, _lib.dbl(coef), _lib.ptr(gs),
-                _lib.ptr(adj), _lib.stream_of(gs))
+                _lib.ptr(work), gptr, _lib.stream_of(gs))
     else:
-      # a second backward pass (retain_graph): the sweep has since recomputed
-      # the earlier segments into the same workspace, so every segment is
-      # recomputed
-      _lib.call("pmx_nbody_adjoint", plan.handle, _lib.ptr(ckpt), k, nsteps,
-                N, f, fac, _lib.dbl(coef), _lib.ptr(gs), _lib.ptr(adj),
-                _lib.stream_of(gs))
-    ctx.tape_used = True
-    return gs, None, None, None, None, None, None, None
+      ckpt, adj, phi = ctx.saved_tensors
+      k = min(every, max(nsteps, 1))
+      if phi is not None:
+        # the last segment's tape is used once; a second backward pass
+        # (retain_graph) recomputes every segment from the potentials
+        last = nsteps > 0 and not getattr(ctx, "tape_used", False)
+        _lib.call("pmx_nbody_adjoint_phi", plan.handle, _lib.ptr(ckpt),
+                  _lib.ptr(phi), k, nsteps, N, f, fac, _lib.dbl(coef),
+                  _lib.ptr(gs), _lib.ptr(adj), 1 if last else 0, gptr,
+                  _lib.stream_of(gs))
+      elif nsteps > 0 and not getattr(ctx, "tape_used", False):
+        _lib.call("pmx_nbody_adjoint_last_taped", plan.handle,
+                  _lib.ptr(ckpt), k, nsteps, N, f, fac, _lib.dbl(coef),
+                  _lib.ptr(gs), _lib.ptr(adj), gptr, _lib.stream_of(gs))
+      else:
+        # a second backward pass (retain_graph): the sweep has since
+        # recomputed the earlier segments into the same workspace, so every
+        # segment is recomputed
+        _lib.call("pmx_nbody_adjoint", plan.handle, _lib.ptr(ckpt), k,
+                  nsteps, N, f, fac, _lib.dbl(coef), _lib.ptr(gs),
+                  _lib.ptr(adj), gptr, _lib.stream_of(gs))
+      ctx.tape_used = True
+    gcpar = None
+    if gcoef is not None:
+      gcpar = _coef_cotangents(coef, fac, gcoef.cpu().numpy(), gfin)
+    out = (gs, None, None, None, None, None, None, None, gcpar)
+    return out[:len(ctx.needs_input_grad)]
 
 
 def _pgd_table(pgdparams, nsteps):
@@ -855,6 +1045,14 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
   plan, B, N, f, fac = _force_setup(cosmo, state, nc, pm_nc_factor)
   coef = step_coefficients(cosmo, stages)
   nsteps = len(stages) - 1
+  cpar = None
+  if cosmo.grad_params() and torch.is_grad_enabled():
+    if pgdparams is not None or return_intermediate_states:
+      raise NotImplementedError(
+          "nbody: gradients w.r.t. cosmological parameters are available "
+          "without pgdparams and return_intermediate_states")
+    cpar = host_fn(lambda c: np.append(step_coefficients(c, stages),
+                                       1.5 * c.Omega_m), cosmo)
   pgd = None
   if pgdparams is not None:
     pgd = _pgd_table(pgdparams, nsteps)
@@ -879,11 +1077,12 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
     st = _f32(state).clone()
     _run(plan, st, N, f, fac, _lib.dbl(coef), pgd, 0, nsteps, 1, _work(plan))
     return st
-  need_grad = torch.is_grad_enabled() and state.requires_grad
+  need_grad = torch.is_grad_enabled() and (state.requires_grad or
+                                           cpar is not None)
   every = checkpoint_every if checkpoint_every is not None else _CKPT_EVERY[0]
   every = _resolve_every(plan, N, len(coef) // 3, every,
                          state.device) if need_grad else 1
-  return _Nbody.apply(state, plan, N, f, fac, coef, every, need_grad)
+  return _Nbody.apply(state, plan, N, f, fac, coef, every, need_grad, cpar)
 
 
 def _nbody_states_differentiable(state, stages, plan, N, f, fac, coef,

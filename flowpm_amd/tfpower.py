@@ -96,10 +96,21 @@ def sigmasqr(cosmo, R, transfer_fn, kmin=0.0001, kmax=1000.0, ksteps=5,
   return 1.0 / (2.0 * np.pi**2.0) * y
 
 
-def linear_matter_power(cosmo, k, a=1.0, transfer_fn=Eisenstein_Hu, **kwargs):
-  k = np.asarray(k, np.float64)
+def _linear_matter_power(cosmo, k, a, transfer_fn, **kwargs):
   g = bkgrd.D1(cosmo, a)
   t = transfer_fn(cosmo, k, **kwargs)
   pknorm = cosmo.sigma8**2 / sigmasqr(cosmo, 8.0, transfer_fn, **kwargs)
   pk = primordial_matter_power(cosmo, k) * t**2 * g**2
   return np.squeeze(pk * pknorm)
+
+
+def linear_matter_power(cosmo, k, a=1.0, transfer_fn=Eisenstein_Hu, **kwargs):
+  """P(k) at a (tfpower.py:162-190): a float64 numpy array, or, when the
+  cosmology holds parameters as tensors requiring grad, a float64 CPU
+  tensor differentiable w.r.t. them."""
+  k = np.asarray(k, np.float64)
+  if cosmo.grad_params():
+    from .cosmology import host_fn
+    return host_fn(
+        lambda c: _linear_matter_power(c, k, a, transfer_fn, **kwargs), cosmo)
+  return _linear_matter_power(cosmo, k, a, transfer_fn, **kwargs)

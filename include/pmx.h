@@ -355,13 +355,24 @@ int pmx_nbody_run_pgd(const pmx_plan* plan, float* state, long long N,
  * gstate (3,B,N,3) holds the cotangent of the final state on entry and the
  * cotangent of the initial state on exit.  Each segment is recomputed from
  * its checkpoint keeping X and the force meshes of every step, then swept
- * in reverse.  work: pmx_nbody_adj_work_floats(plan, N, every) floats. */
+ * in reverse.  work: pmx_nbody_adj_work_floats(plan, N, every) floats.
+ *
+ * gcoef (every reverse sweep below takes it; NULL: not wanted): a device
+ * array of 2 nsteps + 1 doubles to which the sweep ADDS the scalar
+ * cotangents of the time loop's coefficients, from which a caller forms the
+ * gradients w.r.t. the coefficients of step_coefficients and fac
+ * (tfpm.py:273-314, :352-356; the cosmology-parameter gradients of
+ * scripts/check_grads.py:71-98): gcoef[i] = sum_p gP . F_i (F_i the force
+ * of step i, gP the momentum cotangent at the two kicks that use it: the
+ * cotangent of k2_i and of k1_{i+1}), gcoef[nsteps + i] = sum_p gX . P_i
+ * (P_i the momentum of step i's drift: the cotangent of d_i), and
+ * gcoef[2 nsteps] = sum_p gP . F_jump (the cotangent of k1_0). */
 long long pmx_nbody_adj_work_floats(const pmx_plan* plan, long long N,
                                     int every);
 int pmx_nbody_adjoint(const pmx_plan* plan, const float* ckpt, int every,
                       int nsteps, long long N, float f, float fac,
                       const double* coef, float* gstate, float* work,
-                      void* stream);
+                      double* gcoef, void* stream);
 
 /* The checkpointed reverse sweep without recomputing the last segment:
  * pmx_nbody_run_last_segment runs the last segment of a run of nsteps
@@ -379,7 +390,8 @@ int pmx_nbody_run_last_segment(const pmx_plan* plan, float* state,
 int pmx_nbody_adjoint_last_taped(const pmx_plan* plan, const float* ckpt,
                                  int every, int nsteps, long long N, float f,
                                  float fac, const double* coef,
-                                 float* gstate, float* work, void* stream);
+                                 float* gstate, float* work, double* gcoef,
+                                 void* stream);
 
 /* Checkpointing that also keeps the potential of every force evaluation
  * (no reference counterpart: TF keeps every forward intermediate,
@@ -406,7 +418,7 @@ int pmx_nbody_adjoint_phi(const pmx_plan* plan, const float* ckpt,
                           const float* phi, int every, int nsteps,
                           long long N, float f, float fac, const double* coef,
                           float* gstate, float* work, int last_taped,
-                          void* stream);
+                          double* gcoef, void* stream);
 
 /* Several independent groups of simulations of uneven sizes (one plan per
  * size class; the reference batches only equal sizes, tfpm.py:15-63):
@@ -437,7 +449,7 @@ int pmx_nbody_run_taped(const pmx_plan* plan, float* state, long long N,
 int pmx_nbody_adjoint_taped(const pmx_plan* plan, float* tape, int nsteps,
                             long long N, float f, float fac,
                             const double* coef, float* gstate, float* work,
-                            void* stream);
+                            double* gcoef, void* stream);
 
 /* Taped run and reverse sweep with the PGD correction (tfpm.py:369-422,
  * :487-488; pgd as in pmx_nbody_run_pgd).  The tape

@@ -183,18 +183,20 @@ int main() {
   FAILS(pmx_nbody_run(np, fp, 8, 1.0f, 0.4f, dbuf, 0, 1, 1, fp, nullptr));
   FAILS(pmx_nbody_run_noalloc(np, fp, 8, 1.0f, 0.4f, dbuf, 0, 1, 1, nullptr));
   FAILS(pmx_nbody_run_pgd(np, fp, 8, 1.0f, 0.4f, dbuf, cfp, 0, 1, 1, fp, nullptr));
-  FAILS(pmx_nbody_adjoint(np, cfp, 1, 1, 8, 1.0f, 0.4f, dbuf, fp, fp, nullptr));
+  FAILS(pmx_nbody_adjoint(np, cfp, 1, 1, 8, 1.0f, 0.4f, dbuf, fp, fp, dbuf,
+                          nullptr));
   FAILS(pmx_nbody_run_last_segment(np, fp, 8, 1.0f, 0.4f, dbuf, 2, 1, fp, nullptr));
   FAILS(pmx_nbody_adjoint_last_taped(np, cfp, 1, 2, 8, 1.0f, 0.4f, dbuf, fp, fp,
-                                     nullptr));
+                                     dbuf, nullptr));
   FAILS(pmx_nbody_run_phi(np, fp, 8, 1.0f, 0.4f, dbuf, 0, 1, 1, fp, fp,
                           nullptr));
   FAILS(pmx_nbody_run_last_segment_phi(np, fp, 8, 1.0f, 0.4f, dbuf, 2, 1, fp,
                                        fp, nullptr));
   FAILS(pmx_nbody_adjoint_phi(np, cfp, cfp, 1, 2, 8, 1.0f, 0.4f, dbuf, fp, fp,
-                              1, nullptr));
+                              1, dbuf, nullptr));
   FAILS(pmx_nbody_run_taped(np, fp, 8, 1.0f, 0.4f, dbuf, 1, fp, fp, nullptr));
-  FAILS(pmx_nbody_adjoint_taped(np, fp, 1, 8, 1.0f, 0.4f, dbuf, fp, fp, nullptr));
+  FAILS(pmx_nbody_adjoint_taped(np, fp, 1, 8, 1.0f, 0.4f, dbuf, fp, fp, dbuf,
+                                nullptr));
   FAILS(pmx_nbody_run_pgd_taped(np, fp, 8, 1.0f, 0.4f, dbuf, cfp, 1, fp, fp,
                                 nullptr));
   FAILS(pmx_nbody_adjoint_pgd_taped(np, fp, 1, 8, 1.0f, 0.4f, dbuf, cfp, fp,
