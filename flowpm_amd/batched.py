@@ -20,12 +20,13 @@ import torch
 from . import _lib
 from . import _plans
 from . import tfpm
+from .cosmology import grad_params
 
 
 class _Group:
 
   def __init__(self, cosmo, state, stages, nc, pm_nc_factor, grad=False):
-    if cosmo.grad_params():
+    if grad_params(cosmo):
       raise NotImplementedError(
           "BatchedNbody: gradients w.r.t. cosmological parameters are "
           "available through flowpm_amd.nbody, not the batched runner")

@@ -19,7 +19,7 @@ from functools import partial
 import numpy as np
 import torch
 
-__all__ = ["Cosmology", "Planck15", "host_fn"]
+__all__ = ["Cosmology", "Planck15", "host_fn", "grad_params"]
 
 _NAMES = ("Omega_c", "Omega_b", "h", "n_s", "sigma8", "Omega_k", "w0", "wa")
 
@@ -157,12 +157,19 @@ class _HostFn(torch.autograd.Function):
     return (None, None, None) + tuple(out)
 
 
+def grad_params(cosmo):
+  """The parameters a cosmology record holds as tensors that require grad
+  (empty for records of plain floats, including other parameter
+  objects)."""
+  return cosmo.grad_params() if isinstance(cosmo, Cosmology) else {}
+
+
 def host_fn(fn, cosmo):
   """fn(cosmo), a float64 host computation from the record (numpy array or
   scalar), as a float64 CPU tensor; differentiable w.r.t. the parameters
   the record holds as tensors requiring grad (central differences of fn,
   relative step 1e-6: the host functions are smooth in the parameters)."""
-  gp = cosmo.grad_params() if isinstance(cosmo, Cosmology) else {}
+  gp = grad_params(cosmo)
   if not gp or not torch.is_grad_enabled():
     return torch.from_numpy(np.array(fn(cosmo), dtype=np.float64))
   names = tuple(gp)

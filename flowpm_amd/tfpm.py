@@ -26,7 +26,7 @@ import torch
 from . import _lib
 from . import _plans
 from . import tfbackground as bg
-from .cosmology import host_fn
+from .cosmology import grad_params, host_fn
 from .utils import cic_readout, white_noise, _f32, _half_to_full, \
     _full_to_real_half, fwd_real, inv_real
 
@@ -551,7 +551,7 @@ def lpt_init(cosmo, linear, a, order=2, kvec=None, name="LPTInit"):
   _lib.require_cuda(linear)
   coef = lpt_coefficients(cosmo, a, order)
   coef_t = None
-  if cosmo.grad_params() and torch.is_grad_enabled():
+  if grad_params(cosmo) and torch.is_grad_enabled():
     coef_t = host_fn(lambda c: lpt_coefficients(c, a, order), cosmo)
   return _LptInit.apply(linear, coef, order, coef_t)
 
@@ -687,7 +687,7 @@ class _Axpy(torch.autograd.Function):
 def _coef_tensor(cosmo, fn):
   """fn(cosmo) as a tensor differentiable w.r.t. the cosmology's grad
   parameters, or None when it has none."""
-  if cosmo is None or not cosmo.grad_params() or not torch.is_grad_enabled():
+  if not grad_params(cosmo) or not torch.is_grad_enabled():
     return None
   return host_fn(fn, cosmo)
 
@@ -1046,7 +1046,7 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
   coef = step_coefficients(cosmo, stages)
   nsteps = len(stages) - 1
   cpar = None
-  if cosmo.grad_params() and torch.is_grad_enabled():
+  if grad_params(cosmo) and torch.is_grad_enabled():
     if pgdparams is not None or return_intermediate_states:
       raise NotImplementedError(
           "nbody: gradients w.r.t. cosmological parameters are available "

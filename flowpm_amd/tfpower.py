@@ -10,6 +10,7 @@ import numpy as np
 
 from . import constants as const
 from . import tfbackground as bkgrd
+from .cosmology import grad_params, host_fn
 from .scipy.integrate import simps
 
 __all__ = ["primordial_matter_power", "Eisenstein_Hu", "linear_matter_power",
@@ -109,8 +110,7 @@ def linear_matter_power(cosmo, k, a=1.0, transfer_fn=Eisenstein_Hu, **kwargs):
   cosmology holds parameters as tensors requiring grad, a float64 CPU
   tensor differentiable w.r.t. them."""
   k = np.asarray(k, np.float64)
-  if cosmo.grad_params():
-    from .cosmology import host_fn
+  if grad_params(cosmo):
     return host_fn(
         lambda c: _linear_matter_power(c, k, a, transfer_fn, **kwargs), cosmo)
   return _linear_matter_power(cosmo, k, a, transfer_fn, **kwargs)
