@@ -1047,21 +1047,23 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
   nsteps = len(stages) - 1
   cpar = None
   if grad_params(cosmo) and torch.is_grad_enabled():
-    if pgdparams is not None or return_intermediate_states:
+    if pgdparams is not None:
       raise NotImplementedError(
           "nbody: gradients w.r.t. cosmological parameters are available "
-          "without pgdparams and return_intermediate_states")
+          "without pgdparams")
     cpar = host_fn(lambda c: np.append(step_coefficients(c, stages),
                                        1.5 * c.Omega_m), cosmo)
   pgd = None
   if pgdparams is not None:
     pgd = _pgd_table(pgdparams, nsteps)
   grad = torch.is_grad_enabled() and (
-      state.requires_grad or getattr(pgdparams, "requires_grad", False))
+      state.requires_grad or getattr(pgdparams, "requires_grad", False) or
+      cpar is not None)
   if return_intermediate_states:
     if grad:
       return _nbody_states_differentiable(state, stages, plan, N, f, fac,
-                                          coef, pgdparams, checkpoint_every)
+                                          coef, pgdparams, checkpoint_every,
+                                          cpar)
     st = _f32(state).clone()
     work = _work(plan)
     cptr = _lib.dbl(coef)
@@ -1086,14 +1088,15 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
 
 
 def _nbody_states_differentiable(state, stages, plan, N, f, fac, coef,
-                                 pgdparams, checkpoint_every):
+                                 pgdparams, checkpoint_every, cpar=None):
   """[(a_i, state_i)] with gradients: one differentiable nbody run per step
   (row i of the run's coefficient table).  Every run starts with the
   jump-start force at its input positions, which recomputes the force the
   previous step stored (the same kernels on the same positions), so each
   intermediate state equals the single-run one and the composition is the
   same function of the initial state (its F is discarded either way,
-  tfpm.py:470)."""
+  tfpm.py:470).  cpar: the step coefficients and fac as a tensor that
+  requires grad (see _Nbody); each run takes its step's row and fac."""
   out = []
   st = state
   for i in range(len(stages) - 1):
@@ -1110,10 +1113,13 @@ def _nbody_states_differentiable(state, stages, plan, N, f, fac, coef,
              _work(plan))
         st = s2
     else:
-      need = torch.is_grad_enabled() and st.requires_grad
+      cp = None if cpar is None else torch.cat([cpar[3 * i:3 * i + 3],
+                                                cpar[-1:]])
+      need = torch.is_grad_enabled() and (st.requires_grad or
+                                          cp is not None)
       every = checkpoint_every if checkpoint_every is not None else \
           _CKPT_EVERY[0]
       every = _resolve_every(plan, N, 1, every, st.device) if need else 1
-      st = _Nbody.apply(st, plan, N, f, fac, c, every, need)
+      st = _Nbody.apply(st, plan, N, f, fac, c, every, need, cp)
     out.append((stages[i + 1], st))
   return out

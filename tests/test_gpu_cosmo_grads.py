@@ -175,3 +175,29 @@ def test_kick_drift_force_coefficient_gradients(gpu):
   ref = 1.5 * float((w[2].double() * out[2].detach().double()).sum()) / fac
   print("force: %.8g against %.8g" % (float(oc.grad), ref))
   assert abs(float(oc.grad) - ref) < 1e-6 * abs(ref)
+
+
+def test_intermediate_states_give_the_single_run_gradient(gpu):
+  """nbody(return_intermediate_states=True) runs one differentiable step at
+  a time (each with its own row of coefficients and the jump-start force
+  at its input): the Omega_c gradient of a loss on the last state equals
+  the single run's; a loss on every state differentiates too."""
+  n, L = 16, 64.0
+  stages = np.linspace(0.1, 1.0, 5)
+  white = _white(n, 11).float().to(gpu)
+  grads = []
+  for inter in (False, True):
+    oc = torch.tensor(0.2589, dtype=torch.float64, requires_grad=True)
+    cosmo = fa.cosmology.Planck15(Omega_c=oc)
+    pk_tab = fa.tfpower.linear_matter_power(cosmo, K)
+    pk = lambda x: fa.scipy.interpolate.interp_tf(x, K, pk_tab)
+    lin = fa.whitenoise_to_linear(n, L, white, pk)
+    st = fa.lpt_init(cosmo, lin, 0.1)
+    out = fa.nbody(cosmo, st, stages, n, return_intermediate_states=inter)
+    fin = out[-1][1] if inter else out
+    (fa.cic_paint(torch.zeros_like(lin), fin[0]).double()**2).mean().backward()
+    grads.append(float(oc.grad))
+  e = abs(grads[1] - grads[0]) / abs(grads[0])
+  print("single run %.8g, step by step %.8g, rel %.3g" % (grads[0], grads[1],
+                                                         e))
+  assert e < 1e-5
