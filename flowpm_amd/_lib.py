@@ -92,7 +92,7 @@ _SIGS = {
                                      _c.POINTER(_f), _i, _vp, _vp, _vp]),
     "pmx_nbody_adjoint_pgd_taped": (_i, [_vp, _vp, _i, _ll, _f, _f, _dp,
                                          _c.POINTER(_f), _vp, _vp, _vp,
-                                         _vp]),
+                                         _vp, _vp]),
     "pmx_nbody_adj_work_floats": (_ll, [_vp, _ll, _i]),
     "pmx_nbody_adjoint": (_i, [_vp, _vp, _i, _i, _ll, _f, _f, _dp, _vp, _vp,
                                _vp, _vp]),

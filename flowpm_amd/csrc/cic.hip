@@ -430,15 +430,16 @@ __global__ __launch_bounds__(256) void k_readout_dot(
 // Scalar cotangents of the time loop's coefficients (the cosmology-parameter
 // gradients of nbody): for the force F = (fac / f) readout(m4, f X) of one
 // step, a = sum_p (gP - dstep gX) . F and, with Xprev, b = sum_p
-// gX . (X - Xprev) / dstep (the momentum of the drift that produced X from
-// Xprev).  gP and gX are the cotangents after that step's update (gP - dstep
+// gX . (Xd - Xprev) / dstep (the momentum of the drift that produced Xd
+// from Xprev; Xd is X itself unless a PGD correction moved X after the
+// drift).  gP and gX are the cotangents after that step's update (gP - dstep
 // gX restores the momentum cotangent before the drift's adjoint added
 // dstep gX).  One (a, b) pair of float64 partial sums per workgroup.
 __global__ __launch_bounds__(256) void k_coef_dot(
-    const float* __restrict__ X, const float* __restrict__ Xprev,
-    const float* __restrict__ gX, const float* __restrict__ gP,
-    const float4* __restrict__ m4, MeshDims d, float f, float fac,
-    float dstep, double* __restrict__ part) {
+    const float* __restrict__ X, const float* __restrict__ Xd,
+    const float* __restrict__ Xprev, const float* __restrict__ gX,
+    const float* __restrict__ gP, const float4* __restrict__ m4, MeshDims d,
+    float f, float fac, float dstep, double* __restrict__ part) {
   __shared__ double red[2][4];
   const long long tot = (long long)d.B * d.N;
   const long long blk = xcd_block(blockIdx.x, gridDim.x);
@@ -468,9 +469,10 @@ __global__ __launch_bounds__(256) void k_coef_dot(
         ((double)gP[3 * p + 1] - ds * gy) * Fy +
         ((double)gP[3 * p + 2] - ds * gz) * Fz;
     if (Xprev) {
-      bb = (gx * ((double)x0 - (double)Xprev[3 * p]) +
-            gy * ((double)y0 - (double)Xprev[3 * p + 1]) +
-            gz * ((double)z0 - (double)Xprev[3 * p + 2])) / ds;
+      const float* xd = Xd ? Xd : X;
+      bb = (gx * ((double)xd[3 * p] - (double)Xprev[3 * p]) +
+            gy * ((double)xd[3 * p + 1] - (double)Xprev[3 * p + 1]) +
+            gz * ((double)xd[3 * p + 2] - (double)Xprev[3 * p + 2])) / ds;
     }
   }
 #pragma unroll
@@ -844,15 +846,16 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
   return kOk;
 }
 
-int launch_coef_dot(const float* X, const float* Xprev, const float* gX,
-                    const float* gP, const float4* m4, const MeshDims& d,
-                    float f, float fac, float dstep, double* part, double* out,
-                    int ia, int ib, hipStream_t s) {
+int launch_coef_dot(const float* X, const float* Xd, const float* Xprev,
+                    const float* gX, const float* gP, const float4* m4,
+                    const MeshDims& d, float f, float fac, float dstep,
+                    double* part, double* out, int ia, int ib,
+                    hipStream_t s) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
   const long long nb = nblk(tot, 256);
   ProfScope prof_(kProfAdjUpdate, s);
-  hipLaunchKernelGGL(k_coef_dot, dim3((unsigned)nb), dim3(256), 0, s, X,
+  hipLaunchKernelGGL(k_coef_dot, dim3((unsigned)nb), dim3(256), 0, s, X, Xd,
                      Xprev, gX, gP, m4, d, f, fac, dstep, part);
   PMX_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_coef_reduce, dim3(1), dim3(1024), 0, s, part, nb, out,

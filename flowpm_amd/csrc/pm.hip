@@ -1074,7 +1074,7 @@ static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
       if (cg.out) {
         const float* xprev = i > b0 ? tp.xs + (slot - 1) * BN3 : cg.xb0;
         PMX_TRY(launch_coef_dot(
-            tp.xs + slot * BN3, xprev, gX, gP,
+            tp.xs + slot * BN3, nullptr, xprev, gX, gP,
             reinterpret_cast<const float4*>(tp.fs + slot * 4 * M), md, f, fac,
             u.d, coef_scratch(nw), cg.out, i, cg.nsteps + i, s));
       }
@@ -1097,6 +1097,15 @@ static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
     const PgdTerm pt{kl, ks, 1};
     PMX_TRY(force_vjp(p, xpre, gF, gX, gP, N, f, alpha, nw, gm, fpm, u, s,
                       &pt, gX));
+    if (cg.out) {
+      // the force at the corrected positions, the drift to the positions
+      // before the correction
+      const float* xprev = i > b0 ? tp.xs + (slot - 1) * BN3 : cg.xb0;
+      PMX_TRY(launch_coef_dot(
+          tp.xs + slot * BN3, xpre, xprev, gX, gP,
+          reinterpret_cast<const float4*>(tp.fs + slot * 4 * M), md, f, fac,
+          u.d, coef_scratch(nw), cg.out, i, cg.nsteps + i, s));
+    }
   }
   return kOk;
 }
@@ -1181,7 +1190,7 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
                     fac, nw, gm, nsteps > 0 ? fjump : nullptr, u0, s, nullptr,
                     nullptr, &gsc));
   if (gcoef && nsteps > 0)
-    PMX_TRY(launch_coef_dot(ckpt, nullptr, gstate, gstate + BN3,
+    PMX_TRY(launch_coef_dot(ckpt, nullptr, nullptr, gstate, gstate + BN3,
                             reinterpret_cast<const float4*>(fjump),
                             MeshDims{p.d.B, p.d.nx, p.d.ny, p.d.nz, N}, f,
                             fac, 0.0f, coef_scratch(nw), gcoef, 2 * nsteps,
@@ -1212,7 +1221,7 @@ static int nbody_adjoint_taped(const Plan& p, float* tape, int nsteps,
   PMX_TRY(force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
                     fac, nw, gm, tp.fj, u0, s, nullptr, nullptr, &gsc));
   if (gcoef && nsteps > 0)
-    PMX_TRY(launch_coef_dot(tp.xj, nullptr, gstate, gstate + BN3,
+    PMX_TRY(launch_coef_dot(tp.xj, nullptr, nullptr, gstate, gstate + BN3,
                             reinterpret_cast<const float4*>(tp.fj),
                             MeshDims{p.d.B, p.d.nx, p.d.ny, p.d.nz, N}, f,
                             fac, 0.0f, coef_scratch(nw), gcoef, 2 * nsteps,
@@ -1224,7 +1233,7 @@ static int nbody_adjoint_pgd_taped(const Plan& p, float* tape, int nsteps,
                                    long long N, float f, float fac,
                                    const double* coef, const float* pgd,
                                    float* gstate, double* gpgd, float* work,
-                                   hipStream_t s) {
+                                   hipStream_t s, double* gcoef = nullptr) {
   const long long BN3 = (long long)p.d.B * N * 3;
   NbodyWork nw = nbody_work(p, work);
   float* gm = work + nbody_work_floats(p);
@@ -1236,12 +1245,23 @@ static int nbody_adjoint_pgd_taped(const Plan& p, float* tape, int nsteps,
   }
   float gsc = 0.0f;  // stays 0: every PGD step stores gF
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
+  CoefGrad cg;
+  cg.out = gcoef;
+  cg.nsteps = nsteps;
+  cg.xb0 = tp.xj;
   PMX_TRY(reverse_steps(p, tp, 0, nsteps, N, f, fac, coef, gstate, nw, gm, s,
-                        &gsc, pgd, gpgd));
+                        &gsc, pgd, gpgd, cg));
   AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
   particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
-  return force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
-                   fac, nw, gm, tp.fj, u0, s);
+  PMX_TRY(force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
+                    fac, nw, gm, tp.fj, u0, s));
+  if (gcoef && nsteps > 0)
+    PMX_TRY(launch_coef_dot(tp.xj, nullptr, nullptr, gstate, gstate + BN3,
+                            reinterpret_cast<const float4*>(tp.fj),
+                            MeshDims{p.d.B, p.d.nx, p.d.ny, p.d.nz, N}, f,
+                            fac, 0.0f, coef_scratch(nw), gcoef, 2 * nsteps,
+                            -1, s));
+  return kOk;
 }
 
 }  // namespace pmx
@@ -2067,13 +2087,13 @@ int pmx_nbody_adjoint_pgd_taped(const pmx_plan* plan, float* tape,
                                 int nsteps, long long N, float f, float fac,
                                 const double* coef, const float* pgd,
                                 float* gstate, double* gpgd, float* work,
-                                void* stream) {
+                                double* gcoef, void* stream) {
   PMX_REQUIRE(plan && tape && gstate && work && (pgd || nsteps == 0),
               "pmx_nbody_adjoint_pgd_taped: null pointer");
   PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
               "pmx_nbody_adjoint_pgd_taped: bad steps");
   return nbody_adjoint_pgd_taped(P(plan), tape, nsteps, N, f, fac, coef, pgd,
-                                 gstate, gpgd, work, S(stream));
+                                 gstate, gpgd, work, S(stream), gcoef);
 }
 
 int pmx_nbody_adjoint_taped(const pmx_plan* plan, float* tape, int nsteps,

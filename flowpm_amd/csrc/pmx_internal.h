@@ -270,12 +270,13 @@ int launch_readout_dot(const float* X, const float* G, const Mesh3& m,
 
 // Scalar cotangents of one step's coefficients (cic.hip, k_coef_dot):
 // out[ia] += sum_p (gP - dstep gX) . F, F = (fac / f) readout(m4, f X), and
-// (Xprev set) out[ib] += sum_p gX . (X - Xprev) / dstep; part: scratch of
-// 2 * ceil(B N / 256) doubles.
-int launch_coef_dot(const float* X, const float* Xprev, const float* gX,
-                    const float* gP, const float4* m4, const MeshDims& d,
-                    float f, float fac, float dstep, double* part, double* out,
-                    int ia, int ib, hipStream_t s);
+// (Xprev set) out[ib] += sum_p gX . (Xd - Xprev) / dstep (Xd null: X);
+// part: scratch of 2 * ceil(B N / 256) doubles.
+int launch_coef_dot(const float* X, const float* Xd, const float* Xprev,
+                    const float* gX, const float* gP, const float4* m4,
+                    const MeshDims& d, float f, float fac, float dstep,
+                    double* part, double* out, int ia, int ib,
+                    hipStream_t s);
 
 // X += c * readout(m, X * f) (the PGD displacement)
 int launch_pgd_shift(float* X, const Mesh3& m, const MeshDims& d, float f,

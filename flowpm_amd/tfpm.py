@@ -988,7 +988,7 @@ class _NbodyPgd(torch.autograd.Function):
   the hand-written adjoint kernels, including d loss / d (alpha, kl, ks)."""
 
   @staticmethod
-  def forward(ctx, state, pgdparams, plan, N, f, fac, coef):
+  def forward(ctx, state, pgdparams, plan, N, f, fac, coef, cpar=None):
     st = _f32(state).clone()
     nsteps = len(coef) // 3
     pgd = _pgd_table(pgdparams, nsteps)
@@ -1005,13 +1005,22 @@ class _NbodyPgd(torch.autograd.Function):
     ctx.pgd_is_tensor = isinstance(pgdparams, torch.Tensor)
     ctx.pgd_meta = (pgdparams.dtype, pgdparams.device,
                     tuple(pgdparams.shape)) if ctx.pgd_is_tensor else None
+    ctx.want_c = len(ctx.needs_input_grad) > 7 and ctx.needs_input_grad[7]
+    if ctx.want_c:
+      ctx.f_final = st[2].clone()
     return st
 
   @staticmethod
   def backward(ctx, g):
     plan, N, f, fac, coef = ctx.args
     nsteps = len(coef) // 3
-    gs = _f32(g).clone()
+    g = _f32(g)
+    gs = g.clone()
+    gcoef = gfin = None
+    if ctx.want_c:
+      gcoef = torch.zeros(2 * nsteps + 1, dtype=torch.float64,
+                          device=gs.device)
+      gfin = float((g[2].double() * ctx.f_final.double()).sum())
     work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
                                                             0),
                        dtype=torch.float32, device=gs.device)
@@ -1021,12 +1030,17 @@ class _NbodyPgd(torch.autograd.Function):
     (tape,) = ctx.saved_tensors
     _lib.call("pmx_nbody_adjoint_pgd_taped", plan.handle, _lib.ptr(tape),
               nsteps, N, f, fac, _lib.dbl(coef), ctx.pgd, _lib.ptr(gs),
-              _lib.ptr(gp), _lib.ptr(work), _lib.stream_of(gs))
+              _lib.ptr(gp), _lib.ptr(work), _lib.ptr(gcoef),
+              _lib.stream_of(gs))
     gpgd = None
     if want:
       dt, dev, shape = ctx.pgd_meta
       gpgd = gp[:3 * nsteps].reshape(shape).to(dtype=dt, device=dev)
-    return gs, gpgd, None, None, None, None, None
+    gcpar = None
+    if gcoef is not None:
+      gcpar = _coef_cotangents(coef, fac, gcoef.cpu().numpy(), gfin)
+    out = (gs, gpgd, None, None, None, None, None, gcpar)
+    return out[:len(ctx.needs_input_grad)]
 
 
 def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
@@ -1047,10 +1061,6 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
   nsteps = len(stages) - 1
   cpar = None
   if grad_params(cosmo) and torch.is_grad_enabled():
-    if pgdparams is not None:
-      raise NotImplementedError(
-          "nbody: gradients w.r.t. cosmological parameters are available "
-          "without pgdparams")
     cpar = host_fn(lambda c: np.append(step_coefficients(c, stages),
                                        1.5 * c.Omega_m), cosmo)
   pgd = None
@@ -1075,7 +1085,7 @@ def nbody(cosmo, state, stages, nc, pm_nc_factor=1,
     return out
   if pgd is not None:
     if grad:
-      return _NbodyPgd.apply(state, pgdparams, plan, N, f, fac, coef)
+      return _NbodyPgd.apply(state, pgdparams, plan, N, f, fac, coef, cpar)
     st = _f32(state).clone()
     _run(plan, st, N, f, fac, _lib.dbl(coef), pgd, 0, nsteps, 1, _work(plan))
     return st
@@ -1101,20 +1111,20 @@ def _nbody_states_differentiable(state, stages, plan, N, f, fac, coef,
   st = state
   for i in range(len(stages) - 1):
     c = coef[3 * i:3 * i + 3]
+    cp = None if cpar is None else torch.cat([cpar[3 * i:3 * i + 3],
+                                              cpar[-1:]])
     if pgdparams is not None:
       row = pgdparams[i:i + 1] if isinstance(pgdparams, torch.Tensor) else \
           np.asarray(pgdparams, dtype=np.float32).reshape(-1, 3)[i:i + 1]
-      if torch.is_grad_enabled() and (st.requires_grad or
+      if torch.is_grad_enabled() and (st.requires_grad or cp is not None or
                                       getattr(row, "requires_grad", False)):
-        st = _NbodyPgd.apply(st, row, plan, N, f, fac, c)
+        st = _NbodyPgd.apply(st, row, plan, N, f, fac, c, cp)
       else:
         s2 = _f32(st).clone()
         _run(plan, s2, N, f, fac, _lib.dbl(c), _pgd_table(row, 1), 0, 1, 1,
              _work(plan))
         st = s2
     else:
-      cp = None if cpar is None else torch.cat([cpar[3 * i:3 * i + 3],
-                                                cpar[-1:]])
       need = torch.is_grad_enabled() and (st.requires_grad or
                                           cp is not None)
       every = checkpoint_every if checkpoint_every is not None else \

@@ -458,7 +458,9 @@ int pmx_nbody_adjoint_taped(const pmx_plan* plan, float* tape, int nsteps,
  * returns the state cotangent in gstate (as pmx_nbody_adjoint_taped) and,
  * if gpgd is not NULL, d loss / d (alpha, kl, ks) of every step in
  * gpgd[3*i + {0,1,2}] (device float64, overwritten), the gradients
- * scripts/fit_pgd.py:77-117 fits the parameters with.  work:
+ * scripts/fit_pgd.py:77-117 fits the parameters with; gcoef as for
+ * pmx_nbody_adjoint (the drift momentum of step i is the drift to the
+ * positions before its correction).  work:
  * pmx_nbody_adj_work_floats(plan, N, 0) floats. */
 long long pmx_nbody_pgd_tape_floats(const pmx_plan* plan, long long N,
                                     int nsteps);
@@ -470,7 +472,7 @@ int pmx_nbody_adjoint_pgd_taped(const pmx_plan* plan, float* tape,
                                 int nsteps, long long N, float f, float fac,
                                 const double* coef, const float* pgd,
                                 float* gstate, double* gpgd, float* work,
-                                void* stream);
+                                double* gcoef, void* stream);
 
 #ifdef __cplusplus
 }

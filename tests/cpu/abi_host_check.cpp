@@ -200,7 +200,7 @@ int main() {
   FAILS(pmx_nbody_run_pgd_taped(np, fp, 8, 1.0f, 0.4f, dbuf, cfp, 1, fp, fp,
                                 nullptr));
   FAILS(pmx_nbody_adjoint_pgd_taped(np, fp, 1, 8, 1.0f, 0.4f, dbuf, cfp, fp,
-                                    dbuf, fp, nullptr));
+                                    dbuf, fp, dbuf, nullptr));
   {
     const pmx_plan* plans[1] = {nullptr};
     float* states[1] = {fp};
