@@ -19,7 +19,10 @@ Reports (one JSON line):
                            no paint or transforms);
                            fwd_adjoint_recompute8_states_only_ms_per_step:
                            the same keeping the states only (segments
-                           recomputed from scratch)
+                           recomputed from scratch);
+                           fwd_adjoint_with_omega_c_ms_per_step: the
+                           automatic policy with the gradient w.r.t.
+                           Omega_c as well
   roofline                 dominant kernel: algorithmic bytes per launch /
                            average launch time from HIP events recorded on
                            the launch stream during a forward run
@@ -182,7 +185,7 @@ def main():
   del rho_final, zeros
 
   # forward + adjoint (checkpoint every k, recompute in backward)
-  fa_ms = fa8_ms = fa8s_ms = every_used = None
+  fa_ms = fa8_ms = fa8s_ms = fac_ms = every_used = None
   adj_kernels = None
   if not args.no_adjoint:
     x0 = state0.clone().requires_grad_(True)
@@ -220,6 +223,20 @@ def main():
         fa.tfpm.set_keep_potentials(None)
     fa8_ms = ckpt8_ms(True)
     fa8s_ms = ckpt8_ms(False)
+
+    # forward + adjoint with the gradient w.r.t. Omega_c as well (the
+    # reverse sweep also accumulates the step coefficients' cotangents)
+    oc = torch.tensor(cosmo.Omega_c, dtype=torch.float64, requires_grad=True)
+    cosmo_t = fa.cosmology.Planck15(Omega_c=oc)
+    for rep in range(2):  # warm-up, then timed
+      e0.record()
+      fin = fa.nbody(cosmo_t, x0, stages, n, checkpoint_every=args.every)
+      gxc, goc = torch.autograd.grad(fin, (x0, oc), grad_outputs=gseed)
+      e1.record()
+      torch.cuda.synchronize()
+    assert torch.isfinite(gxc).all() and bool(torch.isfinite(goc))
+    fac_ms = e0.elapsed_time(e1) / K
+    del gxc, goc, fin
 
     def fwd_adj():
       fin = fa.nbody(cosmo, x0, stages, n, checkpoint_every=args.every)
@@ -263,6 +280,7 @@ def main():
       "adjoint_over_forward": (fa_ms / ms_step) if fa_ms else None,
       "fwd_adjoint_recompute8_ms_per_step": fa8_ms,
       "fwd_adjoint_recompute8_states_only_ms_per_step": fa8s_ms,
+      "fwd_adjoint_with_omega_c_ms_per_step": fac_ms,
       "final_cic_paint_ms": final_paint_ms,
       "roofline": {
           "bound": "hbm",

@@ -98,11 +98,52 @@ def rk4_integration(dydt, a, y0, cosmo):
   return y
 
 
+def _rk4_growth(a, y0, cosmo):
+  """rk4_integration(growth_ode, a, y0, cosmo) with the equation's
+  a-dependent coefficients evaluated once for every stage point (the nodes
+  and the midpoints) in vectorised float64, the stepping itself on Python
+  floats: the same RK4 to float64 rounding, ~10x faster than evaluating the
+  background functions per stage (the gradients w.r.t. cosmological
+  parameters solve it twice per parameter)."""
+  n = len(a)
+  mid = a[:-1] + (a[1:] - a[:-1]) / 2.0
+  pts = np.concatenate([a, mid])
+  om = Omega_m_a(cosmo, pts)
+  fr = Omega_de_a(cosmo, pts) - 0.5 * om + 2
+  c1 = (1.5 * om / pts**2).tolist()  # d(D')/da = c1 D - c2 D' (- c1 D1^2)
+  c2 = (fr / pts).tolist()
+  av = a.tolist()
+  y = np.empty((n, 4))
+  d1, p1, d2, p2 = (float(v) for v in y0)
+  y[0] = (d1, p1, d2, p2)
+  for i in range(n - 1):
+    h = av[i + 1] - av[i]
+    j0, jm, j1 = i, n + i, i + 1
+
+    def f(j, d1, p1, d2, p2):
+      c, k = c1[j], c2[j]
+      return p1, c * d1 - p1 * k, p2, c * d2 - p2 * k - c * d1 * d1
+
+    k1 = f(j0, d1, p1, d2, p2)
+    k2 = f(jm, d1 + h * k1[0] / 2.0, p1 + h * k1[1] / 2.0,
+           d2 + h * k1[2] / 2.0, p2 + h * k1[3] / 2.0)
+    k3 = f(jm, d1 + h * k2[0] / 2.0, p1 + h * k2[1] / 2.0,
+           d2 + h * k2[2] / 2.0, p2 + h * k2[3] / 2.0)
+    k4 = f(j1, d1 + h * k3[0], p1 + h * k3[1], d2 + h * k3[2],
+           p2 + h * k3[3])
+    d1 = d1 + h * (k1[0] + 2.0 * k2[0] + 2.0 * k3[0] + k4[0]) / 6.0
+    p1 = p1 + h * (k1[1] + 2.0 * k2[1] + 2.0 * k3[1] + k4[1]) / 6.0
+    d2 = d2 + h * (k1[2] + 2.0 * k2[2] + 2.0 * k3[2] + k4[2]) / 6.0
+    p2 = p2 + h * (k1[3] + 2.0 * k2[3] + 2.0 * k3[3] + k4[3]) / 6.0
+    y[i + 1] = (d1, p1, d2, p2)
+  return y
+
+
 def odesolve_func(a, cosmo):
   a = _a(a)
   # matter-dominated initial conditions (tfbackground.py:601)
   y0 = np.array([a[0], 1.0, -3.0 / 7 * a[0]**2, -6.0 / 7 * a[0]])
-  y = rk4_integration(growth_ode, a, y0, cosmo)
+  y = _rk4_growth(a, y0, cosmo)
   second = growth_ode(a, y, cosmo)
   n1, n2 = y[-1, 0], y[-1, 2]
   return {
