@@ -22,7 +22,9 @@ Reports (one JSON line):
                            recomputed from scratch);
                            fwd_adjoint_with_omega_c_ms_per_step: the
                            automatic policy with the gradient w.r.t.
-                           Omega_c as well
+                           Omega_c as well;
+                           fwd_adjoint_compact_tape_ms_per_step: the
+                           compact tape (positions and potentials)
   roofline                 dominant kernel: algorithmic bytes per launch /
                            average launch time from HIP events recorded on
                            the launch stream during a forward run
@@ -185,7 +187,7 @@ def main():
   del rho_final, zeros
 
   # forward + adjoint (checkpoint every k, recompute in backward)
-  fa_ms = fa8_ms = fa8s_ms = fac_ms = every_used = None
+  fa_ms = fa8_ms = fa8s_ms = fac_ms = facompact_ms = every_used = None
   adj_kernels = None
   if not args.no_adjoint:
     x0 = state0.clone().requires_grad_(True)
@@ -223,6 +225,22 @@ def main():
         fa.tfpm.set_keep_potentials(None)
     fa8_ms = ckpt8_ms(True)
     fa8s_ms = ckpt8_ms(False)
+
+    # the compact tape: positions and potentials, force meshes formed again
+    # in the backward pass
+    fa.tfpm.set_compact_tape(True)
+    try:
+      for rep in range(2):  # warm-up, then timed
+        e0.record()
+        fin = fa.nbody(cosmo, x0, stages, n, checkpoint_every=0)
+        (gxc,) = torch.autograd.grad(fin, x0, grad_outputs=gseed)
+        e1.record()
+        torch.cuda.synchronize()
+      assert torch.isfinite(gxc).all()
+      facompact_ms = e0.elapsed_time(e1) / K
+    finally:
+      fa.tfpm.set_compact_tape(None)
+    del gxc, fin
 
     # forward + adjoint with the gradient w.r.t. Omega_c as well (the
     # reverse sweep also accumulates the step coefficients' cotangents)
@@ -281,6 +299,7 @@ def main():
       "fwd_adjoint_recompute8_ms_per_step": fa8_ms,
       "fwd_adjoint_recompute8_states_only_ms_per_step": fa8s_ms,
       "fwd_adjoint_with_omega_c_ms_per_step": fac_ms,
+      "fwd_adjoint_compact_tape_ms_per_step": facompact_ms,
       "final_cic_paint_ms": final_paint_ms,
       "roofline": {
           "bound": "hbm",

@@ -113,6 +113,11 @@ _SIGS = {
                                     _c.POINTER(_i), _c.POINTER(_i),
                                     _c.POINTER(_i), _c.POINTER(_vp), _vp]),
     "pmx_nbody_tape_floats": (_ll, [_vp, _ll, _i]),
+    "pmx_nbody_compact_tape_floats": (_ll, [_vp, _ll, _i]),
+    "pmx_nbody_run_compact_taped": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _vp,
+                                         _vp, _vp]),
+    "pmx_nbody_adjoint_compact_taped": (_i, [_vp, _vp, _i, _ll, _f, _f, _dp,
+                                             _vp, _vp, _vp, _vp]),
     "pmx_nbody_run_taped": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _vp, _vp,
                                  _vp]),
     "pmx_nbody_adjoint_taped": (_i, [_vp, _vp, _i, _ll, _f, _f, _dp, _vp,

@@ -977,6 +977,28 @@ static Tape tape_of(const Plan& p, float* tape, long long N, int nsteps,
   return t;
 }
 
+// Compact tape of a whole run: X of the jump-start force and of every step,
+// then the potential of every force evaluation (slot 0 the jump start,
+// 1 + i step i) instead of its float4 force mesh: (nsteps + 1)(BN3 + M)
+// floats against (nsteps + 1)(BN3 + 4M).  The forward run keeps the packed
+// force meshes of an untaped run; the reverse sweep takes each step's
+// float4 mesh from its potential with the gradient stencil.
+static long long compact_tape_floats(const Plan& p, long long N, int nsteps) {
+  const long long M = p.d.rows * p.d.nz;
+  const long long BN3 = 3LL * p.d.B * N;
+  return (long long)(nsteps + 1) * (BN3 + M);
+}
+static Tape compact_tape_of(const Plan& p, float* tape, long long N,
+                            int nsteps) {
+  const long long BN3 = 3LL * p.d.B * N;
+  Tape t;
+  t.xj = tape;
+  t.xs = tape + BN3;
+  t.phi = tape + (long long)(nsteps + 1) * BN3;
+  t.first = 0;
+  return t;
+}
+
 // PGD run: the tape also keeps, per step, the pre-correction positions and
 // the PGD meshes
 static long long pgd_tape_floats(const Plan& p, long long N, int nsteps) {
@@ -1069,14 +1091,20 @@ static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
     particle_grid(p, N, f, &u.gx, &u.gy, &u.gz);
     const long long slot = i - tp.first;
     if (!pgd) {
+      // the step's float4 force mesh: taped, or (compact tape) from its
+      // potential into the force-mesh scratch
+      float* fm = tp.fs ? tp.fs + slot * 4 * M : nw.f4;
+      if (!tp.fs)
+        PMX_TRY(launch_grad_mesh(p, tp.phi_in + (long long)(1 + i) * M, fm,
+                                 4, s));
       PMX_TRY(force_vjp(p, tp.xs + slot * BN3, gF, gX, gP, N, f, fac, nw, gm,
-                        tp.fs + slot * 4 * M, u, s, nullptr, nullptr, gsc));
+                        fm, u, s, nullptr, nullptr, gsc));
       if (cg.out) {
         const float* xprev = i > b0 ? tp.xs + (slot - 1) * BN3 : cg.xb0;
         PMX_TRY(launch_coef_dot(
             tp.xs + slot * BN3, nullptr, xprev, gX, gP,
-            reinterpret_cast<const float4*>(tp.fs + slot * 4 * M), md, f, fac,
-            u.d, coef_scratch(nw), cg.out, i, cg.nsteps + i, s));
+            reinterpret_cast<const float4*>(fm), md, f, fac, u.d,
+            coef_scratch(nw), cg.out, i, cg.nsteps + i, s));
       }
       continue;
     }
@@ -1203,11 +1231,20 @@ static int nbody_adjoint_taped(const Plan& p, float* tape, int nsteps,
                                long long N, float f, float fac,
                                const double* coef, float* gstate,
                                float* work, hipStream_t s,
-                               double* gcoef = nullptr) {
+                               double* gcoef = nullptr,
+                               bool compact = false) {
   const long long BN3 = (long long)p.d.B * N * 3;
   NbodyWork nw = nbody_work(p, work);
   float* gm = work + nbody_work_floats(p);
-  const Tape tp = tape_of(p, tape, N, nsteps);
+  Tape tp = compact ? compact_tape_of(p, tape, N, nsteps)
+                    : tape_of(p, tape, N, nsteps);
+  if (compact) {
+    // the reverse sweep reads the potentials, and forms each force mesh
+    tp.phi_in = tp.phi;
+    tp.phi = nullptr;
+    PMX_REQUIRE(grad_stencil_ok(p),
+                "compact tape: nz must be a multiple of 4, sides >= 4");
+  }
   float gsc = 0.0f;  // 0: gF stored; else the force cotangent is gsc * gP
   PMX_TRY(adjoint_open(p, N, nsteps, coef, gstate, s));
   CoefGrad cg;
@@ -1218,11 +1255,16 @@ static int nbody_adjoint_taped(const Plan& p, float* tape, int nsteps,
                         s, &gsc, nullptr, nullptr, cg));
   AdjUpdate u0{0.f, 0.f, 0.f, 3, 0, 0, 0};
   particle_grid(p, N, f, &u0.gx, &u0.gy, &u0.gz);
+  float* fj = tp.fj;
+  if (compact) {
+    fj = nw.f4;
+    PMX_TRY(launch_grad_mesh(p, tp.phi_in, fj, 4, s));
+  }
   PMX_TRY(force_vjp(p, tp.xj, gstate + 2 * BN3, gstate, gstate + BN3, N, f,
-                    fac, nw, gm, tp.fj, u0, s, nullptr, nullptr, &gsc));
+                    fac, nw, gm, fj, u0, s, nullptr, nullptr, &gsc));
   if (gcoef && nsteps > 0)
     PMX_TRY(launch_coef_dot(tp.xj, nullptr, nullptr, gstate, gstate + BN3,
-                            reinterpret_cast<const float4*>(tp.fj),
+                            reinterpret_cast<const float4*>(fj),
                             MeshDims{p.d.B, p.d.nx, p.d.ny, p.d.nz, N}, f,
                             fac, 0.0f, coef_scratch(nw), gcoef, 2 * nsteps,
                             -1, s));
@@ -2058,6 +2100,41 @@ int pmx_nbody_run_taped(const pmx_plan* plan, float* state, long long N,
   const Tape tp = tape_of(P(plan), tape, N, nsteps);
   return nbody_run(P(plan), state, N, f, fac, coef, 0, nsteps, 1, work,
                    S(stream), &tp);
+}
+
+long long pmx_nbody_compact_tape_floats(const pmx_plan* plan, long long N,
+                                        int nsteps) {
+  return (plan && N >= 1 && nsteps >= 0)
+             ? compact_tape_floats(P(plan), N, nsteps)
+             : -1;
+}
+
+int pmx_nbody_run_compact_taped(const pmx_plan* plan, float* state,
+                                long long N, float f, float fac,
+                                const double* coef, int nsteps, float* tape,
+                                float* work, void* stream) {
+  PMX_REQUIRE(plan && state && tape && work,
+              "pmx_nbody_run_compact_taped: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_nbody_run_compact_taped: need at least one "
+                      "particle");
+  PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
+              "pmx_nbody_run_compact_taped: bad steps");
+  const Tape tp = compact_tape_of(P(plan), tape, N, nsteps);
+  return nbody_run(P(plan), state, N, f, fac, coef, 0, nsteps, 1, work,
+                   S(stream), &tp);
+}
+
+int pmx_nbody_adjoint_compact_taped(const pmx_plan* plan, float* tape,
+                                    int nsteps, long long N, float f,
+                                    float fac, const double* coef,
+                                    float* gstate, float* work,
+                                    double* gcoef, void* stream) {
+  PMX_REQUIRE(plan && tape && gstate && work,
+              "pmx_nbody_adjoint_compact_taped: null pointer");
+  PMX_REQUIRE(nsteps >= 0 && (coef || nsteps == 0),
+              "pmx_nbody_adjoint_compact_taped: bad steps");
+  return nbody_adjoint_taped(P(plan), tape, nsteps, N, f, fac, coef, gstate,
+                             work, S(stream), gcoef, true);
 }
 
 long long pmx_nbody_pgd_tape_floats(const pmx_plan* plan, long long N,

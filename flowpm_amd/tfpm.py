@@ -33,7 +33,7 @@ from .utils import cic_readout, white_noise, _f32, _half_to_full, \
 __all__ = ["linear_field", "whitenoise_to_linear", "lpt1", "lpt2_source",
            "lpt_init", "apply_longrange", "kick", "drift", "force", "nbody",
            "step_coefficients", "set_checkpoint_every",
-           "set_keep_potentials"]
+           "set_keep_potentials", "set_compact_tape"]
 
 _CKPT_EVERY = [None]
 # automatic policy: record the whole trajectory when the tape fits in this
@@ -47,8 +47,9 @@ def set_checkpoint_every(k):
 
   k >= 1: keep the state every k steps and recompute each segment in the
   backward pass; k = 0: record the positions and force meshes of every step
-  in the forward pass (no recomputation); None: automatic (k = 0 when that
-  tape fits in half of the free device memory, else k = 8)."""
+  in the forward pass (no recomputation; or the positions and potentials,
+  see set_compact_tape); None: automatic (k = 0 when the tape, full or
+  compact, fits in half of the free device memory, else k = 8)."""
   if k is not None and int(k) < 0:
     raise ValueError("checkpoint interval must be >= 0 (or None)")
   _CKPT_EVERY[0] = None if k is None else int(k)
@@ -75,12 +76,42 @@ def _keep_phi(plan, nsteps, device):
   return need <= _TAPE_FRACTION * torch.cuda.mem_get_info(device)[0]
 
 
+_COMPACT = [None]
+
+
+def set_compact_tape(compact):
+  """Whether a taped differentiable nbody run (checkpoint interval 0) keeps
+  the potential of every force evaluation instead of its float4 force mesh
+  (about half the memory: at 256^3 x 40 steps 11 GB against 18.8 GB; the
+  backward pass forms each force mesh again with the gradient stencil, the
+  same gradients): True, False, or None (automatic: compact only when the
+  full tape does not fit in half of the free device memory)."""
+  _COMPACT[0] = None if compact is None else bool(compact)
+
+
+def _tape_fits(plan, N, nsteps, device, name):
+  need = getattr(_lib.lib(), name)(plan.handle, N, nsteps) * 4
+  return need <= _TAPE_FRACTION * torch.cuda.mem_get_info(device)[0]
+
+
+def _compact(plan, N, nsteps, device):
+  if _COMPACT[0] is not None:
+    return _COMPACT[0]
+  return not _tape_fits(plan, N, nsteps, device, "pmx_nbody_tape_floats")
+
+
 def _resolve_every(plan, N, nsteps, every, device):
+  """Checkpoint interval of the automatic policy: 0 (a tape: the full one,
+  or the compact one when only that fits in half of the free device
+  memory), else segments of 8 steps."""
   if every is not None:
     return int(every)
-  tape = _lib.lib().pmx_nbody_tape_floats(plan.handle, N, nsteps) * 4
-  free = torch.cuda.mem_get_info(device)[0]
-  return 0 if tape <= _TAPE_FRACTION * free else _AUTO_SEGMENT
+  if _tape_fits(plan, N, nsteps, device, "pmx_nbody_tape_floats"):
+    return 0
+  if _COMPACT[0] is not False and _tape_fits(
+      plan, N, nsteps, device, "pmx_nbody_compact_tape_floats"):
+    return 0
+  return _AUTO_SEGMENT
 
 
 def _nc3(nc):
@@ -859,11 +890,15 @@ class _Nbody(torch.autograd.Function):
     B = plan.B
     nsteps = len(coef) // 3
     if every == 0:
-      tape = torch.empty(_lib.lib().pmx_nbody_tape_floats(plan.handle, N,
-                                                          nsteps),
+      # the full tape (X and the float4 force mesh of every force), or the
+      # compact one (X and the potential)
+      ctx.compact = _compact(plan, N, nsteps, st.device)
+      kind = "compact_" if ctx.compact else ""
+      tape = torch.empty(getattr(_lib.lib(), "pmx_nbody_%stape_floats" %
+                                 kind)(plan.handle, N, nsteps),
                          dtype=torch.float32, device=st.device)
-      _lib.call("pmx_nbody_run_taped", plan.handle, _lib.ptr(st), N, f, fac,
-                cptr, nsteps, _lib.ptr(tape), _lib.ptr(work), stream)
+      _lib.call("pmx_nbody_run_%staped" % kind, plan.handle, _lib.ptr(st), N,
+                f, fac, cptr, nsteps, _lib.ptr(tape), _lib.ptr(work), stream)
       # released with the graph; the reverse sweep rewrites only the fourth
       # (density cotangent) component of its force meshes, so a second
       # backward pass (retain_graph=True) reads an intact tape
@@ -925,9 +960,10 @@ class _Nbody(torch.autograd.Function):
       work = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N,
                                                               0),
                          dtype=torch.float32, device=gs.device)
-      _lib.call("pmx_nbody_adjoint_taped", plan.handle, _lib.ptr(tape),
-                nsteps, N, f, fac, _lib.dbl(coef), _lib.ptr(gs),
-                _lib.ptr(work), gptr, _lib.stream_of(gs))
+      kind = "compact_" if ctx.compact else ""
+      _lib.call("pmx_nbody_adjoint_%staped" % kind, plan.handle,
+                _lib.ptr(tape), nsteps, N, f, fac, _lib.dbl(coef),
+                _lib.ptr(gs), _lib.ptr(work), gptr, _lib.stream_of(gs))
     else:
       ckpt, adj, phi = ctx.saved_tensors
       k = min(every, max(nsteps, 1))

@@ -451,6 +451,28 @@ int pmx_nbody_adjoint_taped(const pmx_plan* plan, float* tape, int nsteps,
                             const double* coef, float* gstate, float* work,
                             double* gcoef, void* stream);
 
+/* Compact tape (no reference counterpart: TF keeps every forward
+ * intermediate, tfpm.py:425-503): the same run as pmx_nbody_run_taped
+ * keeping X and the potential of every force evaluation instead of its
+ * float4 force mesh (pmx_nbody_compact_tape_floats(plan, N, nsteps) =
+ * (nsteps + 1)(3 B N + B nx ny nz) floats, about half of
+ * pmx_nbody_tape_floats); the forward uses the packed meshes of an untaped
+ * run, and pmx_nbody_adjoint_compact_taped forms each step's force mesh from
+ * its potential with the gradient stencil before the step's VJP: the same
+ * gradients as pmx_nbody_adjoint_taped, bit for bit.  work and gcoef as for
+ * pmx_nbody_adjoint_taped. */
+long long pmx_nbody_compact_tape_floats(const pmx_plan* plan, long long N,
+                                        int nsteps);
+int pmx_nbody_run_compact_taped(const pmx_plan* plan, float* state,
+                                long long N, float f, float fac,
+                                const double* coef, int nsteps, float* tape,
+                                float* work, void* stream);
+int pmx_nbody_adjoint_compact_taped(const pmx_plan* plan, float* tape,
+                                    int nsteps, long long N, float f,
+                                    float fac, const double* coef,
+                                    float* gstate, float* work,
+                                    double* gcoef, void* stream);
+
 /* Taped run and reverse sweep with the PGD correction (tfpm.py:369-422,
  * :487-488; pgd as in pmx_nbody_run_pgd).  The tape
  * (pmx_nbody_pgd_tape_floats floats) also keeps, per step, the positions

@@ -87,6 +87,7 @@ int main() {
       pmx_nbody_tape_floats(nullptr, 8, 2) >= 0 ||
       pmx_nbody_pgd_tape_floats(nullptr, 8, 2) >= 0 ||
       pmx_nbody_phi_floats(nullptr, 2) >= 0 ||
+      pmx_nbody_compact_tape_floats(nullptr, 8, 2) >= 0 ||
       pmx_pgd_adj_work_floats(nullptr, 8) >= 0) {
     printf("FAIL a size function accepted a null plan\n");
     ++g_bad;
@@ -195,6 +196,10 @@ int main() {
   FAILS(pmx_nbody_adjoint_phi(np, cfp, cfp, 1, 2, 8, 1.0f, 0.4f, dbuf, fp, fp,
                               1, dbuf, nullptr));
   FAILS(pmx_nbody_run_taped(np, fp, 8, 1.0f, 0.4f, dbuf, 1, fp, fp, nullptr));
+  FAILS(pmx_nbody_run_compact_taped(np, fp, 8, 1.0f, 0.4f, dbuf, 1, fp, fp,
+                                    nullptr));
+  FAILS(pmx_nbody_adjoint_compact_taped(np, fp, 1, 8, 1.0f, 0.4f, dbuf, fp, fp,
+                                        dbuf, nullptr));
   FAILS(pmx_nbody_adjoint_taped(np, fp, 1, 8, 1.0f, 0.4f, dbuf, fp, fp, dbuf,
                                 nullptr));
   FAILS(pmx_nbody_run_pgd_taped(np, fp, 8, 1.0f, 0.4f, dbuf, cfp, 1, fp, fp,

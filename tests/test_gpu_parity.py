@@ -546,6 +546,41 @@ def test_checkpointed_adjoint_with_and_without_kept_potentials(gpu, every, B):
           (every, B, keep, e1, e2))
 
 
+@pytest.mark.parametrize("B", [1, 2])
+def test_compact_tape_matches_full_tape(gpu, B):
+  """The compact tape (positions and potential of every force; the reverse
+  sweep forms each float4 force mesh with the gradient stencil) gives the
+  full tape's final state and gradients bit for bit, w.r.t. the state and
+  Omega_c, also on a second backward pass."""
+  n, L = 32, 96.0
+  white, pk = _ics(n, L, 25, B=B)
+  c0 = fa.cosmology.Planck15()
+  stages = np.linspace(0.1, 1.0, 6)
+  gseed = torch.randn(3, B, n**3, 3, generator=torch.Generator().manual_seed(
+      26)).to(gpu)
+  lin = fa.whitenoise_to_linear(n, L, white.float().to(gpu), pk)
+  st0 = fa.lpt_init(c0, lin, 0.1, order=2)
+  res = {}
+  for compact in (False, True):
+    fa.tfpm.set_compact_tape(compact)
+    try:
+      oc = torch.tensor(0.2589, dtype=torch.float64, requires_grad=True)
+      c = fa.cosmology.Planck15(Omega_c=oc)
+      x0 = st0.clone().requires_grad_(True)
+      out = fa.nbody(c, x0, stages, n, checkpoint_every=0)
+      g1 = torch.autograd.grad(out, (x0, oc), grad_outputs=gseed,
+                               retain_graph=True)
+      g2 = torch.autograd.grad(out, (x0, oc), grad_outputs=gseed)
+    finally:
+      fa.tfpm.set_compact_tape(None)
+    res[compact] = (out.detach(), g1, g2)
+  (o0, a0, b0), (o1, a1, b1) = res[False], res[True]
+  assert torch.equal(o0, o1)
+  for x, y in ((a0, a1), (b0, b1), (a0, b1)):
+    assert torch.equal(x[0], y[0]) and float(x[1]) == float(y[1])
+  print("B=%d: dL/dOmega_c %.8g with either tape" % (B, float(a0[1])))
+
+
 @pytest.mark.parametrize("f", [1, 2])
 def test_nbody_pgd_against_oracle(gpu, f):
   """nbody with the PGD correction (tfpm.py:369-422, :487-488): final
