@@ -634,12 +634,19 @@ __global__ __launch_bounds__(256) void k_force_vjp_particles(
 // gin is the readout's cotangent (gF for the force, gX itself for the PGD
 // displacement: gX and gin may alias, so neither is __restrict__; every
 // thread loads its inputs before it stores).
-template <bool I4>
+// CG (the step-coefficient cotangents, with an update): the block's partial
+// sums a = sum_p (gP' - d gX') . F_p and b = sum_p gX' . (X_p - Xprev_p) / d
+// of the updated cotangents gX', gP' and the force F_p = cf * (readout of
+// the mesh) go to part[2 * block], part[2 * block + 1] in float64; the
+// separate dot-product pass (k_coef_dot) is then not needed.
+template <bool I4, bool CG>
 __global__ __launch_bounds__(256) void k_force_vjp_finish(
     float* gX, float* __restrict__ gP, float* gF, const float* gin,
     const float* __restrict__ X, Mesh3 m, const float* __restrict__ rbar,
-    MeshDims d, float f, float cf, AdjUpdate u) {
-  __shared__ float sh[4][768];
+    MeshDims d, float f, float cf, AdjUpdate u,
+    const float* __restrict__ Xprev, double* __restrict__ part) {
+  __shared__ float sh[CG ? 5 : 4][768];
+  __shared__ double red[2][4];
   const long long tot = (long long)d.B * d.N;
   const long long lb = xcd_block(blockIdx.x, gridDim.x);
   const bool tiled = u.gx > 0;  // the launcher checked the grid
@@ -654,9 +661,10 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
   // particle streams: non-temporal (read once, kept out of L2 where the
   // gathered mesh is re-read by neighbouring blocks); a full block stages
   // them through LDS so every load and store is a contiguous run
-  float xv[3], gfv[3], gxv[3], gpv[3];
+  float xv[3], gfv[3], gxv[3], gpv[3], xpv[3];
+  constexpr int NQ = CG ? 5 : 4;
   if (full) {
-    float v[4][3];
+    float v[NQ][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const long long o = stage_off(tiled, p0, u.gz, t + 256 * k);
@@ -664,9 +672,10 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
       v[1][k] = gin_gp ? 0.0f : __builtin_nontemporal_load(gin + o);
       v[2][k] = __builtin_nontemporal_load(gX + o);
       v[3][k] = ldgp ? __builtin_nontemporal_load(gP + o) : 0.0f;
+      if (CG) v[NQ - 1][k] = __builtin_nontemporal_load(Xprev + o);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
       for (int k = 0; k < 3; ++k) sh[q][t + 256 * k] = v[q][k];
     __syncthreads();
@@ -676,15 +685,20 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
       gfv[c] = sh[1][3 * t + c];
       gxv[c] = sh[2][3 * t + c];
       gpv[c] = sh[3][3 * t + c];
+      xpv[c] = CG ? sh[NQ - 1][3 * t + c] : 0.0f;
     }
   } else {
-    if (p >= tot) return;  // no barrier follows on this path
+    // the block reduction of CG needs every thread: a thread past the end
+    // computes on the last particle and contributes nothing
+    if (!CG && p >= tot) return;  // no barrier follows on this path
+    const long long q = p < tot ? p : tot - 1;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      xv[c] = X[3 * p + c];
-      gfv[c] = gin_gp ? 0.0f : gin[3 * p + c];
-      gxv[c] = gX[3 * p + c];
-      gpv[c] = ldgp ? gP[3 * p + c] : 0.0f;
+      xv[c] = X[3 * q + c];
+      gfv[c] = gin_gp ? 0.0f : gin[3 * q + c];
+      gxv[c] = gX[3 * q + c];
+      gpv[c] = ldgp ? gP[3 * q + c] : 0.0f;
+      xpv[c] = CG ? Xprev[3 * q + c] : 0.0f;
     }
   }
   if (gin_gp) {
@@ -696,6 +710,7 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
   const float x = xv[0] * f, y = xv[1] * f, z = xv[2] * f;
   const float gfx = gfv[0] * cf, gfy = gfv[1] * cf, gfz = gfv[2] * cf;
   float sx = 0.f, sy = 0.f, sz = 0.f, rx = 0.f, ry = 0.f, rz = 0.f;
+  float ax = 0.f, ay = 0.f, az = 0.f;  // readout of the force mesh (CG)
   if (pos_ok(x, y, z)) {
     const Cic1 cx = cic1(x, d.nx), cy = cic1(y, d.ny), cz = cic1(z, d.nz);
     // the 8 (or 16) gathers are all issued before the first is used
@@ -723,6 +738,12 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
       rx += rb[cc] * kx;
       ry += rb[cc] * ky;
       rz += rb[cc] * kz;
+      if (CG) {
+        const float k = (wx * wy) * wz;
+        ax += fm[cc].x * k;
+        ay += fm[cc].y * k;
+        az += fm[cc].z * k;
+      }
     })
   }
   float g[3], gp[3], gf[3];
@@ -753,12 +774,40 @@ __global__ __launch_bounds__(256) void k_force_vjp_finish(
       if (upd) __builtin_nontemporal_store(sh[1][t + 256 * k], gP + o);
       if (wgf) __builtin_nontemporal_store(sh[2][t + 256 * k], gF + o);
     }
-  } else {
+  } else if (p < tot) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       gX[3 * p + c] = g[c];
       if (upd) gP[3 * p + c] = gp[c];
       if (wgf) gF[3 * p + c] = gf[c];
+    }
+  }
+  if (CG) {
+    const float av[3] = {ax, ay, az};
+    double a = 0.0, bb = 0.0;
+    if (full || p < tot) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double gx = g[c];
+        a += ((double)gp[c] - (double)u.d * gx) * (double)(av[c] * cf);
+        bb += gx * ((double)xv[c] - (double)xpv[c]);
+      }
+      bb /= (double)u.d;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o);
+      bb += __shfl_xor(bb, o);
+    }
+    if ((t & 63) == 0) {
+      red[0][t >> 6] = a;
+      red[1][t >> 6] = bb;
+    }
+    __syncthreads();
+    if (t == 0) {
+      part[2 * blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+      part[2 * blockIdx.x + 1] =
+          (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     }
   }
 }
@@ -912,21 +961,41 @@ int launch_kick_drift_only(float* X, float* P, const float* F,
 int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* gin,
                             const float* X, const Mesh3& m, const float* rbar,
                             const MeshDims& d, float f, float cf,
-                            const AdjUpdate& u, hipStream_t s) {
+                            const AdjUpdate& u, hipStream_t s,
+                            const CoefDot* cd) {
   const long long tot = (long long)d.B * d.N;
   if (tot == 0) return kOk;
-  ProfScope prof_(kProfAdjCic, s);
   AdjUpdate uu = u;
   if (!grid_tiled(u.gx, u.gy, u.gz, d.N)) uu.gx = uu.gy = uu.gz = 0;
   PMX_REQUIRE(!m.f3, "force_vjp_finish: no packed-mesh variant");
+  const unsigned nb = nblk(tot, 256);
+  if (cd) {
+    PMX_REQUIRE(m.f4 && (u.mode == 1 || u.mode == 2) && cd->xprev &&
+                    cd->part && cd->out,
+                "force_vjp_finish: coefficient dots need the interleaved "
+                "mesh, an update and Xprev / part / out");
+    {
+      ProfScope prof_(kProfAdjCic, s);
+      hipLaunchKernelGGL((k_force_vjp_finish<true, true>), dim3(nb), dim3(256),
+                         0, s, gX, gP, gF, gin ? gin : gF, X, m, rbar, d, f,
+                         cf, uu, cd->xprev, cd->part);
+      PMX_LAUNCH_CHECK();
+    }
+    ProfScope prof_(kProfAdjUpdate, s);
+    hipLaunchKernelGGL(k_coef_reduce, dim3(1), dim3(1024), 0, s, cd->part,
+                       (long long)nb, cd->out, cd->ia, cd->ib);
+    PMX_LAUNCH_CHECK();
+    return kOk;
+  }
+  ProfScope prof_(kProfAdjCic, s);
   if (m.f4)
-    hipLaunchKernelGGL(k_force_vjp_finish<true>, dim3(nblk(tot, 256)),
-                       dim3(256), 0, s, gX, gP, gF, gin ? gin : gF, X, m,
-                       rbar, d, f, cf, uu);
+    hipLaunchKernelGGL((k_force_vjp_finish<true, false>), dim3(nb), dim3(256),
+                       0, s, gX, gP, gF, gin ? gin : gF, X, m, rbar, d, f, cf,
+                       uu, nullptr, nullptr);
   else
-    hipLaunchKernelGGL(k_force_vjp_finish<false>, dim3(nblk(tot, 256)),
+    hipLaunchKernelGGL((k_force_vjp_finish<false, false>), dim3(nb),
                        dim3(256), 0, s, gX, gP, gF, gin ? gin : gF, X, m,
-                       rbar, d, f, cf, uu);
+                       rbar, d, f, cf, uu, nullptr, nullptr);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

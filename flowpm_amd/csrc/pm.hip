@@ -884,7 +884,8 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
                      const AdjUpdate& u, hipStream_t s,
                      const PgdTerm* pgd = nullptr,
                      const float* gin = nullptr,
-                     float* gf_scale = nullptr) {
+                     float* gf_scale = nullptr,
+                     const CoefDot* cd = nullptr, bool* cd_done = nullptr) {
   const long long M = p.d.rows * p.d.nz;
   const long long Mb = (long long)p.d.nx * p.d.ny * p.d.nz;
   const long long BN3 = (long long)p.d.B * N * 3;
@@ -914,8 +915,11 @@ static int force_vjp(const Plan& p, const float* X, float* gF, float* gX,
     uu.gin_scale = gin ? 0.0f : gin_scale;
     const bool upd = u.mode == 1 || u.mode == 2;
     uu.implicit_gf = (gf_scale && !pgd && !gin && upd) ? 1 : 0;
+    // the step-coefficient dot products ride along (they need the update)
+    const bool fuse = cd && m.f4 && upd;
     PMX_TRY(launch_force_vjp_finish(gX, gP, gF, gc, X, m, nullptr, md, f,
-                                    fac / f, uu, s));
+                                    fac / f, uu, s, fuse ? cd : nullptr));
+    if (cd_done) *cd_done = fuse;
     if (gf_scale)
       *gf_scale = uu.implicit_gf ? (u.mode == 2 ? u.k1 + u.k2prev : u.k1)
                                  : 0.0f;
@@ -1097,15 +1101,17 @@ static int reverse_steps(const Plan& p, const Tape& tp, int b0, int b1,
       if (!tp.fs)
         PMX_TRY(launch_grad_mesh(p, tp.phi_in + (long long)(1 + i) * M, fm,
                                  4, s));
+      const float* xprev = i > b0 ? tp.xs + (slot - 1) * BN3 : cg.xb0;
+      const CoefDot cd{xprev, coef_scratch(nw), cg.out, i, cg.nsteps + i};
+      bool fused = false;
       PMX_TRY(force_vjp(p, tp.xs + slot * BN3, gF, gX, gP, N, f, fac, nw, gm,
-                        fm, u, s, nullptr, nullptr, gsc));
-      if (cg.out) {
-        const float* xprev = i > b0 ? tp.xs + (slot - 1) * BN3 : cg.xb0;
+                        fm, u, s, nullptr, nullptr, gsc,
+                        cg.out ? &cd : nullptr, &fused));
+      if (cg.out && !fused)
         PMX_TRY(launch_coef_dot(
             tp.xs + slot * BN3, nullptr, xprev, gX, gP,
             reinterpret_cast<const float4*>(fm), md, f, fac, u.d,
             coef_scratch(nw), cg.out, i, cg.nsteps + i, s));
-      }
       continue;
     }
     PMX_TRY(store_implicit_gf(gF, gP, gsc, BN3, s));

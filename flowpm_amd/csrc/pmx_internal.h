@@ -325,13 +325,25 @@ struct AdjUpdate {
   // VJP reads it as gin_scale * gP
   int implicit_gf;
 };
+// Step-coefficient cotangents taken in the same pass: out[ia] += a and
+// out[ib] += b (see launch_coef_dot, with Xd = X), via part (2 doubles per
+// 256 particles).
+struct CoefDot {
+  const float* xprev;
+  double* part;
+  double* out;
+  int ia, ib;
+};
 // m.f4 set: the density cotangent is m.f4[cell].w and rbar is unused.
 // gin: the cotangent of the readout (gF for the force; the position
 // cotangent itself for the PGD displacement); gF is the update target.
+// cd (optional): the coefficient dot products of the updated cotangents
+// (needs m.f4 and an update, mode 1 or 2).
 int launch_force_vjp_finish(float* gX, float* gP, float* gF, const float* gin,
                             const float* X, const Mesh3& m, const float* rbar,
                             const MeshDims& d, float f, float cf,
-                            const AdjUpdate& u, hipStream_t s);
+                            const AdjUpdate& u, hipStream_t s,
+                            const CoefDot* cd = nullptr);
 
 // scatter = 0 skips the mesh-cotangent atomics (done by the brick paint)
 int launch_force_vjp_particles(float* gmx, float* gmy, float* gmz,
