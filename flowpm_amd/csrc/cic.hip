@@ -271,11 +271,15 @@ __device__ __forceinline__ long long stage_off(bool tiled, long long p0,
   return 3 * (p0 + (long long)r * gz) + o;
 }
 
-template <int L, bool P2F>
+// CK: the step also writes the state between its two kicks (X, P after
+// the closing kick, F; the layout of a state (3, B, N, 3)) to c.ckpt, the
+// checkpoint a recomputing reverse sweep restarts from.
+template <int L, bool P2F, bool CK = false>
 __global__ __launch_bounds__(256) void k_readout_kick_drift(
     float* __restrict__ X, float* __restrict__ P, float* __restrict__ F,
     Mesh3 m, MeshDims d, KickDrift c) {
   __shared__ float sx[768], sp[768], sf[768];
+  __shared__ float sck[CK ? 768 : 1];
   const long long tot = (long long)d.B * d.N;
   const long long lb = xcd_block(blockIdx.x, gridDim.x);
   const bool tiled = c.gx > 0;  // the launcher checked the grid
@@ -303,6 +307,8 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
       for (int k = 0; k < 3; ++k) {
         sx[t + 256 * k] = vx[k];
         sp[t + 256 * k] = vp[k];
+        // the checkpoint's X is the force's position, as loaded
+        if (CK) __builtin_nontemporal_store(vx[k], c.ckpt + goff(t + 256 * k));
       }
     } else {
 #pragma unroll
@@ -315,6 +321,7 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
     for (int i = t; i < nf; i += 256) {
       sx[i] = __builtin_nontemporal_load(X + 3 * p0 + i);
       if (kick) sp[i] = __builtin_nontemporal_load(P + 3 * p0 + i);
+      if (CK) __builtin_nontemporal_store(sx[i], c.ckpt + 3 * p0 + i);
     }
   }
   __syncthreads();
@@ -351,6 +358,11 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
         vy = vy + c.k2 * Fy;
         vz = vz + c.k2 * Fz;
       }
+      if (CK) {
+        sck[3 * t] = vx;
+        sck[3 * t + 1] = vy;
+        sck[3 * t + 2] = vz;
+      }
       if (c.do_k1d) {
         vx = vx + c.k1 * Fx;
         vy = vy + c.k1 * Fy;
@@ -377,6 +389,10 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
       if (kick) __builtin_nontemporal_store(sp[i], P + o);
       if (c.do_k1d) __builtin_nontemporal_store(sx[i], Xo + o);
       if (c.do_k1d && c.xcopy) __builtin_nontemporal_store(sx[i], c.xcopy + o);
+      if (CK) {
+        __builtin_nontemporal_store(sck[i], c.ckpt + 3 * tot + o);
+        __builtin_nontemporal_store(sf[i], c.ckpt + 6 * tot + o);
+      }
     }
   } else {
     for (int i = t; i < nf; i += 256) {
@@ -385,6 +401,10 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
       if (c.do_k1d) __builtin_nontemporal_store(sx[i], Xo + 3 * p0 + i);
       if (c.do_k1d && c.xcopy)
         __builtin_nontemporal_store(sx[i], c.xcopy + 3 * p0 + i);
+      if (CK) {
+        __builtin_nontemporal_store(sck[i], c.ckpt + 3 * tot + 3 * p0 + i);
+        __builtin_nontemporal_store(sf[i], c.ckpt + 6 * tot + 3 * p0 + i);
+      }
     }
   }
 }
@@ -567,18 +587,22 @@ __global__ __launch_bounds__(256) void k_pgd_shift(float* __restrict__ X,
 }
 
 // Kick / drift from the stored force (no readout).
+// Xs, Ps: the state read; Xd, Pd: where the kicked / drifted state goes
+// (the same arrays in place; each element is read before it is written)
 __global__ __launch_bounds__(256) void k_kick_drift_only(
-    float* __restrict__ X, float* __restrict__ P, const float* __restrict__ F,
-    long long n3, KickDrift c) {
+    const float* Xs, const float* Ps, const float* __restrict__ F, float* Xd,
+    float* Pd, long long n3, KickDrift c) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n3) return;
-  float v = P[i];
+  float v = Ps[i];
   if (c.do_k2) v = v + c.k2 * F[i];
   if (c.do_k1d) {
     v = v + c.k1 * F[i];
-    X[i] = X[i] + c.d * v;
+    Xd[i] = Xs[i] + c.d * v;
+  } else if (Xd != Xs) {
+    Xd[i] = Xs[i];
   }
-  P[i] = v;
+  Pd[i] = v;
 }
 
 // Force VJP, particle side.  With F_d = c * readout(mesh_d, f X):
@@ -883,7 +907,13 @@ int launch_readout_kick_drift(float* X, float* P, float* F, const Mesh3& m,
   if (!grid_tiled(c.gx, c.gy, c.gz, d.N)) cc.gx = cc.gy = cc.gz = 0;
   int ex;
   const bool p2 = c.f > 0.0f && std::frexp(c.f, &ex) == 0.5f;
-  auto kern = m.f4 ? (p2 ? k_readout_kick_drift<kMeshF4, true>
+  if (c.ckpt)
+    PMX_REQUIRE(m.f3 && c.do_k2 && c.do_k1d && !c.xout && !c.xcopy,
+                "readout: a checkpoint is written from an untaped step with "
+                "both kicks and the packed force mesh");
+  auto kern = c.ckpt ? (p2 ? k_readout_kick_drift<kMeshF3, true, true>
+                           : k_readout_kick_drift<kMeshF3, false, true>)
+              : m.f4 ? (p2 ? k_readout_kick_drift<kMeshF4, true>
                          : k_readout_kick_drift<kMeshF4, false>)
               : m.f3 ? (p2 ? k_readout_kick_drift<kMeshF3, true>
                            : k_readout_kick_drift<kMeshF3, false>)
@@ -949,11 +979,17 @@ int launch_pgd_shift(float* X, const Mesh3& m, const MeshDims& d, float f,
 int launch_kick_drift_only(float* X, float* P, const float* F,
                            const MeshDims& d, const KickDrift& c,
                            hipStream_t s) {
+  return launch_kick_drift_from(X, P, F, X, P, d, c, s);
+}
+
+int launch_kick_drift_from(const float* Xs, const float* Ps, const float* F,
+                           float* Xd, float* Pd, const MeshDims& d,
+                           const KickDrift& c, hipStream_t s) {
   const long long n3 = (long long)d.B * d.N * 3;
   if (n3 == 0) return kOk;
   ProfScope prof_(kProfKickDrift, s);
   hipLaunchKernelGGL(k_kick_drift_only, dim3(nblk(n3, 256)), dim3(256), 0, s,
-                     X, P, F, n3, c);
+                     Xs, Ps, F, Xd, Pd, n3, c);
   PMX_LAUNCH_CHECK();
   return kOk;
 }

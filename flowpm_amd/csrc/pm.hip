@@ -717,10 +717,27 @@ struct Tape {
 
 // pgd (optional): (alpha, kl, ks) of every step; after the drift of step
 // i the positions move by the PGD displacement (tfpm.py:487-488).
+// src (optional, not with jump or pgd): the run starts from this state
+// (X, P, F) instead of the one in state, which it only writes: the first
+// kick and drift read src, so no copy of it is needed.
+// ck_out (optional, with ck_every >= 1; no PGD, no kept positions or force
+// meshes): the state after the closing kick of every step i with
+// (i + 1 - b0) % ck_every == 0 and i + 1 < b1 (the state a run ending at
+// step i + 1 would leave) goes to slot (i + 1 - b0) / ck_every - 1 of
+// ck_out, 3 B N 3 floats each, written by that step's readout.
 static int nbody_run(const Plan& p, float* state, long long N, float f,
                      float fac, const double* coef, int b0, int b1,
                      int jump, float* work, hipStream_t s,
-                     const Tape* tp = nullptr, const float* pgd = nullptr) {
+                     const Tape* tp = nullptr, const float* pgd = nullptr,
+                     const float* src = nullptr, float* ck_out = nullptr,
+                     int ck_every = 0) {
+  PMX_REQUIRE(!(src && (jump || pgd || b0 >= b1)),
+              "nbody: a run from a separate state starts with a kick and "
+              "drift (no jump start, no PGD, at least one step)");
+  PMX_REQUIRE(!(ck_out && (ck_every < 1 || pgd ||
+                           (tp && (tp->xs || tp->fs || tp->xp)))),
+              "nbody: checkpoints are written by untaped runs without PGD "
+              "(every >= 1)");
   PMX_REQUIRE(!(pgd && tp && (!tp->xp || !tp->fp)),
               "nbody: a PGD run keeps its pre-correction positions and PGD "
               "meshes in the tape");
@@ -805,7 +822,19 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     c.do_k1d = 1;
     c.k1 = (float)coef[3 * b0];
     c.d = (float)coef[3 * b0 + 1];
-    PMX_TRY(launch_kick_drift_only(X, P, F, md, c, s));
+    if (src) {
+      // the drifted positions go straight into their tape slot when the
+      // run keeps them (the state's F is written by the run's last step)
+      float* xd = xslot(b0);
+      PMX_TRY(launch_kick_drift_from(src, src + BN3, src + 2 * BN3,
+                                     xd ? xd : X, P, md, c, s));
+      if (xd) {
+        Xc = xd;
+        x_kept = true;
+      }
+    } else {
+      PMX_TRY(launch_kick_drift_only(X, P, F, md, c, s));
+    }
   }
   for (int i = b0; i < b1; ++i) {
     const long long slot = i - tape.first;
@@ -837,6 +866,9 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
     }
     // only the last force of the run has to stay in the state
     kd.skip_f = kd.do_k1d;
+    kd.ckpt = (ck_out && kd.do_k1d && (i + 1 - b0) % ck_every == 0)
+                  ? ck_out + (long long)((i + 1 - b0) / ck_every - 1) * 3 * BN3
+                  : nullptr;
     PMX_TRY(launch_readout_kick_drift(Xc, P, F, mesh_nc(nw.f4, nc), md,
                                       kd, s));
     if (kd.do_k1d && kd.xout) Xc = kd.xout;
@@ -1199,15 +1231,18 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
     Tape tp = segment_tape(p, N, every, work, b0);
     tp.phi_in = phi;
     if (!(last_taped && sg == nseg - 1)) {
-      {
+      const float* ck = ckpt + (long long)sg * 3 * BN3;
+      if (sg == 0 || b0 >= b1) {
+        // the jump start reads the state in place
         ProfScope prof_(kProfAdjUpdate, s);
-        PMX_HIP_CHECK(hipMemcpyAsync(st, ckpt + (long long)sg * 3 * BN3,
-                                     sizeof(float) * 3 * BN3,
+        PMX_HIP_CHECK(hipMemcpyAsync(st, ck, sizeof(float) * 3 * BN3,
                                      hipMemcpyDeviceToDevice, s));
       }
-      // recompute the segment, keeping X and the force meshes of every step
+      // recompute the segment, keeping X and the force meshes of every
+      // step (a later segment starts from its checkpoint directly)
       PMX_TRY(nbody_run(p, st, N, f, fac, coef, b0, b1, sg == 0, work, s,
-                        &tp));
+                        &tp, nullptr,
+                        (sg == 0 || b0 >= b1) ? nullptr : ck));
     }
     CoefGrad cg;
     cg.out = gcoef;
@@ -2249,6 +2284,24 @@ int pmx_nbody_run_phi(const pmx_plan* plan, float* state, long long N,
   tp.phi = phi;
   return nbody_run(P(plan), state, N, f, fac, coef, b0, b1, jump, work,
                    S(stream), &tp);
+}
+
+int pmx_nbody_run_checkpointed(const pmx_plan* plan, float* state,
+                               long long N, float f, float fac,
+                               const double* coef, int b0, int b1, int jump,
+                               int every, float* ckpt, float* work,
+                               float* phi, void* stream) {
+  PMX_REQUIRE(plan && state && work && ckpt,
+              "pmx_nbody_run_checkpointed: null pointer");
+  PMX_REQUIRE(N >= 1, "pmx_nbody_run_checkpointed: need at least one "
+              "particle");
+  PMX_REQUIRE(b0 >= 0 && b1 >= b0 && (coef || b1 == b0) && every >= 1,
+              "pmx_nbody_run_checkpointed: bad step range or every < 1");
+  Tape tp;
+  tp.phi = phi;
+  return nbody_run(P(plan), state, N, f, fac, coef, b0, b1, jump, work,
+                   S(stream), phi ? &tp : nullptr, nullptr, nullptr, ckpt,
+                   every);
 }
 
 int pmx_nbody_run_last_segment_phi(const pmx_plan* plan, float* state,

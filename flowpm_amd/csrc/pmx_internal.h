@@ -249,6 +249,10 @@ struct KickDrift {
   int gx, gy, gz;
   float* xout;    // if set (with do_k1d): drifted X stored here, not in X
   float* xcopy;   // if set (with do_k1d): drifted X also stored here
+  // if set (both kicks, no xout / xcopy, packed mesh): the state between
+  // the two kicks (X, P after the closing kick, F), laid out as a state
+  // (3, B, N, 3), is also stored here (a checkpoint)
+  float* ckpt;
 };
 // The three force meshes: separate (x, y, z), interleaved float4 (f4) or
 // interleaved packed (x, y, z) floats (f3).
@@ -285,6 +289,11 @@ int launch_pgd_shift(float* X, const Mesh3& m, const MeshDims& d, float f,
 int launch_kick_drift_only(float* X, float* P, const float* F,
                            const MeshDims& d, const KickDrift& c,
                            hipStream_t s);
+// the same from the state (Xs, Ps, F) into (Xd, Pd): a copy of the state
+// and the kick / drift in one pass (either pair may be the same arrays)
+int launch_kick_drift_from(const float* Xs, const float* Ps, const float* F,
+                           float* Xd, float* Pd, const MeshDims& d,
+                           const KickDrift& c, hipStream_t s);
 
 // Philox4x64-10 Gaussian white noise (noise.hip).
 int launch_white_noise(float* out, long long n, unsigned long long k0,

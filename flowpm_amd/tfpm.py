@@ -920,8 +920,19 @@ class _Nbody(torch.autograd.Function):
     if _keep_phi(plan, nsteps, st.device):
       phi = torch.empty(_lib.lib().pmx_nbody_phi_floats(plan.handle, nsteps),
                         dtype=torch.float32, device=st.device)
-    for s in range(nseg):
-      ckpt[s].copy_(st)
+    ckpt[0].copy_(st)
+    s0 = 0
+    if nseg >= 3:
+      # every segment but the last in one run, whose readouts write the
+      # checkpoints they cross (no copies, no split kicks)
+      _lib.call("pmx_nbody_run_checkpointed", plan.handle, _lib.ptr(st), N,
+                f, fac, cptr, 0, (nseg - 1) * every, 1, every,
+                _lib.ptr(ckpt[1]), _lib.ptr(work),
+                None if phi is None else _lib.ptr(phi), stream)
+      s0 = nseg - 1
+    for s in range(s0, nseg):
+      if s > 0:
+        ckpt[s].copy_(st)
       b0, b1 = s * every, min(nsteps, (s + 1) * every)
       if s < nseg - 1 or nsteps == 0:
         if phi is None:

@@ -406,6 +406,21 @@ int pmx_nbody_adjoint_last_taped(const pmx_plan* plan, const float* ckpt,
  * fourth-order gradient) instead of painting and transforming: the same
  * gradients for about half the recomputation time.  No PGD. */
 long long pmx_nbody_phi_floats(const pmx_plan* plan, int nsteps);
+/* pmx_nbody_run (phi NULL) or pmx_nbody_run_phi (phi set) that also writes
+ * the checkpoints of the steps it crosses: after step i with
+ * (i + 1 - step_begin) % every == 0 and i + 1 < step_end, the state a run
+ * ending there would leave (X, P after the closing kick, F; 3*B*N*3
+ * floats) goes to slot (i + 1 - step_begin) / every - 1 of ckpt.  The
+ * same states as separate pmx_nbody_run calls per segment with copies
+ * between them (bit for bit), without the copies and the split kicks: the
+ * readout of each such step stores them.  (No reference counterpart: TF
+ * keeps every forward intermediate, tfpm.py:425-503.)  Not with PGD. */
+int pmx_nbody_run_checkpointed(const pmx_plan* plan, float* state,
+                               long long N, float f, float fac,
+                               const double* coef, int step_begin,
+                               int step_end, int jump_start, int every,
+                               float* ckpt, float* work, float* phi,
+                               void* stream);
 int pmx_nbody_run_phi(const pmx_plan* plan, float* state, long long N,
                       float f, float fac, const double* coef, int step_begin,
                       int step_end, int jump_start, float* work, float* phi,

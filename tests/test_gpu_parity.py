@@ -546,6 +546,58 @@ def test_checkpointed_adjoint_with_and_without_kept_potentials(gpu, every, B):
           (every, B, keep, e1, e2))
 
 
+@pytest.mark.parametrize("keep", [False, True])
+def test_checkpointed_run_writes_the_segment_states(gpu, keep):
+  """pmx_nbody_run_checkpointed (the checkpoints written by the readout of
+  each boundary step, no split kick) against pmx_nbody_run per segment with
+  the state copied between segments: the same checkpoints and final state,
+  bit for bit, with and without the kept potentials (7 steps, every 2:
+  checkpoints after steps 2, 4 and 6)."""
+  from flowpm_amd import _lib
+  n, L, B = 32, 96.0, 1
+  white, pk = _ics(n, L, 31, B=B)
+  c = fa.cosmology.Planck15()
+  stages = np.linspace(0.1, 1.0, 8)
+  lin = fa.whitenoise_to_linear(n, L, white.float().to(gpu), pk)
+  st0 = fa.lpt_init(c, lin, 0.1, order=2).contiguous()
+  plan, B, N, f, fac = fa.tfpm._force_setup(c, st0, n, 1)
+  coef = fa.tfpm.step_coefficients(c, stages)
+  cptr = _lib.dbl(coef)
+  nsteps, every = len(coef) // 3, 2
+  L_ = _lib.lib()
+  work = torch.empty(L_.pmx_nbody_work_floats(plan.handle),
+                     dtype=torch.float32, device=gpu)
+  phi_n = L_.pmx_nbody_phi_floats(plan.handle, nsteps)
+  phis = [torch.zeros(phi_n, device=gpu) if keep else None for _ in range(2)]
+  pp = [None if q is None else _lib.ptr(q) for q in phis]
+  stream = _lib.stream_of(st0)
+  # reference: one call per segment, the state copied at each boundary
+  st = st0.clone()
+  ref = []
+  for b0 in range(0, nsteps, every):
+    if b0 > 0:
+      ref.append(st.clone())
+    b1 = min(nsteps, b0 + every)
+    if keep:
+      _lib.call("pmx_nbody_run_phi", plan.handle, _lib.ptr(st), N, f, fac,
+                cptr, b0, b1, 1 if b0 == 0 else 0, _lib.ptr(work), pp[0],
+                stream)
+    else:
+      _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
+                b0, b1, 1 if b0 == 0 else 0, _lib.ptr(work), stream)
+  st2 = st0.clone()
+  ck = torch.full((len(ref),) + tuple(st0.shape), float("nan"), device=gpu)
+  _lib.call("pmx_nbody_run_checkpointed", plan.handle, _lib.ptr(st2), N, f,
+            fac, cptr, 0, nsteps, 1, every, _lib.ptr(ck), _lib.ptr(work),
+            pp[1], stream)
+  torch.cuda.synchronize()
+  for j, r in enumerate(ref):
+    assert torch.equal(ck[j], r), "checkpoint %d differs" % j
+  assert torch.equal(st2, st), "final state differs"
+  if keep:
+    assert torch.equal(phis[0], phis[1]), "kept potentials differ"
+
+
 @pytest.mark.parametrize("B", [1, 2])
 def test_compact_tape_matches_full_tape(gpu, B):
   """The compact tape (positions and potential of every force; the reverse
