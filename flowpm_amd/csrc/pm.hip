@@ -713,6 +713,9 @@ struct Tape {
   // a segment whose potentials the forward run kept.
   float* phi = nullptr;
   const float* phi_in = nullptr;
+  // the caller reads the run's last positions from their slot: the state's
+  // X is left as it is (a reverse sweep's scratch state)
+  bool x_from_tape = false;
 };
 
 // pgd (optional): (alpha, kl, ks) of every step; after the drift of step
@@ -873,7 +876,7 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
                                       kd, s));
     if (kd.do_k1d && kd.xout) Xc = kd.xout;
   }
-  if (Xc != X) {
+  if (Xc != X && !tape.x_from_tape) {
     ProfScope prof_(kProfAdjUpdate, s);
     PMX_HIP_CHECK(hipMemcpyAsync(X, Xc, sizeof(float) * BN3,
                                  hipMemcpyDeviceToDevice, s));
@@ -1230,6 +1233,7 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
     const int b0 = sg * every, b1 = std::min(nsteps, b0 + every);
     Tape tp = segment_tape(p, N, every, work, b0);
     tp.phi_in = phi;
+    tp.x_from_tape = true;  // the sweep reads every position from the tape
     if (!(last_taped && sg == nseg - 1)) {
       const float* ck = ckpt + (long long)sg * 3 * BN3;
       if (sg == 0 || b0 >= b1) {
