@@ -105,8 +105,8 @@ _SIGS = {
                                _vp, _vp]),
     "pmx_nbody_run_last_segment_phi": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i,
                                             _i, _vp, _vp, _vp]),
-    "pmx_nbody_run_checkpointed": (_i, [_vp, _vp, _ll, _f, _f, _dp, _i, _i,
-                                        _i, _i, _vp, _vp, _vp, _vp]),
+    "pmx_nbody_run_checkpointed": (_i, [_vp, _vp, _vp, _ll, _f, _f, _dp, _i,
+                                        _i, _i, _i, _vp, _vp, _vp, _vp]),
     "pmx_nbody_adjoint_phi": (_i, [_vp, _vp, _vp, _i, _i, _ll, _f, _f, _dp,
                                    _vp, _vp, _i, _vp, _vp]),
     "pmx_nbody_batched_step": (_i, [_i, _c.POINTER(_vp), _c.POINTER(_vp),

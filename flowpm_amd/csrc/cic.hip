@@ -289,6 +289,8 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
   const int nf = 3 * np;
   const bool kick = c.do_k2 || c.do_k1d;
   float* const Xo = c.xout ? c.xout : X;  // where the drifted X goes
+  const float* const Xl = c.xin ? c.xin : X;  // where X and P are read
+  const float* const Pl = c.pin ? c.pin : P;
   const int t = threadIdx.x;
   auto goff = [&](int i) { return stage_off(tiled, p0, c.gz, i); };
   // particle state is streamed with non-temporal loads / stores so it does
@@ -300,8 +302,8 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const long long o = goff(t + 256 * k);
-        vx[k] = __builtin_nontemporal_load(X + o);
-        vp[k] = __builtin_nontemporal_load(P + o);
+        vx[k] = __builtin_nontemporal_load(Xl + o);
+        vp[k] = __builtin_nontemporal_load(Pl + o);
       }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -313,14 +315,14 @@ __global__ __launch_bounds__(256) void k_readout_kick_drift(
     } else {
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        vx[k] = __builtin_nontemporal_load(X + goff(t + 256 * k));
+        vx[k] = __builtin_nontemporal_load(Xl + goff(t + 256 * k));
 #pragma unroll
       for (int k = 0; k < 3; ++k) sx[t + 256 * k] = vx[k];
     }
   } else {
     for (int i = t; i < nf; i += 256) {
-      sx[i] = __builtin_nontemporal_load(X + 3 * p0 + i);
-      if (kick) sp[i] = __builtin_nontemporal_load(P + 3 * p0 + i);
+      sx[i] = __builtin_nontemporal_load(Xl + 3 * p0 + i);
+      if (kick) sp[i] = __builtin_nontemporal_load(Pl + 3 * p0 + i);
       if (CK) __builtin_nontemporal_store(sx[i], c.ckpt + 3 * p0 + i);
     }
   }

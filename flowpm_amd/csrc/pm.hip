@@ -720,9 +720,10 @@ struct Tape {
 
 // pgd (optional): (alpha, kl, ks) of every step; after the drift of step
 // i the positions move by the PGD displacement (tfpm.py:487-488).
-// src (optional, not with jump or pgd): the run starts from this state
-// (X, P, F) instead of the one in state, which it only writes: the first
-// kick and drift read src, so no copy of it is needed.
+// src (optional, not with pgd, at least one step): the run starts from
+// this state (X, P, F) instead of the one in state, which it only writes:
+// the jump-start force and readout, or the first kick and drift, read src,
+// so no copy of it is needed.
 // ck_out (optional, with ck_every >= 1; no PGD, no kept positions or force
 // meshes): the state after the closing kick of every step i with
 // (i + 1 - b0) % ck_every == 0 and i + 1 < b1 (the state a run ending at
@@ -734,9 +735,9 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
                      const Tape* tp = nullptr, const float* pgd = nullptr,
                      const float* src = nullptr, float* ck_out = nullptr,
                      int ck_every = 0) {
-  PMX_REQUIRE(!(src && (jump || pgd || b0 >= b1)),
-              "nbody: a run from a separate state starts with a kick and "
-              "drift (no jump start, no PGD, at least one step)");
+  PMX_REQUIRE(!(src && (pgd || b0 >= b1)),
+              "nbody: a run from a separate state needs at least one step "
+              "and no PGD");
   PMX_REQUIRE(!(ck_out && (ck_every < 1 || pgd ||
                            (tp && (tp->xs || tp->fs || tp->xp)))),
               "nbody: checkpoints are written by untaped runs without PGD "
@@ -805,10 +806,11 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
   // density cotangent in its fourth slot); one only the readout reads is
   // packed, 12 bytes per cell
   if (jump) {
-    PMX_TRY(keep_x(tape.xj, X));
+    const float* Xj = src ? src : X;  // the jump-start force's positions
+    PMX_TRY(keep_x(tape.xj, Xj));
     const NbodyWork nw = at(tape.fj);
     const int nc = tape.fj ? 4 : 3;
-    PMX_TRY(force_k(0, X, nw, nc));
+    PMX_TRY(force_k(0, Xj, nw, nc));
     kd.do_k2 = 0;
     kd.do_k1d = (b0 < b1);
     if (kd.do_k1d) {
@@ -816,8 +818,11 @@ static int nbody_run(const Plan& p, float* state, long long N, float f,
       kd.d = (float)coef[3 * b0 + 1];
       drift_to(b0);
     }
+    kd.xin = src;
+    kd.pin = src ? src + BN3 : nullptr;
     PMX_TRY(launch_readout_kick_drift(X, P, F, mesh_nc(nw.f4, nc), md,
                                       kd, s));
+    kd.xin = kd.pin = nullptr;
     if (kd.do_k1d && kd.xout) Xc = kd.xout;
   } else if (b0 < b1) {
     // opening kick + drift of the first step from the stored force
@@ -1236,17 +1241,16 @@ static int nbody_adjoint(const Plan& p, const float* ckpt, int every,
     tp.x_from_tape = true;  // the sweep reads every position from the tape
     if (!(last_taped && sg == nseg - 1)) {
       const float* ck = ckpt + (long long)sg * 3 * BN3;
-      if (sg == 0 || b0 >= b1) {
-        // the jump start reads the state in place
+      if (b0 >= b1) {
+        // no step: the jump start reads the state in place
         ProfScope prof_(kProfAdjUpdate, s);
         PMX_HIP_CHECK(hipMemcpyAsync(st, ck, sizeof(float) * 3 * BN3,
                                      hipMemcpyDeviceToDevice, s));
       }
-      // recompute the segment, keeping X and the force meshes of every
-      // step (a later segment starts from its checkpoint directly)
+      // recompute the segment from its checkpoint (read in place), keeping
+      // X and the force meshes of every step
       PMX_TRY(nbody_run(p, st, N, f, fac, coef, b0, b1, sg == 0, work, s,
-                        &tp, nullptr,
-                        (sg == 0 || b0 >= b1) ? nullptr : ck));
+                        &tp, nullptr, b0 >= b1 ? nullptr : ck));
     }
     CoefGrad cg;
     cg.out = gcoef;
@@ -2291,9 +2295,9 @@ int pmx_nbody_run_phi(const pmx_plan* plan, float* state, long long N,
 }
 
 int pmx_nbody_run_checkpointed(const pmx_plan* plan, float* state,
-                               long long N, float f, float fac,
-                               const double* coef, int b0, int b1, int jump,
-                               int every, float* ckpt, float* work,
+                               const float* src, long long N, float f,
+                               float fac, const double* coef, int b0, int b1,
+                               int jump, int every, float* ckpt, float* work,
                                float* phi, void* stream) {
   PMX_REQUIRE(plan && state && work && ckpt,
               "pmx_nbody_run_checkpointed: null pointer");
@@ -2301,10 +2305,13 @@ int pmx_nbody_run_checkpointed(const pmx_plan* plan, float* state,
               "particle");
   PMX_REQUIRE(b0 >= 0 && b1 >= b0 && (coef || b1 == b0) && every >= 1,
               "pmx_nbody_run_checkpointed: bad step range or every < 1");
+  PMX_REQUIRE(!src || b1 > b0,
+              "pmx_nbody_run_checkpointed: a separate initial state needs "
+              "at least one step");
   Tape tp;
   tp.phi = phi;
   return nbody_run(P(plan), state, N, f, fac, coef, b0, b1, jump, work,
-                   S(stream), phi ? &tp : nullptr, nullptr, nullptr, ckpt,
+                   S(stream), phi ? &tp : nullptr, nullptr, src, ckpt,
                    every);
 }
 

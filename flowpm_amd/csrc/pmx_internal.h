@@ -253,6 +253,10 @@ struct KickDrift {
   // the two kicks (X, P after the closing kick, F), laid out as a state
   // (3, B, N, 3), is also stored here (a checkpoint)
   float* ckpt;
+  // if set: X / P are read from here instead (the results still go to X
+  // or xout and to P)
+  const float* xin;
+  const float* pin;
 };
 // The three force meshes: separate (x, y, z), interleaved float4 (f4) or
 // interleaved packed (x, y, z) floats (f3).

@@ -869,34 +869,37 @@ class _Nbody(torch.autograd.Function):
               cpar=None):
     B = plan.B
     nsteps = len(coef) // 3
-    st = _f32(state).clone()
+    x0 = _f32(state)  # read only
     work = _work(plan)
     cptr = _lib.dbl(coef)
-    stream = _lib.stream_of(st)
+    stream = _lib.stream_of(x0)
     if not need_grad:
+      st = x0.clone()
       _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
                 0, nsteps, 1, _lib.ptr(work), stream)
       return st
     ctx.want_c = len(ctx.needs_input_grad) > 8 and ctx.needs_input_grad[8]
-    st = _Nbody._run_for_grad(ctx, st, plan, N, f, fac, coef, every, work,
+    st = _Nbody._run_for_grad(ctx, x0, plan, N, f, fac, coef, every, work,
                               cptr, stream)
     if ctx.want_c:
       ctx.f_final = st[2].clone()
     return st
 
   @staticmethod
-  def _run_for_grad(ctx, st, plan, N, f, fac, coef, every, work, cptr,
+  def _run_for_grad(ctx, x0, plan, N, f, fac, coef, every, work, cptr,
                     stream):
+    """x0: the initial state (not modified); returns the final state."""
     B = plan.B
     nsteps = len(coef) // 3
     if every == 0:
+      st = x0.clone()
       # the full tape (X and the float4 force mesh of every force), or the
       # compact one (X and the potential)
-      ctx.compact = _compact(plan, N, nsteps, st.device)
+      ctx.compact = _compact(plan, N, nsteps, x0.device)
       kind = "compact_" if ctx.compact else ""
       tape = torch.empty(getattr(_lib.lib(), "pmx_nbody_%stape_floats" %
                                  kind)(plan.handle, N, nsteps),
-                         dtype=torch.float32, device=st.device)
+                         dtype=torch.float32, device=x0.device)
       _lib.call("pmx_nbody_run_%staped" % kind, plan.handle, _lib.ptr(st), N,
                 f, fac, cptr, nsteps, _lib.ptr(tape), _lib.ptr(work), stream)
       # released with the graph; the reverse sweep rewrites only the fourth
@@ -907,29 +910,33 @@ class _Nbody(torch.autograd.Function):
       return st
     nseg = max(1, (nsteps + every - 1) // every)
     ckpt = torch.empty((nseg, 3, B, N, 3), dtype=torch.float32,
-                       device=st.device)
+                       device=x0.device)
     # the reverse sweep's workspace holds the recomputation tape of one
     # segment; the last segment is run into it here, so the backward pass
     # recomputes the other segments only
     k = min(every, max(nsteps, 1))
     adj = torch.empty(_lib.lib().pmx_nbody_adj_work_floats(plan.handle, N, k),
-                      dtype=torch.float32, device=st.device)
+                      dtype=torch.float32, device=x0.device)
     # the potential of every force evaluation, when kept: the backward pass
     # then recomputes a segment's force meshes with the gradient stencil
     phi = None
-    if _keep_phi(plan, nsteps, st.device):
+    if _keep_phi(plan, nsteps, x0.device):
       phi = torch.empty(_lib.lib().pmx_nbody_phi_floats(plan.handle, nsteps),
-                        dtype=torch.float32, device=st.device)
-    ckpt[0].copy_(st)
+                        dtype=torch.float32, device=x0.device)
+    ckpt[0].copy_(x0)
     s0 = 0
     if nseg >= 3:
-      # every segment but the last in one run, whose readouts write the
-      # checkpoints they cross (no copies, no split kicks)
-      _lib.call("pmx_nbody_run_checkpointed", plan.handle, _lib.ptr(st), N,
-                f, fac, cptr, 0, (nseg - 1) * every, 1, every,
-                _lib.ptr(ckpt[1]), _lib.ptr(work),
+      # every segment but the last in one run from the first checkpoint,
+      # whose readouts write the checkpoints they cross (no copies, no split
+      # kicks)
+      st = torch.empty_like(x0)
+      _lib.call("pmx_nbody_run_checkpointed", plan.handle, _lib.ptr(st),
+                _lib.ptr(ckpt[0]), N, f, fac, cptr, 0, (nseg - 1) * every, 1,
+                every, _lib.ptr(ckpt[1]), _lib.ptr(work),
                 None if phi is None else _lib.ptr(phi), stream)
       s0 = nseg - 1
+    else:
+      st = x0.clone()
     for s in range(s0, nseg):
       if s > 0:
         ckpt[s].copy_(st)

@@ -413,11 +413,14 @@ long long pmx_nbody_phi_floats(const pmx_plan* plan, int nsteps);
  * floats) goes to slot (i + 1 - step_begin) / every - 1 of ckpt.  The
  * same states as separate pmx_nbody_run calls per segment with copies
  * between them (bit for bit), without the copies and the split kicks: the
- * readout of each such step stores them.  (No reference counterpart: TF
- * keeps every forward intermediate, tfpm.py:425-503.)  Not with PGD. */
+ * readout of each such step stores them.  src (NULL: the state in place):
+ * the initial state is read from src (not modified; at least one step) and
+ * state only receives the result, so the caller's copy of the initial
+ * state can serve as both.  (No reference counterpart: TF keeps every
+ * forward intermediate, tfpm.py:425-503.)  Not with PGD. */
 int pmx_nbody_run_checkpointed(const pmx_plan* plan, float* state,
-                               long long N, float f, float fac,
-                               const double* coef, int step_begin,
+                               const float* src, long long N, float f,
+                               float fac, const double* coef, int step_begin,
                                int step_end, int jump_start, int every,
                                float* ckpt, float* work, float* phi,
                                void* stream);

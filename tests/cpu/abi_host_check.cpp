@@ -193,8 +193,8 @@ int main() {
                           nullptr));
   FAILS(pmx_nbody_run_last_segment_phi(np, fp, 8, 1.0f, 0.4f, dbuf, 2, 1, fp,
                                        fp, nullptr));
-  FAILS(pmx_nbody_run_checkpointed(np, fp, 8, 1.0f, 0.4f, dbuf, 0, 2, 1, 1, fp,
-                                   fp, nullptr, nullptr));
+  FAILS(pmx_nbody_run_checkpointed(np, fp, nullptr, 8, 1.0f, 0.4f, dbuf, 0, 2,
+                                   1, 1, fp, fp, nullptr, nullptr));
   FAILS(pmx_nbody_adjoint_phi(np, cfp, cfp, 1, 2, 8, 1.0f, 0.4f, dbuf, fp, fp,
                               1, dbuf, nullptr));
   FAILS(pmx_nbody_run_taped(np, fp, 8, 1.0f, 0.4f, dbuf, 1, fp, fp, nullptr));

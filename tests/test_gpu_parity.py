@@ -551,8 +551,9 @@ def test_checkpointed_run_writes_the_segment_states(gpu, keep):
   """pmx_nbody_run_checkpointed (the checkpoints written by the readout of
   each boundary step, no split kick) against pmx_nbody_run per segment with
   the state copied between segments: the same checkpoints and final state,
-  bit for bit, with and without the kept potentials (7 steps, every 2:
-  checkpoints after steps 2, 4 and 6)."""
+  bit for bit, with and without the kept potentials, in place and from a
+  separate initial state (7 steps, every 2: checkpoints after steps 2, 4
+  and 6)."""
   from flowpm_amd import _lib
   n, L, B = 32, 96.0, 1
   white, pk = _ics(n, L, 31, B=B)
@@ -585,17 +586,25 @@ def test_checkpointed_run_writes_the_segment_states(gpu, keep):
     else:
       _lib.call("pmx_nbody_run", plan.handle, _lib.ptr(st), N, f, fac, cptr,
                 b0, b1, 1 if b0 == 0 else 0, _lib.ptr(work), stream)
-  st2 = st0.clone()
-  ck = torch.full((len(ref),) + tuple(st0.shape), float("nan"), device=gpu)
-  _lib.call("pmx_nbody_run_checkpointed", plan.handle, _lib.ptr(st2), N, f,
-            fac, cptr, 0, nsteps, 1, every, _lib.ptr(ck), _lib.ptr(work),
-            pp[1], stream)
-  torch.cuda.synchronize()
-  for j, r in enumerate(ref):
-    assert torch.equal(ck[j], r), "checkpoint %d differs" % j
-  assert torch.equal(st2, st), "final state differs"
-  if keep:
-    assert torch.equal(phis[0], phis[1]), "kept potentials differ"
+  for from_src in (False, True):
+    # in place, or reading the initial state from a separate array (which
+    # must stay as it was)
+    st2 = torch.full_like(st0, float("nan")) if from_src else st0.clone()
+    src = st0.clone()
+    ck = torch.full((len(ref),) + tuple(st0.shape), float("nan"), device=gpu)
+    if keep:
+      phis[1].zero_()
+    _lib.call("pmx_nbody_run_checkpointed", plan.handle, _lib.ptr(st2),
+              _lib.ptr(src) if from_src else None, N, f, fac, cptr, 0,
+              nsteps, 1, every, _lib.ptr(ck), _lib.ptr(work), pp[1], stream)
+    torch.cuda.synchronize()
+    for j, r in enumerate(ref):
+      assert torch.equal(ck[j], r), "checkpoint %d differs (src %s)" % (
+          j, from_src)
+    assert torch.equal(st2, st), "final state differs (src %s)" % from_src
+    assert torch.equal(src, st0), "the initial state was modified"
+    if keep:
+      assert torch.equal(phis[0], phis[1]), "kept potentials differ"
 
 
 @pytest.mark.parametrize("B", [1, 2])
